@@ -1,0 +1,149 @@
+/*
+ * kp_hip.h - C ABI of the MI355X-native kazen_proto hot path.
+ *
+ * Plain C types only (no torch, no HIP types in the signatures). Every call
+ * returns a status (KP_OK = 0); on failure kp_last_error() describes it. The
+ * library never exits the process and never throws across this boundary
+ * (unlike the reference's CUDA_CHECK/OPTIX_CHECK -> exit(1),
+ * src/core/optix_utils.h:16-58).
+ *
+ * Ownership: caller memory stays the caller's; host arrays are copied during
+ * the call. Device memory created by the library belongs to the kp_ctx.
+ * Threading: one kp_ctx issues work on one HIP stream; calls on one context
+ * must be serialised by the caller (the reference serialises scene building
+ * and calls intersect concurrently only through const CPU code paths).
+ *
+ * Pointer kinds: KP_PTR_HOST arrays are staged through the context;
+ * KP_PTR_DEVICE arrays must be device memory of the context's device and the
+ * call is asynchronous on the context stream (errors detected by the kernels
+ * are reported by the next kp_synchronize).
+ */
+#ifndef KP_HIP_H
+#define KP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KP_OK 0
+#define KP_ERR_INVALID 1     /* bad argument */
+#define KP_ERR_STATE 2       /* call out of order (e.g. trace before build) */
+#define KP_ERR_HIP 3         /* HIP runtime error */
+#define KP_ERR_ALLOC 4       /* device allocation failed */
+#define KP_ERR_KERNEL 5      /* a kernel reported an internal limit */
+
+#define KP_PTR_HOST 0
+#define KP_PTR_DEVICE 1
+
+typedef struct kp_ctx kp_ctx;
+
+/* Ray: origin, tmin, direction, tmax (32 bytes). Directions need not be
+ * normalised; t is measured in units of |d|. Valid hits have t in
+ * [tmin, tmax]. Mirrors Ray (reference src/core/ray.h:10-45) without the
+ * differentials and time. */
+typedef struct {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} kp_ray;
+
+/* Closest hit: prim = caller primitive index (global across meshes, in the
+ * order meshes were added) or 0xFFFFFFFF on a miss (t = +inf). u, v are the
+ * barycentric weights of vertices 1 and 2 (Embree's convention, used by the
+ * reference at src/core/accel.cpp:393-394). 16 bytes. */
+typedef struct {
+    float t;
+    uint32_t prim;
+    float u, v;
+} kp_hit;
+
+typedef struct {
+    uint32_t leaf_size;   /* max triangles per leaf, 1..4 (default 4) */
+    uint32_t sah_bins;    /* SAH bins, 4..64 (default 32) */
+    uint32_t reserved[6];
+} kp_build_opts;
+
+typedef struct {
+    uint64_t n_tris;
+    uint64_t n_nodes;          /* 128-byte BVH8 nodes */
+    uint64_t n_bvh2_nodes;
+    uint64_t bvh_bytes;        /* nodes + triangle records on the device */
+    uint32_t max_depth;        /* BVH8 levels */
+    uint32_t spill_entries;    /* per-lane stack spill capacity */
+    uint32_t grid_closest;     /* persistent blocks per launch */
+    uint32_t grid_occluded;
+    double build_seconds;      /* host BVH build wall time */
+    uint32_t device_cus;
+    uint32_t reserved[7];
+} kp_stats;
+
+/* Context ---------------------------------------------------------------- */
+/* Creates a context on HIP device `device`. Replaces the accelerator
+ * construction of Scene::Scene / set_accelerator (src/core/scene.cpp:281,
+ * src/core/scene.h:60-77). */
+int kp_create(kp_ctx** out, int device);
+void kp_destroy(kp_ctx* ctx);
+/* Last error text of the context (or of the last failed kp_create when ctx is
+ * NULL). Valid until the next call on the context. */
+const char* kp_last_error(const kp_ctx* ctx);
+/* Issue all further work on `hip_stream` (a hipStream_t, NULL = the context's
+ * own stream). */
+int kp_set_stream(kp_ctx* ctx, void* hip_stream);
+/* Waits for the context's stream and reports kernel-detected errors. */
+int kp_synchronize(kp_ctx* ctx);
+
+/* Scene ------------------------------------------------------------------ */
+/* Adds a triangle mesh (copied). xform: optional row-major 3x4 object->world
+ * matrix (NULL = identity), applied on the host. The mesh's primitives get
+ * global ids [first, first + n_tris) where first = triangles added before.
+ * Replaces Accelerator::add_trianglemesh (src/core/accel.h:26,
+ * EmbreeAccel::add_trianglemesh src/core/accel.cpp:258-315) together with
+ * Scene::add_mesh (src/core/scene.h:144-160). */
+int kp_mesh_add(kp_ctx* ctx, const float* xyz, uint32_t n_verts, const uint32_t* idx,
+                uint32_t n_tris, const float* xform12, uint32_t* mesh_id);
+/* Builds the BVH over every mesh added so far and uploads it. Replaces
+ * Accelerator::build (src/core/accel.h:32; EmbreeAccel::build accel.h:75-77,
+ * add_instances accel.cpp:317-346) and Scene::build_bvh (scene.h:162-164). */
+int kp_bvh_build(kp_ctx* ctx, const kp_build_opts* opts);
+int kp_get_stats(kp_ctx* ctx, kp_stats* out);
+
+/* Queries ---------------------------------------------------------------- */
+/* Closest hit for n rays. Replaces HitableInterface::intersect(const Ray&,
+ * Intersection&) / intersect(const Ray&, float&) (src/core/hitable.h:12-13)
+ * as implemented by EmbreeAccel::intersect (src/core/accel.cpp:348-432),
+ * batched. */
+int kp_intersect_closest(kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hits,
+                         int ptr_kind);
+/* Occlusion for n rays: out[i] = 1 if any primitive is hit with t in
+ * [tmin, tmax]. Replaces HitableInterface::occluded (src/core/hitable.h:15-33),
+ * batched; 4 bytes per result (uint32). */
+int kp_occluded(kp_ctx* ctx, const kp_ray* rays, uint32_t n, uint32_t* out, int ptr_kind);
+/* Instrumented closest hit: same results, plus the totals of BVH8 nodes
+ * visited and triangles tested (for the bytes-per-ray roofline figure). */
+int kp_intersect_closest_counted(kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hits,
+                                 int ptr_kind, uint64_t* nodes_visited,
+                                 uint64_t* tris_tested);
+/* Host-only BVH8 build (no device needed): builds the same 8-wide BVH that
+ * kp_bvh_build uploads and hands back its node and triangle records, for
+ * inspection, offline builds and host-side checks. tris9: 9 floats per
+ * triangle (world space). */
+typedef struct kp_bvh8_blob kp_bvh8_blob;
+int kp_bvh8_build_host(const float* tris9, uint32_t n_tris, const kp_build_opts* opts,
+                       kp_bvh8_blob** out);
+/* n_nodes 128-byte nodes, n_tris 48-byte triangle records. */
+int kp_bvh8_blob_info(const kp_bvh8_blob* b, uint64_t* n_nodes, uint64_t* n_tris,
+                      uint32_t* max_depth);
+const void* kp_bvh8_blob_nodes(const kp_bvh8_blob* b);
+const void* kp_bvh8_blob_tris(const kp_bvh8_blob* b);
+void kp_bvh8_blob_free(kp_bvh8_blob* b);
+
+/* Tuning: refill a wave once fewer than `lanes` (1..64) of its lanes are
+ * busy (default 40). */
+int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KP_HIP_H */

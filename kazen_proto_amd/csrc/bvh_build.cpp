@@ -1,0 +1,315 @@
+// Host BVH builder of the HIP intersector: binned-SAH BVH2, collapsed into
+// 8-wide nodes with 16-bit quantised child boxes (layout in kp_layout.h).
+//
+// Replaces the acceleration-structure build behind Accelerator::build /
+// add_trianglemesh (reference src/core/accel.h:26-32; Embree build at
+// src/core/accel.cpp:185-346, in-tree median-split BVH at accel.cpp:99-132).
+#include "kp_bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace kp {
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    void grow_pt(const float* p) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    double half_area() const {
+        double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+        if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.0;
+        return dx * dy + dy * dz + dz * dx;
+    }
+};
+
+struct N2 {
+    Box box;
+    uint32_t a = 0, b = 0;    // inner: children; leaf: first, count
+    bool leaf = false;
+};
+
+constexpr uint32_t kMaxBins = 64;
+
+struct Builder {
+    const float* tris;
+    uint32_t n;
+    uint32_t leaf_max;
+    uint32_t bins;
+    std::vector<Box> tb;
+    std::vector<float> cen;
+    std::vector<uint32_t> idx;
+    std::vector<N2> nodes;
+    std::atomic<uint32_t> next_node{1};
+    std::atomic<int> threads_left{0};
+
+    uint32_t alloc2() { return next_node.fetch_add(2); }
+
+    void build(uint32_t node, uint32_t begin, uint32_t end, int depth) {
+        for (;;) {
+            Box nb, cb;
+            for (uint32_t i = begin; i < end; ++i) {
+                nb.grow(tb[idx[i]]);
+                cb.grow_pt(&cen[3 * (size_t)idx[i]]);
+            }
+            N2& nd = nodes[node];
+            nd.box = nb;
+            const uint32_t cnt = end - begin;
+            if (cnt <= 1) { nd.leaf = true; nd.a = begin; nd.b = cnt; return; }
+            // Binned SAH over all three axes.
+            double best = INFINITY;
+            int best_axis = -1, best_split = -1;
+            Box bb[kMaxBins];
+            uint32_t bc[kMaxBins];
+            double ra[kMaxBins];
+            uint32_t rn[kMaxBins];
+            for (int axis = 0; axis < 3; ++axis) {
+                const float ext = cb.hi[axis] - cb.lo[axis];
+                if (!(ext > 0.0f)) continue;
+                const float scale = (float)bins / ext;
+                for (uint32_t i = 0; i < bins; ++i) { bb[i] = Box{}; bc[i] = 0u; }
+                for (uint32_t i = begin; i < end; ++i) {
+                    uint32_t t = idx[i];
+                    int bi = (int)((cen[3 * (size_t)t + axis] - cb.lo[axis]) * scale);
+                    bi = std::min(std::max(bi, 0), (int)bins - 1);
+                    bb[bi].grow(tb[t]);
+                    bc[bi]++;
+                }
+                Box acc;
+                uint32_t an = 0;
+                for (int i = (int)bins - 1; i > 0; --i) {
+                    acc.grow(bb[i]);
+                    an += bc[i];
+                    ra[i] = acc.half_area();
+                    rn[i] = an;
+                }
+                Box lacc;
+                uint32_t ln = 0;
+                for (int i = 1; i < (int)bins; ++i) {
+                    lacc.grow(bb[i - 1]);
+                    ln += bc[i - 1];
+                    if (ln == 0 || rn[i] == 0) continue;
+                    double c = lacc.half_area() * ln + ra[i] * rn[i];
+                    if (c < best) { best = c; best_axis = axis; best_split = i; }
+                }
+            }
+            const double pa = nb.half_area();
+            const double split_cost = 1.0 + (pa > 0.0 ? best / pa : (double)cnt);
+            uint32_t mid;
+            if (best_axis < 0 || (cnt <= leaf_max && (double)cnt <= split_cost)) {
+                if (cnt <= leaf_max) { nd.leaf = true; nd.a = begin; nd.b = cnt; return; }
+                // No useful split (coincident centroids): split by index.
+                mid = begin + cnt / 2;
+            } else {
+                const int axis = best_axis;
+                const float ext = cb.hi[axis] - cb.lo[axis];
+                const float scale = (float)bins / ext;
+                const float lo = cb.lo[axis];
+                const int split = best_split;
+                const int nb_ = (int)bins;
+                auto pm = std::partition(idx.begin() + begin, idx.begin() + end, [&](uint32_t t) {
+                    int bi = (int)((cen[3 * (size_t)t + axis] - lo) * scale);
+                    bi = std::min(std::max(bi, 0), nb_ - 1);
+                    return bi < split;
+                });
+                mid = (uint32_t)(pm - idx.begin());
+                if (mid == begin || mid == end) mid = begin + cnt / 2;
+            }
+            const uint32_t l = alloc2();
+            nd.leaf = false;
+            nd.a = l;
+            nd.b = l + 1;
+            // Large subtrees are built on their own thread.
+            if (end - mid > 65536 && threads_left.fetch_sub(1) > 0) {
+                std::thread th([this, l, mid, end, depth]() { build(l + 1, mid, end, depth + 1); });
+                build(l, begin, mid, depth + 1);
+                th.join();
+                return;
+            }
+            // Recurse into the smaller side and loop on the larger one, which
+            // bounds the recursion depth by log2(n).
+            if (mid - begin < end - mid) {
+                build(l, begin, mid, depth + 1);
+                node = l + 1;
+                begin = mid;
+            } else {
+                build(l + 1, mid, end, depth + 1);
+                node = l;
+                end = mid;
+            }
+            ++depth;
+        }
+    }
+};
+
+}  // namespace
+
+bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh8Host& out,
+                std::string& err) {
+    out = Bvh8Host{};
+    if (n_tris == 0) { err = "empty scene: no triangles"; return false; }
+    if (n_tris >= (1u << 28)) { err = "too many triangles (limit 2^28)"; return false; }
+    const uint32_t leaf_max = std::max(1u, std::min(opt.leaf_size ? opt.leaf_size : 4u, 4u));
+    Builder b;
+    b.tris = tris;
+    b.n = n_tris;
+    b.leaf_max = leaf_max;
+    b.bins = std::max(4u, std::min(opt.sah_bins ? opt.sah_bins : 32u, kMaxBins));
+    b.tb.resize(n_tris);
+    b.cen.resize(3 * (size_t)n_tris);
+    b.idx.resize(n_tris);
+    float max_abs = 1.0f;
+    for (uint32_t i = 0; i < n_tris; ++i) {
+        const float* t = tris + 9 * (size_t)i;
+        for (int v = 0; v < 3; ++v) {
+            for (int k = 0; k < 3; ++k) {
+                if (!std::isfinite(t[3 * v + k])) { err = "non-finite vertex coordinate"; return false; }
+                max_abs = std::max(max_abs, std::fabs(t[3 * v + k]));
+            }
+            b.tb[i].grow_pt(t + 3 * v);
+        }
+        for (int k = 0; k < 3; ++k) b.cen[3 * (size_t)i + k] = 0.5f * (b.tb[i].lo[k] + b.tb[i].hi[k]);
+        b.idx[i] = i;
+    }
+    // Widen every box by 2^-16 of the largest coordinate so the box test is
+    // conservative with respect to the watertight triangle test.
+    const float pad = std::ldexp(max_abs, -16);
+    b.nodes.resize(2 * (size_t)n_tris);
+    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    b.threads_left = (int)std::min(hw, 64u) - 1;
+    b.build(0, 0, n_tris, 0);
+    const uint32_t n2 = b.next_node.load();
+    b.nodes.resize(n2);
+    for (auto& nd : b.nodes)
+        for (int k = 0; k < 3; ++k) { nd.box.lo[k] -= pad; nd.box.hi[k] += pad; }
+
+    // Collapse to 8-wide nodes, breadth first.
+    std::vector<KpNode8> nodes8;
+    std::vector<uint32_t> tri_order;   // reordered triangle -> original index
+    nodes8.reserve(n2 / 4 + 1);
+    tri_order.reserve(n_tris);
+    struct Pending { uint32_t n2, n8; };
+    std::vector<Pending> queue;
+    nodes8.emplace_back();
+    queue.push_back({0, 0});
+    uint32_t max_depth = 1;
+    std::vector<uint32_t> depth8(1, 1);
+    for (size_t qi = 0; qi < queue.size(); ++qi) {
+        const Pending pend = queue[qi];
+        std::vector<uint32_t> kids;
+        const N2& root = b.nodes[pend.n2];
+        if (root.leaf) {
+            kids.push_back(pend.n2);
+        } else {
+            kids.push_back(root.a);
+            kids.push_back(root.b);
+            while (kids.size() < 8) {
+                int pick = -1;
+                double best = -1.0;
+                for (size_t i = 0; i < kids.size(); ++i) {
+                    const N2& c = b.nodes[kids[i]];
+                    if (c.leaf) continue;
+                    double a = c.box.half_area();
+                    if (a > best) { best = a; pick = (int)i; }
+                }
+                if (pick < 0) break;
+                const N2& c = b.nodes[kids[pick]];
+                kids[pick] = c.a;
+                kids.push_back(c.b);
+            }
+        }
+        KpNode8 nd;
+        std::memset(&nd, 0, sizeof(nd));
+        Box nb;
+        for (uint32_t k : kids) nb.grow(b.nodes[k].box);
+        nd.px = nb.lo[0];
+        nd.py = nb.lo[1];
+        nd.pz = nb.lo[2];
+        int e[3];
+        for (int k = 0; k < 3; ++k) {
+            double ext = (double)nb.hi[k] - (double)nb.lo[k];
+            int ek = -126;
+            if (ext > 0.0) ek = (int)std::ceil(std::log2(ext / 65534.0));
+            ek = std::max(-126, std::min(127, ek));
+            while (std::ldexp(65534.0, ek) < ext && ek < 127) ++ek;
+            e[k] = ek;
+        }
+        nd.ex = (int8_t)e[0];
+        nd.ey = (int8_t)e[1];
+        nd.ez = (int8_t)e[2];
+        const double p[3] = {nd.px, nd.py, nd.pz};
+        uint32_t inner_rank = 0;
+        const uint32_t child_base = (uint32_t)nodes8.size();
+        const uint32_t prim_base = (uint32_t)tri_order.size();
+        nd.child_base = child_base;
+        nd.prim_base = prim_base;
+        for (int s = 0; s < 8; ++s) {
+            if (s >= (int)kids.size()) {
+                for (int k = 0; k < 3; ++k) { nd.q[k][s] = 0xFFFF; nd.q[3 + k][s] = 0; }
+                nd.meta[s] = 0;
+                continue;
+            }
+            const N2& c = b.nodes[kids[s]];
+            for (int k = 0; k < 3; ++k) {
+                const double sc = std::ldexp(1.0, e[k]);
+                double ql = std::floor(((double)c.box.lo[k] - p[k]) / sc);
+                double qh = std::ceil(((double)c.box.hi[k] - p[k]) / sc);
+                ql = std::max(0.0, std::min(65535.0, ql));
+                qh = std::max(0.0, std::min(65535.0, qh));
+                nd.q[k][s] = (uint16_t)ql;
+                nd.q[3 + k][s] = (uint16_t)qh;
+            }
+            if (c.leaf) {
+                const uint32_t off = (uint32_t)tri_order.size() - prim_base;
+                for (uint32_t i = 0; i < c.b; ++i) tri_order.push_back(b.idx[c.a + i]);
+                nd.meta[s] = (uint8_t)((c.b << 5) | off);
+            } else {
+                nd.imask |= (uint8_t)(1u << s);
+                nd.meta[s] = 1;
+                const uint32_t child = child_base + inner_rank++;
+                queue.push_back({kids[s], child});
+                (void)child;
+            }
+        }
+        // reserve the inner children (contiguous, slot order)
+        for (uint32_t i = 0; i < inner_rank; ++i) {
+            nodes8.emplace_back();
+            depth8.push_back(depth8[pend.n8] + 1);
+            max_depth = std::max(max_depth, depth8.back());
+        }
+        nodes8[pend.n8] = nd;
+    }
+    out.nodes = std::move(nodes8);
+    out.tri_order = std::move(tri_order);
+    out.tris.resize(12 * (size_t)n_tris);
+    for (uint32_t i = 0; i < n_tris; ++i) {
+        const uint32_t src = out.tri_order[i];
+        const float* t = tris + 9 * (size_t)src;
+        float* d = &out.tris[12 * (size_t)i];
+        d[0] = t[0]; d[1] = t[1]; d[2] = t[2];
+        std::memcpy(&d[3], &src, 4);
+        d[4] = t[3]; d[5] = t[4]; d[6] = t[5]; d[7] = 0.0f;
+        d[8] = t[6]; d[9] = t[7]; d[10] = t[8]; d[11] = 0.0f;
+    }
+    out.max_depth = max_depth;
+    out.n_bvh2_nodes = n2;
+    out.pad = pad;
+    return true;
+}
+
+}  // namespace kp

@@ -1,0 +1,355 @@
+// C ABI of the HIP intersector: context, scene upload, BVH build, queries.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/kp_hip.h"
+#include "kp_ctx.h"
+
+namespace {
+thread_local std::string g_create_error;
+
+bool valid_kind(int k) { return k == KP_PTR_HOST || k == KP_PTR_DEVICE; }
+}  // namespace
+
+namespace kp {
+
+int fail(kp_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(kp_ctx* c, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();   // clear the sticky launch error
+    return fail(c, e == hipErrorOutOfMemory ? KP_ERR_ALLOC : KP_ERR_HIP, m);
+}
+
+int ensure_buffer(kp_ctx* c, void** buf, size_t* cap, size_t bytes) {
+    if (*cap >= bytes && *buf) return KP_OK;
+    if (*buf) {
+        KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+        KP_HIP_TRY(c, hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    KP_HIP_TRY(c, hipMalloc(buf, want));
+    *cap = want;
+    return KP_OK;
+}
+
+static TraceArgs base_args(kp_ctx* c) {
+    TraceArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nodes = c->d_nodes;
+    a.tris = c->d_tris;
+    a.counter = c->d_words;
+    a.err = c->d_words + 1;
+    a.spill = c->d_spill;
+    a.spill_cap = c->spill_cap;
+    a.refill_below = c->refill_below;
+    a.counters = c->d_counters;
+    a.max_steps = c->max_steps;
+    return a;
+}
+
+int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, bool count) {
+    if (n == 0) return KP_OK;
+    TraceArgs a = base_args(c);
+    a.rays = rays;
+    a.hits = hits;
+    a.n = n;
+    hipError_t e = launch_closest(a, c->grid_closest, count, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "closest-hit launch");
+    return KP_OK;
+}
+
+int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out) {
+    if (n == 0) return KP_OK;
+    TraceArgs a = base_args(c);
+    a.rays = rays;
+    a.occl = out;
+    a.n = n;
+    hipError_t e = launch_occluded(a, c->grid_occluded, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "occlusion launch");
+    return KP_OK;
+}
+
+static void free_bvh(kp_ctx* c) {
+    if (c->d_nodes) (void)hipFree(c->d_nodes);
+    if (c->d_tris) (void)hipFree(c->d_tris);
+    if (c->d_spill) (void)hipFree(c->d_spill);
+    c->d_nodes = nullptr;
+    c->d_tris = nullptr;
+    c->d_spill = nullptr;
+    c->built = false;
+}
+
+}  // namespace kp
+
+extern "C" {
+
+int kp_create(kp_ctx** out, int device) {
+    if (!out) return kp::fail(nullptr, KP_ERR_INVALID, "kp_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+        return kp::fail(nullptr, KP_ERR_HIP, "kp_create: no HIP device available");
+    if (device < 0 || device >= ndev) return kp::fail(nullptr, KP_ERR_INVALID, "kp_create: bad device index");
+    kp_ctx* c = new kp_ctx();
+    c->device = device;
+    auto bail = [&](hipError_t he, const char* what) {
+        std::string m = std::string("kp_create: ") + what + ": " + hipGetErrorString(he);
+        kp_destroy(c);
+        return kp::fail(nullptr, KP_ERR_HIP, m);
+    };
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return bail(e, "properties");
+    c->cus = prop.multiProcessorCount;
+    if ((e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "stream");
+    c->stream = c->own_stream;
+    if ((e = hipMalloc((void**)&c->d_words, 64)) != hipSuccess) return bail(e, "alloc");
+    if ((e = hipMemset(c->d_words, 0, 64)) != hipSuccess) return bail(e, "memset");
+    if ((e = hipMalloc((void**)&c->d_counters, 64)) != hipSuccess) return bail(e, "alloc");
+    if ((e = hipMemset(c->d_counters, 0, 64)) != hipSuccess) return bail(e, "memset");
+    int bc = 0, bo = 0;
+    if ((e = kp::trace_occupancy(&bc, &bo)) != hipSuccess) return bail(e, "occupancy");
+    c->grid_closest = (uint32_t)(c->cus * (bc > 0 ? bc : 1));
+    c->grid_occluded = (uint32_t)(c->cus * (bo > 0 ? bo : 1));
+    *out = c;
+    return KP_OK;
+}
+
+void kp_destroy(kp_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    kp::render_state_free(c);
+    kp::free_bvh(c);
+    if (c->d_words) (void)hipFree(c->d_words);
+    if (c->d_counters) (void)hipFree(c->d_counters);
+    if (c->d_stage_in) (void)hipFree(c->d_stage_in);
+    if (c->d_stage_out) (void)hipFree(c->d_stage_out);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+const char* kp_last_error(const kp_ctx* c) {
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int kp_set_stream(kp_ctx* c, void* s) {
+    if (!c) return KP_ERR_INVALID;
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return KP_OK;
+}
+
+int kp_synchronize(kp_ctx* c) {
+    if (!c) return KP_ERR_INVALID;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    uint32_t flags = 0;
+    KP_HIP_TRY(c, hipMemcpy(&flags, c->d_words + 1, 4, hipMemcpyDeviceToHost));
+    if (flags) {
+        KP_HIP_TRY(c, hipMemset(c->d_words + 1, 0, 4));
+        return kp::fail(c, KP_ERR_KERNEL,
+                        (flags & 1u) ? "traversal stack exceeded its spill capacity"
+                                     : "traversal exceeded its step limit");
+    }
+    return KP_OK;
+}
+
+int kp_mesh_add(kp_ctx* c, const float* xyz, uint32_t n_verts, const uint32_t* idx,
+                uint32_t n_tris, const float* xf, uint32_t* mesh_id) {
+    if (!c) return KP_ERR_INVALID;
+    if (!xyz || !idx || n_verts == 0 || n_tris == 0)
+        return kp::fail(c, KP_ERR_INVALID, "kp_mesh_add: empty mesh or NULL arrays");
+    if ((uint64_t)c->soup.size() / 9 + n_tris >= (1ull << 28))
+        return kp::fail(c, KP_ERR_INVALID, "kp_mesh_add: scene exceeds 2^28 triangles");
+    for (uint64_t i = 0; i < 3ull * n_tris; ++i)
+        if (idx[i] >= n_verts) return kp::fail(c, KP_ERR_INVALID, "kp_mesh_add: vertex index out of range");
+    const size_t base = c->soup.size();
+    c->soup.resize(base + 9 * (size_t)n_tris);
+    for (uint32_t t = 0; t < n_tris; ++t) {
+        for (int v = 0; v < 3; ++v) {
+            const float* p = xyz + 3 * (size_t)idx[3 * (size_t)t + v];
+            float* d = &c->soup[base + 9 * (size_t)t + 3 * v];
+            if (xf) {
+                for (int r = 0; r < 3; ++r)
+                    d[r] = xf[4 * r + 0] * p[0] + xf[4 * r + 1] * p[1] + xf[4 * r + 2] * p[2] + xf[4 * r + 3];
+            } else {
+                d[0] = p[0]; d[1] = p[1]; d[2] = p[2];
+            }
+        }
+    }
+    const uint32_t id = (uint32_t)c->mesh_first.size();
+    c->mesh_first.push_back((uint32_t)(base / 9));
+    c->mesh_count.push_back(n_tris);
+    c->mesh_material.push_back(0);
+    if (mesh_id) *mesh_id = id;
+    return KP_OK;
+}
+
+int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
+    if (!c) return KP_ERR_INVALID;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t n = (uint32_t)(c->soup.size() / 9);
+    kp::BuildOptions bo;
+    if (o) {
+        if (o->leaf_size) bo.leaf_size = o->leaf_size;
+        if (o->sah_bins) bo.sah_bins = o->sah_bins;
+    }
+    kp::Bvh8Host h;
+    std::string err;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!kp::build_bvh8(c->soup.data(), n, bo, h, err)) return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: " + err);
+    auto t1 = std::chrono::steady_clock::now();
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    kp::free_bvh(c);
+    const size_t nb = h.nodes.size() * sizeof(KpNode8), tb = h.tris.size() * sizeof(float);
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tb));
+    KP_HIP_TRY(c, hipMemcpy(c->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMemcpy(c->d_tris, h.tris.data(), tb, hipMemcpyHostToDevice));
+    // Worst-case stack: 7 pushes per level below the root, plus slack; the
+    // top KP_TRACE_STACK entries live in LDS.
+    c->spill_cap = 7u * h.max_depth + 8u;
+    const uint32_t grid = c->grid_closest > c->grid_occluded ? c->grid_closest : c->grid_occluded;
+    c->spill_lanes = (size_t)grid * KP_TRACE_BLOCK;
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
+    c->n_tris_built = n;
+    c->n_nodes = (uint32_t)h.nodes.size();
+    c->n_bvh2 = h.n_bvh2_nodes;
+    c->max_depth = h.max_depth;
+    // A traversal visits every node and leaf at most once.
+    c->max_steps = 2u * (uint32_t)h.nodes.size() + n + 64u;
+    c->build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    c->built = true;
+    return KP_OK;
+}
+
+int kp_get_stats(kp_ctx* c, kp_stats* s) {
+    if (!c || !s) return KP_ERR_INVALID;
+    std::memset(s, 0, sizeof(*s));
+    s->n_tris = c->n_tris_built;
+    s->n_nodes = c->n_nodes;
+    s->n_bvh2_nodes = c->n_bvh2;
+    s->bvh_bytes = (uint64_t)c->n_nodes * KP_NODE_BYTES + (uint64_t)c->n_tris_built * KP_TRI_BYTES;
+    s->max_depth = c->max_depth;
+    s->spill_entries = c->spill_cap;
+    s->grid_closest = c->grid_closest;
+    s->grid_occluded = c->grid_occluded;
+    s->build_seconds = c->build_seconds;
+    s->device_cus = (uint32_t)c->cus;
+    return KP_OK;
+}
+
+static int closest_common(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hits, int kind,
+                          bool count, uint64_t* nodes, uint64_t* tris) {
+    if (!c) return KP_ERR_INVALID;
+    if (!c->built) return kp::fail(c, KP_ERR_STATE, "intersect before kp_bvh_build");
+    if (!valid_kind(kind)) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
+    if (n == 0) return KP_OK;
+    if (!rays || !hits) return kp::fail(c, KP_ERR_INVALID, "NULL rays or hits");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    if (count) KP_HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, 16, c->stream));
+    int rc;
+    if (kind == KP_PTR_DEVICE) {
+        rc = kp::trace_closest_dev(c, (const float4*)rays, n, (float4*)hits, count);
+        if (rc) return rc;
+    } else {
+        const size_t rb = (size_t)n * sizeof(kp_ray), hb = (size_t)n * sizeof(kp_hit);
+        if ((rc = kp::ensure_buffer(c, &c->d_stage_in, &c->stage_in_bytes, rb))) return rc;
+        if ((rc = kp::ensure_buffer(c, &c->d_stage_out, &c->stage_out_bytes, hb))) return rc;
+        KP_HIP_TRY(c, hipMemcpyAsync(c->d_stage_in, rays, rb, hipMemcpyHostToDevice, c->stream));
+        rc = kp::trace_closest_dev(c, (const float4*)c->d_stage_in, n, (float4*)c->d_stage_out, count);
+        if (rc) return rc;
+        KP_HIP_TRY(c, hipMemcpyAsync(hits, c->d_stage_out, hb, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (count || kind == KP_PTR_HOST) {
+        if ((rc = kp_synchronize(c))) return rc;
+    }
+    if (count) {
+        unsigned long long v[2];
+        KP_HIP_TRY(c, hipMemcpy(v, c->d_counters, 16, hipMemcpyDeviceToHost));
+        if (nodes) *nodes = v[0];
+        if (tris) *tris = v[1];
+    }
+    return KP_OK;
+}
+
+int kp_intersect_closest(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hits, int kind) {
+    return closest_common(c, rays, n, hits, kind, false, nullptr, nullptr);
+}
+
+int kp_intersect_closest_counted(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hits, int kind,
+                                 uint64_t* nodes, uint64_t* tris) {
+    return closest_common(c, rays, n, hits, kind, true, nodes, tris);
+}
+
+int kp_occluded(kp_ctx* c, const kp_ray* rays, uint32_t n, uint32_t* out, int kind) {
+    if (!c) return KP_ERR_INVALID;
+    if (!c->built) return kp::fail(c, KP_ERR_STATE, "occluded before kp_bvh_build");
+    if (!valid_kind(kind)) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
+    if (n == 0) return KP_OK;
+    if (!rays || !out) return kp::fail(c, KP_ERR_INVALID, "NULL rays or out");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    if (kind == KP_PTR_DEVICE) return kp::trace_occluded_dev(c, (const float4*)rays, n, out);
+    const size_t rb = (size_t)n * sizeof(kp_ray), ob = (size_t)n * 4;
+    if ((rc = kp::ensure_buffer(c, &c->d_stage_in, &c->stage_in_bytes, rb))) return rc;
+    if ((rc = kp::ensure_buffer(c, &c->d_stage_out, &c->stage_out_bytes, ob))) return rc;
+    KP_HIP_TRY(c, hipMemcpyAsync(c->d_stage_in, rays, rb, hipMemcpyHostToDevice, c->stream));
+    rc = kp::trace_occluded_dev(c, (const float4*)c->d_stage_in, n, (uint32_t*)c->d_stage_out);
+    if (rc) return rc;
+    KP_HIP_TRY(c, hipMemcpyAsync(out, c->d_stage_out, ob, hipMemcpyDeviceToHost, c->stream));
+    return kp_synchronize(c);
+}
+
+struct kp_bvh8_blob {
+    kp::Bvh8Host h;
+};
+
+int kp_bvh8_build_host(const float* tris9, uint32_t n, const kp_build_opts* o, kp_bvh8_blob** out) {
+    if (!out || !tris9) return kp::fail(nullptr, KP_ERR_INVALID, "kp_bvh8_build_host: NULL argument");
+    *out = nullptr;
+    kp::BuildOptions bo;
+    if (o) {
+        if (o->leaf_size) bo.leaf_size = o->leaf_size;
+        if (o->sah_bins) bo.sah_bins = o->sah_bins;
+    }
+    kp_bvh8_blob* b = new kp_bvh8_blob();
+    std::string err;
+    if (!kp::build_bvh8(tris9, n, bo, b->h, err)) {
+        delete b;
+        return kp::fail(nullptr, KP_ERR_INVALID, "kp_bvh8_build_host: " + err);
+    }
+    *out = b;
+    return KP_OK;
+}
+
+int kp_bvh8_blob_info(const kp_bvh8_blob* b, uint64_t* nn, uint64_t* nt, uint32_t* md) {
+    if (!b) return KP_ERR_INVALID;
+    if (nn) *nn = b->h.nodes.size();
+    if (nt) *nt = b->h.tri_order.size();
+    if (md) *md = b->h.max_depth;
+    return KP_OK;
+}
+
+const void* kp_bvh8_blob_nodes(const kp_bvh8_blob* b) { return b ? b->h.nodes.data() : nullptr; }
+const void* kp_bvh8_blob_tris(const kp_bvh8_blob* b) { return b ? b->h.tris.data() : nullptr; }
+void kp_bvh8_blob_free(kp_bvh8_blob* b) { delete b; }
+
+int kp_set_trace_tuning(kp_ctx* c, uint32_t lanes) {
+    if (!c) return KP_ERR_INVALID;
+    if (lanes < 1 || lanes > 64) return kp::fail(c, KP_ERR_INVALID, "refill threshold must be 1..64");
+    c->refill_below = lanes;
+    return KP_OK;
+}
+
+}  // extern "C"

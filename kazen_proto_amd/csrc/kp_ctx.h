@@ -1,0 +1,68 @@
+// Context object behind the C ABI (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "kp_bvh.h"
+#include "kp_kernels.h"
+
+struct kp_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // scene (host copy, world space)
+    std::vector<float> soup;          // 9 floats per triangle
+    std::vector<uint32_t> mesh_first; // first global primitive of each mesh
+    std::vector<uint32_t> mesh_count;
+    std::vector<uint32_t> mesh_material;
+
+    // BVH on the device
+    bool built = false;
+    uint32_t n_tris_built = 0;
+    uint32_t n_nodes = 0, n_bvh2 = 0, max_depth = 0;
+    double build_seconds = 0.0;
+    uint4* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    uint2* d_spill = nullptr;
+    uint32_t spill_cap = 0;
+    size_t spill_lanes = 0;
+    uint32_t grid_closest = 0, grid_occluded = 0;
+    uint32_t refill_below = 40;
+    uint32_t max_steps = 64;
+
+    // small device words: [0] work counter, [1] error flags
+    uint32_t* d_words = nullptr;
+    unsigned long long* d_counters = nullptr;
+
+    // staging for host-pointer calls
+    void* d_stage_in = nullptr;
+    size_t stage_in_bytes = 0;
+    void* d_stage_out = nullptr;
+    size_t stage_out_bytes = 0;
+
+    // render state lives in kp_render.cpp
+    struct RenderState* render = nullptr;
+};
+
+namespace kp {
+// Records a HIP failure on the context and returns KP_ERR_HIP.
+int hip_fail(kp_ctx* c, hipError_t e, const char* what);
+int fail(kp_ctx* c, int code, const std::string& msg);
+// Grows a device buffer to at least `bytes`.
+int ensure_buffer(kp_ctx* c, void** buf, size_t* cap, size_t bytes);
+// Trace helpers shared with the integrator (device pointers, async).
+int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, bool count);
+int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out);
+void render_state_free(kp_ctx* c);
+}  // namespace kp
+
+#define KP_HIP_TRY(ctx, expr)                                  \
+    do {                                                       \
+        hipError_t kp_e_ = (expr);                             \
+        if (kp_e_ != hipSuccess) return kp::hip_fail((ctx), kp_e_, #expr); \
+    } while (0)

@@ -1,0 +1,33 @@
+// Kernel-launch interface between the C-ABI layer and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Threads per block of the traversal kernels and LDS short-stack depth.
+#define KP_TRACE_BLOCK 256
+#define KP_TRACE_STACK 8
+
+namespace kp {
+
+struct TraceArgs {
+    const uint4* nodes;     // BVH8 nodes, 8 x uint4 each
+    const float4* tris;     // 3 x float4 per triangle
+    const float4* rays;     // 2 x float4 per ray {o, tmin}, {d, tmax}
+    float4* hits;           // closest: {t, prim bits, u, v}
+    uint32_t* occl;         // occluded: 0/1 per ray
+    uint32_t n;             // ray count
+    uint32_t* counter;      // work counter (reset by the launcher)
+    uint2* spill;           // per-lane stack spill area
+    uint32_t spill_cap;     // entries per lane
+    uint32_t refill_below;  // refill a wave once fewer lanes than this are busy
+    uint32_t* err;          // bit0 stack overflow, bit1 step limit
+    uint32_t max_steps;     // per-ray step limit (> any legitimate traversal)
+    unsigned long long* counters;   // [nodes visited, triangles tested]
+};
+
+hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStream_t s);
+hipError_t launch_occluded(const TraceArgs& a, uint32_t grid, hipStream_t s);
+hipError_t trace_occupancy(int* blocks_per_cu_closest, int* blocks_per_cu_occluded);
+hipError_t launch_reset_u32(uint32_t* p, uint32_t n, hipStream_t s);
+
+}  // namespace kp

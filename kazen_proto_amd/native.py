@@ -1,0 +1,223 @@
+"""ctypes binding of the C ABI in include/kp_hip.h (libkp_hip.so).
+
+The library is loaded from this package directory; if it is missing the
+import of any entry point raises (there is no CPU fallback).  Arrays may be
+numpy (host memory, KP_PTR_HOST) or torch CUDA tensors (device memory of the
+context's device, KP_PTR_DEVICE).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkp_hip.so")
+
+KP_OK = 0
+KP_PTR_HOST = 0
+KP_PTR_DEVICE = 1
+
+_p = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int
+
+_lib = None
+
+
+class KpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("kp error %d: %s" % (code, msg))
+        self.code = code
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("leaf_size", _u32), ("sah_bins", _u32), ("reserved", _u32 * 6)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_tris", _u64), ("n_nodes", _u64), ("n_bvh2_nodes", _u64),
+                ("bvh_bytes", _u64), ("max_depth", _u32), ("spill_entries", _u32),
+                ("grid_closest", _u32), ("grid_occluded", _u32), ("build_seconds", ctypes.c_double),
+                ("device_cus", _u32), ("reserved", _u32 * 7)]
+
+
+# name -> (restype, argtypes); mirrors include/kp_hip.h
+SIGNATURES = {
+    "kp_create": (_i32, [ctypes.POINTER(_p), _i32]),
+    "kp_destroy": (None, [_p]),
+    "kp_last_error": (ctypes.c_char_p, [_p]),
+    "kp_set_stream": (_i32, [_p, _p]),
+    "kp_synchronize": (_i32, [_p]),
+    "kp_mesh_add": (_i32, [_p, _p, _u32, _p, _u32, _p, ctypes.POINTER(_u32)]),
+    "kp_bvh_build": (_i32, [_p, _p]),
+    "kp_get_stats": (_i32, [_p, _p]),
+    "kp_intersect_closest": (_i32, [_p, _p, _u32, _p, _i32]),
+    "kp_occluded": (_i32, [_p, _p, _u32, _p, _i32]),
+    "kp_intersect_closest_counted": (_i32, [_p, _p, _u32, _p, _i32, ctypes.POINTER(_u64),
+                                            ctypes.POINTER(_u64)]),
+    "kp_bvh8_build_host": (_i32, [_p, _u32, _p, ctypes.POINTER(_p)]),
+    "kp_bvh8_blob_info": (_i32, [_p, ctypes.POINTER(_u64), ctypes.POINTER(_u64),
+                                 ctypes.POINTER(_u32)]),
+    "kp_bvh8_blob_nodes": (_p, [_p]),
+    "kp_bvh8_blob_tris": (_p, [_p]),
+    "kp_bvh8_blob_free": (None, [_p]),
+    "kp_set_trace_tuning": (_i32, [_p, _u32]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libkp_hip.so is not built: run __graft_entry__.build() "
+                               "(the HIP path has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def bvh8_build_host(tris9, leaf_size=4, sah_bins=32):
+    """Host-only BVH8 build -> (nodes uint8[n*128], tris float32[n*12], max_depth)."""
+    tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    o = BuildOpts()
+    o.leaf_size = leaf_size
+    o.sah_bins = sah_bins
+    blob = _p()
+    rc = lib().kp_bvh8_build_host(_np_ptr(tris9), tris9.shape[0], ctypes.byref(o), ctypes.byref(blob))
+    if rc != KP_OK:
+        raise KpError(rc, lib().kp_last_error(None).decode())
+    try:
+        nn, nt, md = _u64(), _u64(), _u32()
+        lib().kp_bvh8_blob_info(blob, ctypes.byref(nn), ctypes.byref(nt), ctypes.byref(md))
+        nodes = np.ctypeslib.as_array(ctypes.cast(lib().kp_bvh8_blob_nodes(blob),
+                                                  ctypes.POINTER(ctypes.c_uint8)),
+                                      shape=(nn.value * 128,)).copy()
+        tris = np.ctypeslib.as_array(ctypes.cast(lib().kp_bvh8_blob_tris(blob),
+                                                 ctypes.POINTER(ctypes.c_float)),
+                                     shape=(nt.value * 12,)).copy()
+        return nodes, tris, int(md.value)
+    finally:
+        lib().kp_bvh8_blob_free(blob)
+
+
+class Context:
+    """One HIP device context: scene, BVH and queries (one stream)."""
+
+    def __init__(self, device=0):
+        self._h = _p()
+        rc = lib().kp_create(ctypes.byref(self._h), device)
+        if rc != KP_OK:
+            raise KpError(rc, lib().kp_last_error(None).decode())
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != KP_OK:
+            raise KpError(rc, lib().kp_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr):
+        self._check(lib().kp_set_stream(self._h, _p(stream_ptr) if stream_ptr else None))
+
+    def synchronize(self):
+        self._check(lib().kp_synchronize(self._h))
+
+    def mesh_add(self, verts, idx, xform=None):
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        xf = None
+        if xform is not None:
+            xf = np.ascontiguousarray(xform, np.float32).reshape(12)
+        mid = _u32()
+        self._check(lib().kp_mesh_add(self._h, _np_ptr(verts), verts.shape[0], _np_ptr(idx),
+                                      idx.shape[0], _np_ptr(xf) if xf is not None else None,
+                                      ctypes.byref(mid)))
+        return mid.value
+
+    def mesh_add_soup(self, tris9):
+        tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+        n = tris9.shape[0]
+        return self.mesh_add(tris9.reshape(-1, 3), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3))
+
+    def bvh_build(self, leaf_size=4, sah_bins=32):
+        o = BuildOpts()
+        o.leaf_size = leaf_size
+        o.sah_bins = sah_bins
+        self._check(lib().kp_bvh_build(self._h, ctypes.byref(o)))
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().kp_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_ if k != "reserved"}
+
+    def set_trace_tuning(self, refill_below_lanes):
+        self._check(lib().kp_set_trace_tuning(self._h, refill_below_lanes))
+
+    def intersect_closest(self, rays, out=None, counted=False):
+        """rays: [n,8] float32 (numpy or torch CUDA). Returns [n,4] float32
+        hits {t, prim bits, u, v} of the same kind (and counts if counted)."""
+        if _is_torch(rays):
+            import torch
+            assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous()
+            n = rays.numel() // 8
+            if out is None:
+                out = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+            rp, hp, kind = _p(rays.data_ptr()), _p(out.data_ptr()), KP_PTR_DEVICE
+        else:
+            rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+            n = rays.shape[0]
+            if out is None:
+                out = np.zeros((n, 4), np.float32)
+            rp, hp, kind = _np_ptr(rays), _np_ptr(out), KP_PTR_HOST
+        if counted:
+            nv, nt = _u64(), _u64()
+            self._check(lib().kp_intersect_closest_counted(self._h, rp, n, hp, kind,
+                                                           ctypes.byref(nv), ctypes.byref(nt)))
+            return out, int(nv.value), int(nt.value)
+        self._check(lib().kp_intersect_closest(self._h, rp, n, hp, kind))
+        return out
+
+    def occluded(self, rays, out=None):
+        if _is_torch(rays):
+            import torch
+            n = rays.numel() // 8
+            if out is None:
+                out = torch.empty((n,), dtype=torch.int32, device=rays.device)
+            self._check(lib().kp_occluded(self._h, _p(rays.data_ptr()), n, _p(out.data_ptr()),
+                                          KP_PTR_DEVICE))
+            return out
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        if out is None:
+            out = np.zeros(n, np.uint32)
+        self._check(lib().kp_occluded(self._h, _np_ptr(rays), n, _np_ptr(out), KP_PTR_HOST))
+        return out
+
+
+def hit_fields(hits):
+    """Split an [n,4] float32 hit array into (t, prim uint32, u, v) numpy views."""
+    h = np.asarray(hits)
+    return h[:, 0], h[:, 1].view(np.uint32), h[:, 2], h[:, 3]

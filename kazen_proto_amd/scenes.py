@@ -1,0 +1,274 @@
+"""Procedural workloads: seeded scenes, cameras and ray sets.
+
+A scene is a list of meshes (world-space triangle soups) with one material
+each; emissive meshes are area lights (one light per mesh, as the reference
+creates one GeometryLight per emissive mesh, src/core/scene.h:144-160).
+Materials use the closure parameter table of the integrator:
+  type 0 Lambert(albedo), 1 conductor(dist, alpha, Schlick F0),
+  2 rough dielectric(dist, alpha, eta); dist 0 GGX, 1 Beckmann; Le = emission.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+LAMBERT, CONDUCTOR, DIELECTRIC = 0, 1, 2
+GGX, BECKMANN = 0, 1
+
+
+def material(type_=LAMBERT, dist=GGX, albedo=(0.0, 0.0, 0.0), F0=(0.0, 0.0, 0.0), alpha=0.5,
+             eta=1.5, Le=(0.0, 0.0, 0.0)):
+    return dict(type=int(type_), dist=int(dist), albedo=tuple(map(float, albedo)),
+                F0=tuple(map(float, F0)), alpha=float(alpha), eta=float(eta),
+                Le=tuple(map(float, Le)))
+
+
+@dataclass
+class Scene:
+    meshes: list = field(default_factory=list)        # float32 [n,9] each
+    mesh_material: list = field(default_factory=list)  # material index per mesh
+    materials: list = field(default_factory=list)
+    eye: tuple = (0.0, 0.0, 3.0)
+    lookat: tuple = (0.0, 0.0, 0.0)
+    up: tuple = (0.0, 1.0, 0.0)
+    fov_deg: float = 45.0
+    width: int = 256
+    height: int = 256
+    spp: int = 16
+    max_depth: int = 5
+    rr_depth: int = 3
+    seed: int = 0
+    name: str = ""
+
+    def add_mesh(self, tris9, mat_index):
+        self.meshes.append(np.ascontiguousarray(tris9, np.float32).reshape(-1, 9))
+        self.mesh_material.append(int(mat_index))
+
+    @property
+    def n_tris(self):
+        return int(sum(m.shape[0] for m in self.meshes))
+
+    def soup(self):
+        return np.concatenate(self.meshes, axis=0)
+
+    def tri_material(self):
+        return np.concatenate([np.full(m.shape[0], mi, np.int32)
+                               for m, mi in zip(self.meshes, self.mesh_material)])
+
+    def tri_light(self):
+        """Light id per triangle (-1: not emissive); one light per emissive mesh."""
+        out, lid = [], 0
+        for m, mi in zip(self.meshes, self.mesh_material):
+            if any(v != 0.0 for v in self.materials[mi]["Le"]):
+                out.append(np.full(m.shape[0], lid, np.int32))
+                lid += 1
+            else:
+                out.append(np.full(m.shape[0], -1, np.int32))
+        return np.concatenate(out)
+
+    @property
+    def tan_half_fov(self):
+        return float(np.float32(math.tan(math.radians(self.fov_deg) * 0.5)))
+
+
+# ------------------------------------------------------------------ meshes --
+def quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); normal = (p1-p0) x (p2-p0)."""
+    p = [np.asarray(x, np.float64) for x in (p0, p1, p2, p3)]
+    return np.array([np.concatenate([p[0], p[1], p[2]]),
+                     np.concatenate([p[0], p[2], p[3]])], np.float32)
+
+
+def uv_sphere(n_lon, n_lat, center=(0, 0, 0), radius=1.0, drop_north_degenerate=True,
+              displace=None):
+    """UV sphere: n_lon x n_lat quads, two triangles each.  The north-pole band
+    has one degenerate triangle per quad, dropped when drop_north_degenerate
+    (so 64x32 gives 4,032 triangles); the south band keeps its zero-area ones.
+    displace(p) -> radial scale factor per unit-sphere vertex."""
+    j = np.arange(n_lat + 1)
+    i = np.arange(n_lon)
+    theta = np.pi * j / n_lat
+    phi = 2.0 * np.pi * i / n_lon
+    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+    sp, cp = np.sin(phi)[None, :], np.cos(phi)[None, :]
+    x = st * cp
+    y = np.broadcast_to(ct, (n_lat + 1, n_lon))
+    z = st * sp
+    p = np.stack([x, y, z], -1)                       # [lat+1, lon, 3]
+    p[0, :] = (0.0, 1.0, 0.0)
+    p[-1, :] = (0.0, -1.0, 0.0)
+    if displace is not None:
+        s = displace(p.reshape(-1, 3)).reshape(n_lat + 1, n_lon, 1)
+        p = p * s
+    p = p * radius + np.asarray(center, np.float64)
+    i1 = (i + 1) % n_lon
+    v00 = p[:-1][:, i]
+    v01 = p[:-1][:, i1]
+    v10 = p[1:][:, i]
+    v11 = p[1:][:, i1]
+    t_a = np.concatenate([v00, v10, v11], -1)         # [lat, lon, 9]
+    t_b = np.concatenate([v00, v11, v01], -1)
+    tris = np.stack([t_a, t_b], 2)                    # [lat, lon, 2, 9]
+    if drop_north_degenerate:
+        north = tris[0, :, 0:1]                       # keep (v00, v10, v11) at the north band
+        rest = tris[1:].reshape(-1, 9)
+        return np.concatenate([north.reshape(-1, 9), rest], 0).astype(np.float32)
+    return tris.reshape(-1, 9).astype(np.float32)
+
+
+# ------------------------------------------------------------ Perlin noise --
+def _perlin_perm(rng):
+    p = rng.permutation(256)
+    return np.concatenate([p, p]).astype(np.int64)
+
+
+def perlin3(pts, perm):
+    """Improved Perlin noise (K. Perlin, "Improving Noise", SIGGRAPH 2002)."""
+    pts = np.asarray(pts, np.float64)
+    fl = np.floor(pts)
+    ci = fl.astype(np.int64) & 255
+    f = pts - fl
+    u = f * f * f * (f * (f * 6 - 15) + 10)
+
+    def grad(h, x, y, z):
+        h = h & 15
+        a = np.where(h < 8, x, y)
+        b = np.where(h < 4, y, np.where((h == 12) | (h == 14), x, z))
+        return np.where(h & 1, -a, a) + np.where(h & 2, -b, b)
+
+    X, Y, Z = ci[:, 0], ci[:, 1], ci[:, 2]
+    x, y, z = f[:, 0], f[:, 1], f[:, 2]
+    A = perm[X] + Y
+    AA = perm[A] + Z
+    AB = perm[A + 1] + Z
+    B = perm[X + 1] + Y
+    BA = perm[B] + Z
+    BB = perm[B + 1] + Z
+    lerp = lambda t, a, b: a + t * (b - a)
+    g000 = grad(perm[AA], x, y, z)
+    g100 = grad(perm[BA], x - 1, y, z)
+    g010 = grad(perm[AB], x, y - 1, z)
+    g110 = grad(perm[BB], x - 1, y - 1, z)
+    g001 = grad(perm[AA + 1], x, y, z - 1)
+    g101 = grad(perm[BA + 1], x - 1, y, z - 1)
+    g011 = grad(perm[AB + 1], x, y - 1, z - 1)
+    g111 = grad(perm[BB + 1], x - 1, y - 1, z - 1)
+    return lerp(u[:, 2],
+                lerp(u[:, 1], lerp(u[:, 0], g000, g100), lerp(u[:, 0], g010, g110)),
+                lerp(u[:, 1], lerp(u[:, 0], g001, g101), lerp(u[:, 0], g011, g111)))
+
+
+# ------------------------------------------------------------ ray helpers --
+def pinhole_rays(eye, lookat, up, fov_deg, width, height):
+    """One ray per pixel centre, float32 [w*h, 8] {o, tmin=0, d, tmax=inf}."""
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(lookat, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, np.float64))
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    th = math.tan(math.radians(fov_deg) * 0.5)
+    aspect = width / height
+    px = (np.arange(width) + 0.5) / width
+    py = (np.arange(height) + 0.5) / height
+    nx = (2 * px - 1) * th * aspect
+    ny = (1 - 2 * py) * th
+    d = fwd[None, None, :] + nx[None, :, None] * right + ny[:, None, None] * upv
+    d = d.reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    n = d.shape[0]
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = eye
+    r[:, 3] = 0.0
+    r[:, 4:7] = d
+    r[:, 7] = np.inf
+    return r
+
+
+def incoherent_rays(n, rng, box=1.2):
+    """Origins uniform in [-box, box]^3, directions uniform on S^2."""
+    o = rng.uniform(-box, box, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = 0.0
+    r[:, 4:7] = d
+    r[:, 7] = np.inf
+    return r
+
+
+# ------------------------------------------------------ intersection bench --
+def microbench_triangles(seed=1, n_lon=1024, n_lat=512, n_random=100_000, amplitude=0.05,
+                         frequency=4.0):
+    """Displaced unit sphere (n_lon x n_lat quads, all 2*n_lon*n_lat triangles
+    kept) plus n_random small triangles (edge U[1e-3,1e-2], centres uniform in
+    [-1,1]^3).  Defaults: 1,048,576 + 100,000 = 1,148,576 triangles."""
+    rng = np.random.default_rng(seed)
+    perm = _perlin_perm(rng)
+    disp = lambda p: 1.0 + amplitude * perlin3(p * frequency, perm)
+    sphere = uv_sphere(n_lon, n_lat, drop_north_degenerate=False, displace=disp)
+    c = rng.uniform(-1.0, 1.0, size=(n_random, 3))
+    edge = rng.uniform(1e-3, 1e-2, size=(n_random, 1))
+    rad = edge / math.sqrt(3.0)
+    a = rng.normal(size=(n_random, 3))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    b = rng.normal(size=(n_random, 3))
+    b -= (b * a).sum(1, keepdims=True) * a
+    b /= np.linalg.norm(b, axis=1, keepdims=True)
+    vs = []
+    for k in range(3):
+        ang = 2.0 * math.pi * k / 3.0
+        vs.append(c + rad * (math.cos(ang) * a + math.sin(ang) * b))
+    small = np.concatenate(vs, axis=1).astype(np.float32)
+    return np.concatenate([sphere, small], 0)
+
+
+def microbench_rays(seed=1, n_primary=4_194_304, n_incoherent=4_194_304):
+    """(primary, incoherent) float32 [n,8] ray sets.  Primary rays: pinhole at
+    (0,0,3) aimed at the origin, 45 degree vertical fov, square image of
+    n_primary pixel centres."""
+    side = int(round(math.sqrt(n_primary)))
+    assert side * side == n_primary, "n_primary must be a square"
+    prim = pinhole_rays((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, side, side)
+    rng = np.random.default_rng(seed + 1000)
+    inc = incoherent_rays(n_incoherent, rng)
+    return prim, inc
+
+
+# ------------------------------------------------------------ Cornell box --
+def cornell_box(width=256, height=256, spp=16, max_depth=5, rr_depth=3, seed=7,
+                sphere_lon=64, sphere_lat=32):
+    """Closed-ish box [-1,1]^3 (5 walls, open toward the camera), a 0.25 x 0.25
+    ceiling light (Le 15) and a Lambert 0.6 UV sphere (64x32 -> 4,032 tris)."""
+    s = Scene(name="cornell", width=width, height=height, spp=spp, max_depth=max_depth,
+              rr_depth=rr_depth, seed=seed, eye=(0.0, 0.0, 3.4), lookat=(0.0, 0.0, 0.0),
+              up=(0.0, 1.0, 0.0), fov_deg=45.0)
+    grey = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0.75, 0.75, 0.75)))
+    red = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0.75, 0.1, 0.1)))
+    green = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0.1, 0.75, 0.1)))
+    sph = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0.6, 0.6, 0.6)))
+    light = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0, 0, 0), Le=(15, 15, 15)))
+    # walls, normals pointing into the box
+    s.add_mesh(quad((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1)), grey)     # floor
+    s.add_mesh(quad((-1, 1, 1), (-1, 1, -1), (1, 1, -1), (1, 1, 1)), grey)         # ceiling
+    s.add_mesh(quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1)), grey)     # back
+    s.add_mesh(quad((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1)), red)      # left
+    s.add_mesh(quad((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1)), green)        # right
+    h = 0.125
+    y = 0.999
+    s.add_mesh(quad((-h, y, -h), (h, y, -h), (h, y, h), (-h, y, h)), light)        # faces -y
+    s.add_mesh(uv_sphere(sphere_lon, sphere_lat, center=(0.35, -0.6, -0.25), radius=0.4), sph)
+    return s
+
+
+def small_test_scene(seed=3, n_random=400):
+    """Tiny scene for brute-force parity: a 16x8 sphere, a floor quad and random
+    triangles, including shared edges and vertices."""
+    rng = np.random.default_rng(seed)
+    parts = [uv_sphere(16, 8, center=(0.1, 0.0, -0.2), radius=0.6),
+             quad((-2, -1, 2), (2, -1, 2), (2, -1, -2), (-2, -1, -2))]
+    c = rng.uniform(-1.0, 1.0, size=(n_random, 3))
+    v = c[:, None, :] + rng.normal(scale=0.08, size=(n_random, 3, 3))
+    parts.append(v.reshape(-1, 9).astype(np.float32))
+    return np.concatenate(parts, 0).astype(np.float32)

@@ -1,0 +1,257 @@
+"""Python bindings (ctypes) of the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py import this module; the product
+(kazen_proto_amd) never does. The C++ sources next to this file restate the
+reference algorithm and cite it file:line.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_p = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("dist", ctypes.c_int32),
+                ("albedo", ctypes.c_float * 3), ("F0", ctypes.c_float * 3),
+                ("alpha", ctypes.c_float), ("eta", ctypes.c_float),
+                ("Le", ctypes.c_float * 3), ("pad", ctypes.c_float)]
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [("width", _u32), ("height", _u32), ("spp", _u32), ("max_depth", _u32),
+                ("rr_depth", _u32), ("seed", _u32), ("spp_offset", _u32), ("flags", _u32),
+                ("eye", ctypes.c_float * 3), ("lookat", ctypes.c_float * 3),
+                ("up", ctypes.c_float * 3), ("tan_half_fov", ctypes.c_float)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "or_pcg32_u32": (None, [_u64, _u64, _p, _i32]),
+            "or_pcg32_f32": (None, [_u64, _u64, _p, _i32]),
+            "or_intersect_triangle": (_i32, [_p, _p, _p, _p, _p, _p, _p]),
+            "or_closest_bruteforce": (None, [_p, _u32, _p, _u32, _p, _i32]),
+            "or_count_near_ties": (_u32, [_p, _u32, _p, _u32, _i32]),
+            "or_bvh_build": (_p, [_p, _u32]),
+            "or_bvh_free": (None, [_p]),
+            "or_bvh_node_count": (_u32, [_p]),
+            "or_bvh_closest": (None, [_p, _p, _u32, _p, _i32]),
+            "or_bvh_occluded": (None, [_p, _p, _u32, _p, _i32]),
+            "or_bvh_closest_timed": (ctypes.c_double, [_p, _p, _u32, _p, _i32]),
+            "or_hardware_threads": (_i32, []),
+            "or_bvh8_closest": (None, [_p, _u64, _p, _u64, _p, _u32, _p, _p, _p, _i32]),
+            "or_bsdf_eval": (None, [_p, _p, _p, _p, _p]),
+            "or_bsdf_sample": (_i32, [_p, _p, _p, _p, _p, _p]),
+            "or_mf_D": (ctypes.c_double, [_i32, ctypes.c_float, _p]),
+            "or_mf_lambda": (ctypes.c_double, [_i32, ctypes.c_float, _p]),
+            "or_mf_sample_vndf": (None, [_i32, ctypes.c_float, _p, _p, _p]),
+            "or_fresnel_dielectric": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+            "or_erfinv": (ctypes.c_double, [ctypes.c_double]),
+            "or_scene_create": (_p, [_p, _u32, _p, _p, _p, _u32]),
+            "or_scene_free": (None, [_p]),
+            "or_render": (_i32, [_p, _p, _p, _p, _i32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def pcg32_u32(initstate, initseq, n):
+    out = np.zeros(n, np.uint32)
+    lib().or_pcg32_u32(initstate, initseq, _ptr(out), n)
+    return out
+
+
+def pcg32_f32(initstate, initseq, n):
+    out = np.zeros(n, np.float32)
+    lib().or_pcg32_f32(initstate, initseq, _ptr(out), n)
+    return out
+
+
+def intersect_triangle(ray8, v0, v1, v2):
+    ray8 = np.ascontiguousarray(ray8, np.float32)
+    v = [np.ascontiguousarray(x, np.float32) for x in (v0, v1, v2)]
+    t = ctypes.c_float()
+    u = ctypes.c_float()
+    w = ctypes.c_float()
+    hit = lib().or_intersect_triangle(_ptr(ray8), _ptr(v[0]), _ptr(v[1]), _ptr(v[2]),
+                                      ctypes.byref(t), ctypes.byref(u), ctypes.byref(w))
+    return bool(hit), t.value, u.value, w.value
+
+
+def _hits_view(h):
+    """float32 [n,4] hit array -> (t, prim, u, v)."""
+    return h[:, 0], h[:, 1].view(np.uint32), h[:, 2], h[:, 3]
+
+
+def closest_bruteforce(tris, rays, threads=0):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    hits = np.zeros((rays.shape[0], 4), np.float32)
+    lib().or_closest_bruteforce(_ptr(tris), tris.shape[0], _ptr(rays), rays.shape[0], _ptr(hits),
+                                threads)
+    return hits
+
+
+def count_near_ties(tris, rays, threads=0):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    return int(lib().or_count_near_ties(_ptr(tris), tris.shape[0], _ptr(rays), rays.shape[0],
+                                        threads))
+
+
+class Bvh:
+    """The oracle's own binary SAH BVH (independent of the product's BVH8)."""
+
+    def __init__(self, tris):
+        self._tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self._h = lib().or_bvh_build(_ptr(self._tris), self._tris.shape[0])
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_bvh_free(self._h)
+            self._h = None
+
+    @property
+    def node_count(self):
+        return int(lib().or_bvh_node_count(self._h))
+
+    def closest(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        lib().or_bvh_closest(self._h, _ptr(rays), rays.shape[0], _ptr(hits), threads)
+        return hits
+
+    def closest_timed(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        sec = lib().or_bvh_closest_timed(self._h, _ptr(rays), rays.shape[0], _ptr(hits), threads)
+        return hits, sec
+
+    def occluded(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint8)
+        lib().or_bvh_occluded(self._h, _ptr(rays), rays.shape[0], _ptr(out), threads)
+        return out
+
+
+def hardware_threads():
+    return int(lib().or_hardware_threads())
+
+
+def bvh8_closest(nodes, tris12, rays, threads=0):
+    """Traverse the product's BVH8 records on the host (exact decode)."""
+    nodes = np.ascontiguousarray(nodes, np.uint8)
+    tris12 = np.ascontiguousarray(tris12, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    hits = np.zeros((rays.shape[0], 4), np.float32)
+    nv = ctypes.c_uint64()
+    nt = ctypes.c_uint64()
+    lib().or_bvh8_closest(_ptr(nodes), nodes.size // 128, _ptr(tris12), tris12.size // 12,
+                          _ptr(rays), rays.shape[0], _ptr(hits), ctypes.byref(nv),
+                          ctypes.byref(nt), threads)
+    return hits, int(nv.value), int(nt.value)
+
+
+def material(type_, dist=0, albedo=(0, 0, 0), F0=(0, 0, 0), alpha=0.5, eta=1.5, Le=(0, 0, 0)):
+    m = Material()
+    m.type = type_
+    m.dist = dist
+    m.albedo[:] = albedo
+    m.F0[:] = F0
+    m.alpha = alpha
+    m.eta = eta
+    m.Le[:] = Le
+    return m
+
+
+def bsdf_eval(mat, wo, wi):
+    wo = np.ascontiguousarray(wo, np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = ctypes.c_float()
+    lib().or_bsdf_eval(ctypes.byref(mat), _ptr(wo), _ptr(wi), _ptr(f), ctypes.byref(pdf))
+    return f, pdf.value
+
+
+def bsdf_sample(mat, wo, u3):
+    wo = np.ascontiguousarray(wo, np.float32)
+    u3 = np.ascontiguousarray(u3, np.float32)
+    wi = np.zeros(3, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = ctypes.c_float()
+    ok = lib().or_bsdf_sample(ctypes.byref(mat), _ptr(wo), _ptr(u3), _ptr(wi), _ptr(f),
+                              ctypes.byref(pdf))
+    return bool(ok), wi, f, pdf.value
+
+
+def mf_D(dist, alpha, m):
+    return lib().or_mf_D(dist, alpha, _ptr(np.ascontiguousarray(m, np.float32)))
+
+
+def mf_lambda(dist, alpha, w):
+    return lib().or_mf_lambda(dist, alpha, _ptr(np.ascontiguousarray(w, np.float32)))
+
+
+def mf_sample_vndf(dist, alpha, wo, u2):
+    m = np.zeros(3, np.float32)
+    lib().or_mf_sample_vndf(dist, alpha, _ptr(np.ascontiguousarray(wo, np.float32)),
+                            _ptr(np.ascontiguousarray(u2, np.float32)), _ptr(m))
+    return m
+
+
+def fresnel_dielectric(cos_i, eta):
+    return lib().or_fresnel_dielectric(cos_i, eta)
+
+
+def erfinv(x):
+    return lib().or_erfinv(x)
+
+
+class Scene:
+    """Oracle scene: triangle soup, per-triangle material and light ids."""
+
+    def __init__(self, tris, tri_material, tri_light, materials):
+        self._tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self._tm = np.ascontiguousarray(tri_material, np.int32)
+        self._tl = np.ascontiguousarray(tri_light, np.int32)
+        arr = (Material * len(materials))(*materials)
+        self._mats = arr
+        self._h = lib().or_scene_create(_ptr(self._tris), self._tris.shape[0], _ptr(self._tm),
+                                        _ptr(self._tl), ctypes.cast(arr, ctypes.c_void_p),
+                                        len(materials))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_scene_free(self._h)
+            self._h = None
+
+    def render(self, desc, threads=0, want_sample0=False):
+        out = np.zeros((desc.height, desc.width, 3), np.float64)
+        s0 = np.zeros((desc.height, desc.width, 3), np.float32) if want_sample0 else None
+        rc = lib().or_render(self._h, ctypes.byref(desc), _ptr(out),
+                             _ptr(s0) if s0 is not None else None, threads)
+        if rc != 0:
+            raise RuntimeError("or_render failed (%d)" % rc)
+        return (out, s0) if want_sample0 else out

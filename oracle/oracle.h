@@ -1,0 +1,142 @@
+/*
+ * CPU oracle for the kazen_proto path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP library
+ * is compared against. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it. The product (libkp_hip.so) never
+ * links, loads or calls it.
+ *
+ * It restates, in plain C++ that runs on host cores, the reference's
+ * intended algorithm for:
+ *   - rays / closest-hit / occlusion   (src/core/ray.h:10-45, hitable.h:12-33,
+ *                                        accel.cpp:348-432 - Embree robust mode)
+ *   - the PCG32 sampler                 (src/core/sampler.cpp:14-60 via pcg32.h)
+ *   - closures                          (src/shading/bsdfs.cpp, microfacet.h, fresnel.h)
+ *   - the path integrator               (src/core/integrator.cpp:362-561, light.cpp:52-85)
+ * with the reference's defects replaced by the intended math (each deviation
+ * is cited next to the code that implements it).
+ *
+ * All entry points are extern "C" with plain pointers so that Python (ctypes)
+ * can drive them.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RNG ---------------------------------------------------------------- */
+/* Fill out[n] with successive pcg32 outputs of the stream seeded with
+ * (initstate, initseq). */
+void or_pcg32_u32(uint64_t initstate, uint64_t initseq, uint32_t* out, int n);
+/* Same stream, mapped to [0,1) floats with 23 random mantissa bits. */
+void or_pcg32_f32(uint64_t initstate, uint64_t initseq, float* out, int n);
+
+/* ---- Geometry ----------------------------------------------------------- */
+/* Ray record: {ox,oy,oz,tmin,dx,dy,dz,tmax}, 32 bytes. Hit record:
+ * {t, prim, u, v}, 16 bytes; prim = 0xFFFFFFFF on a miss; u,v = barycentric
+ * weights of vertices 1 and 2 (Embree convention). */
+
+/* Single watertight ray-triangle test. Returns 1 on hit with t in
+ * [tmin, tmax]; writes t,u,v. */
+int or_intersect_triangle(const float* ray8, const float* v0, const float* v1,
+                          const float* v2, float* t, float* u, float* v);
+
+/* Brute-force closest hit over all triangles (O(R*N)), threads = 0 -> all
+ * cores. tris: n_tris*9 floats (v0,v1,v2). */
+void or_closest_bruteforce(const float* tris, uint32_t n_tris, const float* rays,
+                           uint32_t n_rays, float* hits, int threads);
+
+/* Count rays whose closest hit has a second candidate (another primitive)
+ * with t within 1 ulp of the closest t.  Brute force. */
+uint32_t or_count_near_ties(const float* tris, uint32_t n_tris, const float* rays,
+                            uint32_t n_rays, int threads);
+
+/* Oracle BVH (binary, binned SAH, independent of the product's BVH). */
+typedef struct or_bvh or_bvh;
+or_bvh* or_bvh_build(const float* tris, uint32_t n_tris);
+void or_bvh_free(or_bvh* b);
+uint32_t or_bvh_node_count(const or_bvh* b);
+/* Closest hit through the oracle BVH, threaded over rays. */
+void or_bvh_closest(const or_bvh* b, const float* rays, uint32_t n_rays,
+                    float* hits, int threads);
+/* Occlusion (any hit in [tmin,tmax]) -> out[i] in {0,1}. */
+void or_bvh_occluded(const or_bvh* b, const float* rays, uint32_t n_rays,
+                     uint8_t* out, int threads);
+/* Wall-clock seconds for or_bvh_closest on the given rays (timed here so the
+ * measurement excludes Python overhead). */
+double or_bvh_closest_timed(const or_bvh* b, const float* rays, uint32_t n_rays,
+                            float* hits, int threads);
+int or_hardware_threads(void);
+
+/* Closest hit through the HIP library's BVH8 records (128-byte nodes, 48-byte
+ * triangles, layout documented in kazen_proto_amd/csrc/kp_layout.h), decoded
+ * in exact double arithmetic and traversed on the host. Used to check the
+ * builder on the CPU and to count the nodes visited / triangles tested per
+ * ray (counts may be NULL). */
+void or_bvh8_closest(const void* nodes, uint64_t n_nodes, const void* tris,
+                     uint64_t n_tris, const float* rays, uint32_t n_rays,
+                     float* hits, uint64_t* nodes_visited, uint64_t* tris_tested,
+                     int threads);
+
+/* ---- Shading ------------------------------------------------------------ */
+/* Material record, shared layout with the scene description:
+ *   type: 0 Lambert, 1 conductor (Schlick F0), 2 rough dielectric
+ *   dist: 0 GGX, 1 Beckmann
+ *   params: albedo[3] | F0[3] | alpha | eta | Le[3]                       */
+typedef struct {
+    int32_t type;
+    int32_t dist;
+    float albedo[3];
+    float F0[3];
+    float alpha;
+    float eta;
+    float Le[3];
+    float pad;
+} or_material;   /* 64 bytes */
+
+/* BSDF evaluation in the y-up local frame (cos theta = w.y). wo points
+ * toward the previous vertex, wi toward the next one.  Writes f[3] (without
+ * the cosine) and the solid-angle pdf of sampling wi given wo. */
+void or_bsdf_eval(const or_material* m, const float* wo, const float* wi,
+                  float* f, float* pdf);
+/* Sample wi given wo and 3 uniforms; writes wi[3], f[3], pdf. Returns 0 when
+ * the sample is invalid. */
+int or_bsdf_sample(const or_material* m, const float* wo, const float* u3,
+                   float* wi, float* f, float* pdf);
+/* Microfacet pieces, for the closed-form tests. */
+double or_mf_D(int dist, float alpha, const float* m);
+double or_mf_lambda(int dist, float alpha, const float* w);
+void or_mf_sample_vndf(int dist, float alpha, const float* wo, const float* u2,
+                       float* m);
+double or_fresnel_dielectric(double cos_i, double eta);
+double or_erfinv(double x);
+
+/* ---- Path integrator ---------------------------------------------------- */
+typedef struct {
+    uint32_t width, height, spp, max_depth, rr_depth;
+    uint32_t seed;
+    uint32_t spp_offset;   /* first sample index (for resumable renders) */
+    uint32_t flags;        /* bit0: write per-sample radiance of sample 0 */
+    float eye[3], lookat[3], up[3];
+    float tan_half_fov;    /* tan(vfov/2), computed once by the caller */
+} or_render_desc;
+
+/* Scene for rendering: world-space triangle soup + per-triangle material id.
+ * Lights are the triangles whose material has Le != 0, grouped by light_id
+ * (-1 = not a light); each light group is one area light sampled uniformly
+ * by area (reference GeometryLight / TriangleMesh::sample). */
+typedef struct or_scene or_scene;
+or_scene* or_scene_create(const float* tris, uint32_t n_tris,
+                          const int32_t* tri_material, const int32_t* tri_light,
+                          const or_material* materials, uint32_t n_materials);
+void or_scene_free(or_scene* s);
+/* FP64 render: out[h*w*3] = mean radiance per pixel. sample0 (optional,
+ * may be NULL): radiance of the first sample of every pixel, FP32. */
+int or_render(const or_scene* s, const or_render_desc* d, double* out,
+              float* sample0, int threads);
+
+#ifdef __cplusplus
+}
+#endif
