@@ -41,7 +41,6 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
 
-    import torch
     from kazen_proto_amd import native, scenes
 
     t0 = time.time()
@@ -52,11 +51,7 @@ def main():
                                                          inc.shape[0], gen_s),
           file=sys.stderr, flush=True)
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    stream = torch.cuda.current_stream()
     ctx = native.Context(0)
-    ctx.set_stream(stream.cuda_stream)
     ctx.set_trace_tuning(args.refill)
     ctx.mesh_add_soup(tris)
     ctx.bvh_build(leaf_size=args.leaf)
@@ -65,10 +60,11 @@ def main():
                                                           st["build_seconds"]),
           file=sys.stderr, flush=True)
 
-    d_inc = torch.from_numpy(inc).to(dev)
-    d_prim = torch.from_numpy(prim).to(dev)
-    h_inc = torch.empty((inc.shape[0], 4), dtype=torch.float32, device=dev)
-    h_prim = torch.empty((prim.shape[0], 4), dtype=torch.float32, device=dev)
+    # inputs resident in HBM before the timed region
+    d_inc = native.DeviceBuffer.from_array(ctx, inc)
+    d_prim = native.DeviceBuffer.from_array(ctx, prim)
+    h_inc = native.DeviceBuffer(ctx, (inc.shape[0], 4), np.float32)
+    h_prim = native.DeviceBuffer(ctx, (prim.shape[0], 4), np.float32)
 
     def step(r, h):
         ctx.intersect_closest(r, out=h)
@@ -79,16 +75,14 @@ def main():
     ctx.synchronize()
 
     def timed(r, h, k):
-        torch.cuda.synchronize()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
+        # one process, one GPU: the barrier is the stream synchronisation
+        ctx.synchronize()
+        ctx.timer_begin()           # HIP event on the launch stream
         for _ in range(k):
             step(r, h)
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        ctx.synchronize()
-        return ev0.elapsed_time(ev1) / k
+        ms = ctx.timer_end()        # HIP event, waits for the stream
+        ctx.synchronize()           # also surfaces kernel-reported errors
+        return ms / k
 
     ms_inc = timed(d_inc, h_inc, args.steps)
     ms_prim = timed(d_prim, h_prim, max(1, args.steps // 2))
@@ -125,7 +119,7 @@ def main():
                "kind": "port",
                "sample": "all %d incoherent rays, oracle binary-SAH BVH + watertight test, "
                          "median of 3 runs (%.2fs each)" % (inc.shape[0], sec)}
-        got = h_inc.cpu().numpy()
+        got = h_inc.numpy()
         mm = int(np.count_nonzero(np.any(got.view(np.uint32) != hits_cpu.view(np.uint32), axis=1)))
         parity = {"incoherent_mismatches": mm, "rays": int(n)}
 

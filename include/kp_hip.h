@@ -142,6 +142,19 @@ void kp_bvh8_blob_free(kp_bvh8_blob* b);
  * busy (default 40). */
 int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
 
+/* Device memory and timing helpers for callers without their own HIP
+ * runtime handle (the reference has no counterpart: its callers never see
+ * device memory). Buffers belong to the caller and must be freed with
+ * kp_device_free before kp_destroy. Copies are synchronous. */
+int kp_device_alloc(kp_ctx* ctx, uint64_t bytes, void** ptr);
+int kp_device_free(kp_ctx* ctx, void* ptr);
+int kp_copy_to_device(kp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int kp_copy_to_host(kp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* HIP events on the context stream: kp_timer_end returns the milliseconds
+ * between the two events (waits for the end event). */
+int kp_timer_begin(kp_ctx* ctx);
+int kp_timer_end(kp_ctx* ctx, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

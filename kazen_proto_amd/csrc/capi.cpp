@@ -135,6 +135,8 @@ void kp_destroy(kp_ctx* c) {
     if (c->d_counters) (void)hipFree(c->d_counters);
     if (c->d_stage_in) (void)hipFree(c->d_stage_in);
     if (c->d_stage_out) (void)hipFree(c->d_stage_out);
+    if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
+    if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -344,6 +346,59 @@ int kp_bvh8_blob_info(const kp_bvh8_blob* b, uint64_t* nn, uint64_t* nt, uint32_
 const void* kp_bvh8_blob_nodes(const kp_bvh8_blob* b) { return b ? b->h.nodes.data() : nullptr; }
 const void* kp_bvh8_blob_tris(const kp_bvh8_blob* b) { return b ? b->h.tris.data() : nullptr; }
 void kp_bvh8_blob_free(kp_bvh8_blob* b) { delete b; }
+
+int kp_device_alloc(kp_ctx* c, uint64_t bytes, void** ptr) {
+    if (!c || !ptr) return KP_ERR_INVALID;
+    *ptr = nullptr;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipMalloc(ptr, bytes ? bytes : 1));
+    return KP_OK;
+}
+
+int kp_device_free(kp_ctx* c, void* ptr) {
+    if (!c) return KP_ERR_INVALID;
+    if (!ptr) return KP_OK;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    KP_HIP_TRY(c, hipFree(ptr));
+    return KP_OK;
+}
+
+int kp_copy_to_device(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (!dst && bytes) || (!src && bytes)) return KP_ERR_INVALID;
+    if (!bytes) return KP_OK;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return KP_OK;
+}
+
+int kp_copy_to_host(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (!dst && bytes) || (!src && bytes)) return KP_ERR_INVALID;
+    if (!bytes) return KP_OK;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return KP_OK;
+}
+
+int kp_timer_begin(kp_ctx* c) {
+    if (!c) return KP_ERR_INVALID;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->ev_begin) KP_HIP_TRY(c, hipEventCreate(&c->ev_begin));
+    if (!c->ev_end) KP_HIP_TRY(c, hipEventCreate(&c->ev_end));
+    KP_HIP_TRY(c, hipEventRecord(c->ev_begin, c->stream));
+    return KP_OK;
+}
+
+int kp_timer_end(kp_ctx* c, float* ms) {
+    if (!c || !ms) return KP_ERR_INVALID;
+    if (!c->ev_begin) return kp::fail(c, KP_ERR_STATE, "kp_timer_end without kp_timer_begin");
+    KP_HIP_TRY(c, hipEventRecord(c->ev_end, c->stream));
+    KP_HIP_TRY(c, hipEventSynchronize(c->ev_end));
+    KP_HIP_TRY(c, hipEventElapsedTime(ms, c->ev_begin, c->ev_end));
+    return KP_OK;
+}
 
 int kp_set_trace_tuning(kp_ctx* c, uint32_t lanes) {
     if (!c) return KP_ERR_INVALID;

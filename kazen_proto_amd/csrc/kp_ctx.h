@@ -13,6 +13,7 @@ struct kp_ctx {
     int cus = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::string err;
 
     // scene (host copy, world space)

@@ -2,8 +2,10 @@
 
 The library is loaded from this package directory; if it is missing the
 import of any entry point raises (there is no CPU fallback).  Arrays may be
-numpy (host memory, KP_PTR_HOST) or torch CUDA tensors (device memory of the
-context's device, KP_PTR_DEVICE).
+numpy (host memory, KP_PTR_HOST) or DeviceBuffer objects (device memory
+allocated through the library, KP_PTR_DEVICE).  The binding deliberately does
+not use torch: torch wheels bundle their own HIP runtime, and two HIP
+runtimes in one process do not share devices.
 """
 import ctypes
 import os
@@ -63,6 +65,12 @@ SIGNATURES = {
     "kp_bvh8_blob_tris": (_p, [_p]),
     "kp_bvh8_blob_free": (None, [_p]),
     "kp_set_trace_tuning": (_i32, [_p, _u32]),
+    "kp_device_alloc": (_i32, [_p, _u64, ctypes.POINTER(_p)]),
+    "kp_device_free": (_i32, [_p, _p]),
+    "kp_copy_to_device": (_i32, [_p, _p, _p, _u64]),
+    "kp_copy_to_host": (_i32, [_p, _p, _p, _u64]),
+    "kp_timer_begin": (_i32, [_p]),
+    "kp_timer_end": (_i32, [_p, ctypes.POINTER(ctypes.c_float)]),
 }
 
 
@@ -85,8 +93,49 @@ def _np_ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def _is_torch(x):
-    return type(x).__module__.startswith("torch")
+class DeviceBuffer:
+    """Device memory owned by the caller, allocated through a Context."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._p = _p()
+        ctx._check(lib().kp_device_alloc(ctx.handle, self.nbytes, ctypes.byref(self._p)))
+
+    @classmethod
+    def from_array(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.shape, a.dtype)
+        b.upload(a)
+        return b
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        self.ctx._check(lib().kp_copy_to_device(self.ctx.handle, self._p, _np_ptr(a), self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(lib().kp_copy_to_host(self.ctx.handle, _np_ptr(out), self._p, self.nbytes))
+        return out
+
+    def free(self):
+        if self._p and self.ctx.handle:
+            self.ctx._check(lib().kp_device_free(self.ctx.handle, self._p))
+        self._p = _p()
+
+    def __del__(self):
+        try:
+            if self._p and self.ctx.handle:
+                lib().kp_device_free(self.ctx.handle, self._p)
+        except Exception:
+            pass
 
 
 def bvh8_build_host(tris9, leaf_size=4, sah_bins=32):
@@ -177,15 +226,14 @@ class Context:
         self._check(lib().kp_set_trace_tuning(self._h, refill_below_lanes))
 
     def intersect_closest(self, rays, out=None, counted=False):
-        """rays: [n,8] float32 (numpy or torch CUDA). Returns [n,4] float32
-        hits {t, prim bits, u, v} of the same kind (and counts if counted)."""
-        if _is_torch(rays):
-            import torch
-            assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous()
-            n = rays.numel() // 8
+        """rays: [n,8] float32 numpy array or DeviceBuffer.  Returns [n,4]
+        float32 hits {t, prim bits, u, v} of the same kind (plus the node and
+        triangle counts when counted=True)."""
+        if isinstance(rays, DeviceBuffer):
+            n = rays.nbytes // 32
             if out is None:
-                out = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
-            rp, hp, kind = _p(rays.data_ptr()), _p(out.data_ptr()), KP_PTR_DEVICE
+                out = DeviceBuffer(self, (n, 4), np.float32)
+            rp, hp, kind = rays.ptr, out.ptr, KP_PTR_DEVICE
         else:
             rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
             n = rays.shape[0]
@@ -201,13 +249,11 @@ class Context:
         return out
 
     def occluded(self, rays, out=None):
-        if _is_torch(rays):
-            import torch
-            n = rays.numel() // 8
+        if isinstance(rays, DeviceBuffer):
+            n = rays.nbytes // 32
             if out is None:
-                out = torch.empty((n,), dtype=torch.int32, device=rays.device)
-            self._check(lib().kp_occluded(self._h, _p(rays.data_ptr()), n, _p(out.data_ptr()),
-                                          KP_PTR_DEVICE))
+                out = DeviceBuffer(self, (n,), np.uint32)
+            self._check(lib().kp_occluded(self._h, rays.ptr, n, out.ptr, KP_PTR_DEVICE))
             return out
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         n = rays.shape[0]
@@ -215,6 +261,14 @@ class Context:
             out = np.zeros(n, np.uint32)
         self._check(lib().kp_occluded(self._h, _np_ptr(rays), n, _np_ptr(out), KP_PTR_HOST))
         return out
+
+    def timer_begin(self):
+        self._check(lib().kp_timer_begin(self._h))
+
+    def timer_end(self):
+        ms = ctypes.c_float()
+        self._check(lib().kp_timer_end(self._h, ctypes.byref(ms)))
+        return float(ms.value)
 
 
 def hit_fields(hits):

@@ -41,15 +41,18 @@ def test_golden_fixture_host_pointers(leaf):
 
 
 def test_golden_fixture_device_pointers():
-    import torch
     g = golden()
     c = make_ctx(g["tris"])
-    c.set_stream(torch.cuda.current_stream().cuda_stream)
-    rays = torch.from_numpy(g["rays"]).cuda()
+    rays = native.DeviceBuffer.from_array(c, g["rays"])
     h = c.intersect_closest(rays)
     c.synchronize()
-    assert mismatches(h.cpu().numpy(), g["hits"]) == 0
-    c.set_stream(None)
+    assert mismatches(h.numpy(), g["hits"]) == 0
+    occ = c.occluded(rays)
+    c.synchronize()
+    assert occ.numpy().shape == (g["rays"].shape[0],)
+    rays.free()
+    h.free()
+    occ.free()
     c.close()
 
 
