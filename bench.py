@@ -35,6 +35,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--refill", type=int, default=40)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one device copy kernel of MIB MiB (reference kernel of "
+                         "known traffic for HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -52,6 +55,16 @@ def main():
           file=sys.stderr, flush=True)
 
     ctx = native.Context(0)
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        a = native.DeviceBuffer(ctx, (nb // 4,), np.float32)
+        b = native.DeviceBuffer(ctx, (nb // 4,), np.float32)
+        ctx.device_copy(b, a)
+        ctx.synchronize()
+        print("[bench] copy calibration kernel: %d bytes read, %d written" % (nb, nb),
+              file=sys.stderr, flush=True)
+        a.free()
+        b.free()
     ctx.set_trace_tuning(args.refill)
     ctx.mesh_add_soup(tris)
     ctx.bvh_build(leaf_size=args.leaf)

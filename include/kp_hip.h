@@ -142,6 +142,70 @@ void kp_bvh8_blob_free(kp_bvh8_blob* b);
  * busy (default 40). */
 int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
 
+/* Integrator -------------------------------------------------------------- */
+#define KP_MAT_LAMBERT 0      /* Diffuse (src/shading/bsdfs.cpp:27-41) */
+#define KP_MAT_CONDUCTOR 1    /* microfacet reflection, Schlick F0
+                                 (KpPrincipleSpecularReflection, bsdfs.cpp:688-730) */
+#define KP_MAT_DIELECTRIC 2   /* rough dielectric (KpGlass, bsdfs.cpp:441-554) */
+#define KP_DIST_GGX 0
+#define KP_DIST_BECKMANN 1
+
+/* Closure parameters of one material (64 bytes). Stands in for the OSL
+ * shader group of a mesh (the reference maps OSL closures to these lobes in
+ * process_closure, src/shading/bsdf.cpp:184-328). Le != 0 makes every mesh
+ * using the material a one-sided area light (GeometryLight,
+ * src/core/light.cpp:52-85), emitting on the side its winding normal
+ * (v1-v0)x(v2-v0) points to. */
+typedef struct {
+    int32_t type;        /* KP_MAT_* */
+    int32_t dist;        /* KP_DIST_* (microfacet closures) */
+    float albedo[3];     /* Lambert reflectance */
+    float F0[3];         /* conductor normal-incidence reflectance */
+    float alpha;         /* microfacet roughness (> 0) */
+    float eta;           /* dielectric n_inside / n_outside */
+    float Le[3];         /* emitted radiance */
+    float pad;
+} kp_material;
+
+/* One render: width x height pixels, spp samples per pixel starting at sample
+ * index spp_offset (so renders resume), paths of up to max_depth scattering
+ * vertices, Russian roulette from vertex rr_depth, pinhole camera with
+ * tan(vertical fov / 2). Replaces the Film/Camera/Renderer setup of the
+ * reference (src/core/renderer.cpp:18-90, camera.h:50-96). */
+typedef struct {
+    uint32_t width, height, spp, max_depth, rr_depth;
+    uint32_t seed;
+    uint32_t spp_offset;
+    uint32_t flags;      /* reserved, 0 */
+    float eye[3], lookat[3], up[3];
+    float tan_half_fov;
+} kp_render_desc;
+
+typedef struct {
+    uint64_t paths;           /* camera paths traced by the last accumulate */
+    uint64_t extension_rays;  /* closest-hit rays */
+    uint64_t shadow_rays;     /* occlusion rays */
+    double seconds;           /* wall time of the last accumulate */
+    uint32_t passes;
+    uint32_t paths_per_pass;
+    uint32_t reserved[6];
+} kp_render_stats;
+
+int kp_material_add(kp_ctx* ctx, const kp_material* m, uint32_t* material_id);
+/* Assigns a material to a mesh (default 0). */
+int kp_mesh_set_material(kp_ctx* ctx, uint32_t mesh_id, uint32_t material_id);
+/* Film: FP32 RGB sum + Kahan compensation per pixel, zeroed here. */
+int kp_film_reset(kp_ctx* ctx, uint32_t width, uint32_t height);
+/* Traces desc->spp samples per pixel into the film (wavefront path
+ * integrator, replacing PathIntegrator::Li, src/core/integrator.cpp:362-561,
+ * and the never-defined Renderer::render(SceneGPU&) of renderer.h:30-31). */
+int kp_render_accumulate(kp_ctx* ctx, const kp_render_desc* desc);
+/* Mean radiance per pixel (row-major, top row first, RGB float). */
+int kp_film_read(kp_ctx* ctx, float* rgb, int ptr_kind);
+/* kp_film_reset + kp_render_accumulate + kp_film_read. */
+int kp_render(kp_ctx* ctx, const kp_render_desc* desc, float* rgb, int ptr_kind);
+int kp_render_get_stats(kp_ctx* ctx, kp_render_stats* out);
+
 /* Device memory and timing helpers for callers without their own HIP
  * runtime handle (the reference has no counterpart: its callers never see
  * device memory). Buffers belong to the caller and must be freed with
@@ -150,6 +214,10 @@ int kp_device_alloc(kp_ctx* ctx, uint64_t bytes, void** ptr);
 int kp_device_free(kp_ctx* ctx, void* ptr);
 int kp_copy_to_device(kp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int kp_copy_to_host(kp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte vector loads and
+ * stores), asynchronous on the context stream. Also serves as the known-size
+ * reference kernel when calibrating HBM traffic counters. bytes % 16 == 0. */
+int kp_device_copy(kp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 /* HIP events on the context stream: kp_timer_end returns the milliseconds
  * between the two events (waits for the end event). */
 int kp_timer_begin(kp_ctx* ctx);

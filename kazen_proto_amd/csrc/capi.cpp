@@ -232,6 +232,7 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     c->max_steps = 2u * (uint32_t)h.nodes.size() + n + 64u;
     c->build_seconds = std::chrono::duration<double>(t1 - t0).count();
     c->built = true;
+    kp::render_scene_dirty(c);
     return KP_OK;
 }
 
@@ -379,6 +380,15 @@ int kp_copy_to_host(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
     KP_HIP_TRY(c, hipSetDevice(c->device));
     KP_HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return KP_OK;
+}
+
+int kp_device_copy(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return KP_ERR_INVALID;
+    if (bytes % 16) return kp::fail(c, KP_ERR_INVALID, "kp_device_copy: bytes must be a multiple of 16");
+    if (!bytes) return KP_OK;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, kp::launch_copy16(dst, src, bytes / 16, c->cus, c->stream));
     return KP_OK;
 }
 

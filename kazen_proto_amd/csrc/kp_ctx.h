@@ -60,6 +60,8 @@ int ensure_buffer(kp_ctx* c, void** buf, size_t* cap, size_t bytes);
 int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, bool count);
 int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out);
 void render_state_free(kp_ctx* c);
+// Marks the integrator's device copy of the scene stale (meshes changed).
+void render_scene_dirty(kp_ctx* c);
 }  // namespace kp
 
 #define KP_HIP_TRY(ctx, expr)                                  \

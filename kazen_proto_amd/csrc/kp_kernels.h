@@ -16,6 +16,7 @@ struct TraceArgs {
     float4* hits;           // closest: {t, prim bits, u, v}
     uint32_t* occl;         // occluded: 0/1 per ray
     uint32_t n;             // ray count
+    const uint32_t* n_dev;  // if set, the ray count is read from device memory
     uint32_t* counter;      // work counter (reset by the launcher)
     uint2* spill;           // per-lane stack spill area
     uint32_t spill_cap;     // entries per lane
@@ -29,5 +30,6 @@ hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStre
 hipError_t launch_occluded(const TraceArgs& a, uint32_t grid, hipStream_t s);
 hipError_t trace_occupancy(int* blocks_per_cu_closest, int* blocks_per_cu_occluded);
 hipError_t launch_reset_u32(uint32_t* p, uint32_t n, hipStream_t s);
+hipError_t launch_copy16(void* dst, const void* src, uint64_t n16, int cus, hipStream_t s);
 
 }  // namespace kp

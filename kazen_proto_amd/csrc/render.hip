@@ -1,6 +1,768 @@
-// Wavefront path integrator (placeholder until the integrator lands).
+// Wavefront path integrator (gfx950).
+//
+// Replaces PathIntegrator::Li (reference src/core/integrator.cpp:362-561)
+// driven by Renderer::render's tile/pixel/sample loops (src/core/renderer.cpp:
+// 18-90), and defines the GPU render overload the reference declares but
+// never implements (src/core/renderer.h:30-31).
+//
+// One pass traces k samples of every pixel as n = width*height*k independent
+// paths (slot = pixel*k + j).  Per bounce:
+//   extend  : closest hit of the compacted ray queue (trace.hip)
+//   shade   : emission with MIS, next-event estimation (writes a shadow ray),
+//             Russian roulette, BSDF sampling (writes the next ray); both
+//             queues are compacted with one ballot + one atomic per wave
+//   connect : occlusion of the shadow queue, then the unoccluded
+//             contributions are added to their paths
+// After the last bounce every pixel folds its k path radiances into the film
+// (FP32 sum with Kahan compensation, one owner per pixel, fixed order), so
+// the film does not depend on scheduling.
+//
+// Sampling protocol of a path (shared with the oracle): PCG32 stream with
+// state ((sample << 32) ^ seed ^ 0x9E3779B97F4A7C15) and sequence = pixel
+// index; 2 draws for the pixel position, then 8 draws at every shaded vertex:
+// light pick, light triangle, two barycentric, roulette, lobe, two direction.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/kp_hip.h"
+#include "kp_closures.h"
 #include "kp_ctx.h"
 
 namespace kp {
-void render_state_free(kp_ctx*) {}
+
+struct LightRec {
+    uint32_t first;      // first entry in light_tris
+    uint32_t count;      // triangles
+    uint32_t cdf_first;  // first of count+1 CDF entries
+    uint32_t pad;
+    float area;
+    float Le[3];
+};
+
+struct PathState {
+    float4* rays[2];      // ray queues {o, tmin} {d, tmax}, compacted
+    uint32_t* qmap[2];    // queue entry -> path slot
+    float4* hits;         // closest hits of the current queue
+    float4* s0;           // per slot: beta.xyz, pdf of the last BSDF sample
+    float4* s1;           // per slot: radiance.xyz
+    uint64_t* rng;        // per slot: PCG32 state
+    float4* srays;        // shadow queue rays
+    float4* scontrib;     // shadow contribution.xyz, slot bits in w
+    uint32_t* socc;       // occlusion result per shadow ray
+    uint32_t* cnt;        // [0],[1] ray queue sizes, [2] shadow queue size
+    unsigned long long* totals;   // [0] extension rays, [1] shadow rays
+};
+
+struct RenderArgs {
+    const float4* stris;     // per primitive: {v0, mat}, {v1, light}, {v2, 0}
+    const Material* mats;
+    const LightRec* lights;
+    const uint32_t* light_tris;
+    const float* light_cdf;
+    uint32_t n_lights;
+    uint32_t width, height, npix, k, n_paths;
+    uint32_t seed, s0, max_depth, rr_depth;
+    float3 eye, fwd, right, upv;
+    float tan_half, aspect;
+    PathState ps;
+};
+
+// ------------------------------------------------------------------ RNG --
+struct Pcg {
+    uint64_t state, inc;
+    __device__ __forceinline__ uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    __device__ __forceinline__ float nextf() {
+        return __uint_as_float((next() >> 9) | 0x3f800000u) - 1.0f;
+    }
+};
+
+__device__ __forceinline__ Pcg pcg_seed(uint64_t initstate, uint64_t initseq) {
+    Pcg p;
+    p.state = 0u;
+    p.inc = (initseq << 1u) | 1u;
+    p.next();
+    p.state += initstate;
+    p.next();
+    return p;
+}
+
+__device__ __forceinline__ f3 ld3(const float4& v) { return mk(v.x, v.y, v.z); }
+
+// Waechter & Binder, Ray Tracing Gems ch. 6 (constants as in the reference's
+// intersection.h:78-80).
+__device__ __forceinline__ f3 offset_ray(f3 p, f3 n) {
+    const float origin = 1.0f / 32.0f, fscale = 1.0f / 65536.0f, iscale = 256.0f;
+    const int ox = (int)(iscale * n.x), oy = (int)(iscale * n.y), oz = (int)(iscale * n.z);
+    const f3 pi = mk(__int_as_float(__float_as_int(p.x) + (p.x < 0.0f ? -ox : ox)),
+                     __int_as_float(__float_as_int(p.y) + (p.y < 0.0f ? -oy : oy)),
+                     __int_as_float(__float_as_int(p.z) + (p.z < 0.0f ? -oz : oz)));
+    return mk(fabsf(p.x) < origin ? p.x + fscale * n.x : pi.x,
+              fabsf(p.y) < origin ? p.y + fscale * n.y : pi.y,
+              fabsf(p.z) < origin ? p.z + fscale * n.z : pi.z);
+}
+
+// DiscrectPDF::sample (src/base/dpdf.h:32-37): first entry >= u, minus one.
+__device__ __forceinline__ uint32_t cdf_sample(const float* cdf, uint32_t count, float u) {
+    uint32_t lo = 0, hi = count + 1;   // search cdf[0..count]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] < u) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t idx = lo > 0 ? lo - 1 : 0;
+    return idx < count - 1 ? idx : count - 1;
+}
+
+// ------------------------------------------------------------- kernels --
+__global__ __launch_bounds__(256) void k_generate(RenderArgs a) {
+#pragma clang fp contract(off)
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot == 0) {
+        a.ps.cnt[0] = a.n_paths;
+        a.ps.cnt[1] = 0;
+        a.ps.cnt[2] = 0;
+    }
+    if (slot >= a.n_paths) return;
+    const uint32_t pixel = slot / a.k;
+    const uint32_t j = slot - pixel * a.k;
+    const uint32_t sample = a.s0 + j;
+    const uint32_t px = pixel % a.width, py = pixel / a.width;
+    const uint64_t initstate = ((uint64_t)sample << 32) ^ (uint64_t)a.seed ^ 0x9E3779B97F4A7C15ull;
+    Pcg rng = pcg_seed(initstate, pixel);
+    const float ux = rng.nextf(), uy = rng.nextf();
+    const float sx = ((float)px + ux) / (float)a.width, sy = ((float)py + uy) / (float)a.height;
+    const float nx = (2.0f * sx - 1.0f) * a.tan_half * a.aspect;
+    const float ny = (1.0f - 2.0f * sy) * a.tan_half;
+    const f3 fwd = mk(a.fwd.x, a.fwd.y, a.fwd.z), right = mk(a.right.x, a.right.y, a.right.z);
+    const f3 upv = mk(a.upv.x, a.upv.y, a.upv.z);
+    const f3 d = normalize(fwd + right * nx + upv * ny);
+    a.ps.rays[0][2 * (size_t)slot] = make_float4(a.eye.x, a.eye.y, a.eye.z, 0.0f);
+    a.ps.rays[0][2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, __int_as_float(0x7f800000));
+    a.ps.qmap[0][slot] = slot;
+    a.ps.s0[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.ps.s1[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a.ps.rng[slot] = rng.state;
+}
+
+// Resets the output queue counters of the coming shade, accumulating totals.
+__global__ void k_bookkeep(RenderArgs a, int cur) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        a.ps.totals[0] += a.ps.cnt[cur];
+        a.ps.cnt[cur ^ 1] = 0;
+        a.ps.cnt[2] = 0;
+    }
+}
+
+__global__ void k_count_shadow(RenderArgs a) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) a.ps.totals[1] += a.ps.cnt[2];
+}
+
+// Appends lanes with `want` to a queue; returns the lane's position.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
+    const uint64_t m = __ballot(want);
+    if (!m) return 0u;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t depth) {
+    const uint32_t n = a.ps.cnt[cur];
+    const uint32_t total = gridDim.x * 256;
+    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t iters = (n + total - 1) / total;
+    const float4* __restrict__ rin = a.ps.rays[cur];
+    const uint32_t* __restrict__ qin = a.ps.qmap[cur];
+    float4* __restrict__ rout = a.ps.rays[cur ^ 1];
+    uint32_t* __restrict__ qout = a.ps.qmap[cur ^ 1];
+    const float inv_nl = a.n_lights ? 1.0f / (float)a.n_lights : 0.0f;
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t q = it * total + gid;
+        const bool active = q < n;
+        bool cont = false, shad = false;
+        float4 nr0 = make_float4(0, 0, 0, 0), nr1 = nr0, sr0 = nr0, sr1 = nr0, sc = nr0;
+        uint32_t slot = 0;
+        if (active) {
+            slot = qin[q];
+            const float4 hit = a.ps.hits[q];
+            const float4 r1 = rin[2 * (size_t)q + 1];
+            float4 st0 = a.ps.s0[slot];
+            float4 st1 = a.ps.s1[slot];
+            f3 beta = mk(st0.x, st0.y, st0.z);
+            const float prev_pdf = st0.w;
+            f3 L = mk(st1.x, st1.y, st1.z);
+            const uint32_t prim = __float_as_uint(hit.y);
+            const uint32_t pixel = slot / a.k;
+            Pcg rng;
+            rng.state = a.ps.rng[slot];
+            rng.inc = ((uint64_t)pixel << 1u) | 1u;
+            if (prim != 0xFFFFFFFFu) {
+                const float4 t0 = a.stris[3 * (size_t)prim], t1 = a.stris[3 * (size_t)prim + 1],
+                             t2 = a.stris[3 * (size_t)prim + 2];
+                const f3 v0 = ld3(t0), v1 = ld3(t1), v2 = ld3(t2);
+                const Material& m = a.mats[__float_as_uint(t0.w)];
+                const int lid = __float_as_int(t1.w);
+                const f3 ng = normalize(cross(v1 - v0, v2 - v0));
+                const float b0 = 1.0f - hit.z - hit.w;
+                const f3 P = v0 * b0 + v1 * hit.z + v2 * hit.w;
+                const f3 dir = mk(r1.x, r1.y, r1.z);
+                const f3 Le = mk(m.Le[0], m.Le[1], m.Le[2]);
+                const float cos_hit = dot(dir, ng);
+                if (!is_zero(Le) && cos_hit < 0.0f) {
+                    float w = 1.0f;
+                    if (depth > 1 && lid >= 0) {
+                        const LightRec lr = a.lights[lid];
+                        const float pl = inv_nl * (1.0f / lr.area) * (hit.x * hit.x) / (-cos_hit);
+                        w = prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
+                    }
+                    L = L + mul(beta, Le) * w;
+                }
+                if (depth <= a.max_depth) {
+                    float u[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) u[i] = rng.nextf();
+                    const f3 wo_w = -dir;
+                    const f3 nrm = (m.type == 2) ? ng : (dot(wo_w, ng) < 0.0f ? -ng : ng);
+                    const Frame fr(nrm);
+                    const f3 wo = fr.to_local(wo_w);
+                    // next-event estimation (GeometryLight::sample, light.cpp:52-77)
+                    if (a.n_lights > 0) {
+                        uint32_t li = (uint32_t)(u[0] * (float)a.n_lights);
+                        li = li < a.n_lights - 1 ? li : a.n_lights - 1;
+                        const LightRec lr = a.lights[li];
+                        const uint32_t lt = a.light_tris[lr.first + cdf_sample(a.light_cdf + lr.cdf_first, lr.count, u[1])];
+                        const f3 la = ld3(a.stris[3 * (size_t)lt]), lb = ld3(a.stris[3 * (size_t)lt + 1]),
+                                 lc = ld3(a.stris[3 * (size_t)lt + 2]);
+                        const float su0 = sqrtf(u[2]);
+                        const float bu = 1.0f - su0, bv = u[3] * su0;
+                        const f3 pl = la * bu + lb * bv + lc * (1.0f - bu - bv);
+                        const f3 nl_w = normalize(cross(lb - la, lc - la));
+                        const f3 org = offset_ray(P, dot(pl - P, ng) > 0.0f ? ng : -ng);
+                        const f3 dl = pl - org;
+                        const float dist = length(dl);
+                        if (dist > 0.0f) {
+                            const f3 wi_w = dl * (1.0f / dist);
+                            const float cos_l = -dot(wi_w, nl_w);
+                            if (cos_l > 0.0f) {
+                                const f3 wi = fr.to_local(wi_w);
+                                f3 f;
+                                float pdf_b;
+                                bsdf_eval(m, wo, wi, f, pdf_b);
+                                if (!is_zero(f)) {
+                                    const float pdf_l = inv_nl * (1.0f / lr.area) * (dist * dist) / cos_l;
+                                    const float w = pdf_l * pdf_l / (pdf_l * pdf_l + pdf_b * pdf_b);
+                                    const f3 c = mul(mul(beta, f), mk(lr.Le[0], lr.Le[1], lr.Le[2])) *
+                                                 (fabsf(wi.y) * w / pdf_l);
+                                    shad = true;
+                                    sr0 = make_float4(org.x, org.y, org.z, 0.0f);
+                                    sr1 = make_float4(wi_w.x, wi_w.y, wi_w.z, dist * 0.9999f);
+                                    sc = make_float4(c.x, c.y, c.z, __uint_as_float(slot));
+                                }
+                            }
+                        }
+                    }
+                    bool alive = true;
+                    // Russian roulette (integrator.cpp:491-497)
+                    if (depth >= a.rr_depth) {
+                        const float eta_rr = 0.95f;
+                        const float qs = fminf(maxc(beta) * eta_rr * eta_rr, 0.99f);
+                        if (!(u[4] < qs)) alive = false;
+                        else beta = beta * (1.0f / qs);
+                    }
+                    if (alive) {
+                        f3 wi, f;
+                        float pdf;
+                        if (bsdf_sample(m, wo, u[5], u[6], u[7], wi, f, pdf)) {
+                            beta = mul(beta, f) * (fabsf(wi.y) / pdf);
+                            if (!is_zero(beta)) {
+                                const f3 wi_w = fr.to_world(wi);
+                                const f3 o = offset_ray(P, dot(wi_w, ng) > 0.0f ? ng : -ng);
+                                nr0 = make_float4(o.x, o.y, o.z, 0.0f);
+                                nr1 = make_float4(wi_w.x, wi_w.y, wi_w.z, __int_as_float(0x7f800000));
+                                st0 = make_float4(beta.x, beta.y, beta.z, pdf);
+                                cont = true;
+                            }
+                        }
+                    }
+                }
+            }
+            if (cont) a.ps.s0[slot] = st0;
+            a.ps.s1[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            a.ps.rng[slot] = rng.state;
+        }
+        const uint32_t qo = wave_append(cont, &a.ps.cnt[cur ^ 1]);
+        if (cont) {
+            rout[2 * (size_t)qo] = nr0;
+            rout[2 * (size_t)qo + 1] = nr1;
+            qout[qo] = slot;
+        }
+        const uint32_t so = wave_append(shad, &a.ps.cnt[2]);
+        if (shad) {
+            a.ps.srays[2 * (size_t)so] = sr0;
+            a.ps.srays[2 * (size_t)so + 1] = sr1;
+            a.ps.scontrib[so] = sc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_connect(RenderArgs a) {
+    const uint32_t n = a.ps.cnt[2];
+    const uint32_t total = gridDim.x * 256;
+    for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < n; s += total) {
+        if (a.ps.socc[s]) continue;
+        const float4 c = a.ps.scontrib[s];
+        const uint32_t slot = __float_as_uint(c.w);
+        float4 l = a.ps.s1[slot];
+        l.x += c.x;
+        l.y += c.y;
+        l.z += c.z;
+        a.ps.s1[slot] = l;
+    }
+}
+
+// Folds the k path radiances of every pixel into the film, in sample order,
+// with Kahan compensation.
+__global__ __launch_bounds__(256) void k_film(RenderArgs a, float4* sum, float4* comp) {
+#pragma clang fp contract(off)
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.npix) return;
+    float4 s = sum[p], c = comp[p];
+    for (uint32_t j = 0; j < a.k; ++j) {
+        const float4 x = a.ps.s1[(size_t)p * a.k + j];
+        float y, t;
+        y = x.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
+        y = x.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
+        y = x.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
+    }
+    sum[p] = s;
+    comp[p] = c;
+}
+
+__global__ void k_film_resolve(const float4* sum, const float4* comp, uint32_t npix, float inv_count,
+                               float* out) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    const float4 s = sum[p], c = comp[p];
+    out[3 * (size_t)p] = (s.x - c.x) * inv_count;
+    out[3 * (size_t)p + 1] = (s.y - c.y) * inv_count;
+    out[3 * (size_t)p + 2] = (s.z - c.z) * inv_count;
+}
+
+__global__ void k_zero_f4(float4* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace kp
+
+// =================================================================== host ==
+struct RenderState {
+    bool scene_ready = false;
+    std::vector<kp::Material> mats;
+    // device scene
+    float4* stris = nullptr;
+    kp::Material* d_mats = nullptr;
+    kp::LightRec* d_lights = nullptr;
+    uint32_t* d_light_tris = nullptr;
+    float* d_light_cdf = nullptr;
+    uint32_t n_lights = 0;
+    // paths
+    uint32_t cap_paths = 0;
+    void* path_mem = nullptr;
+    kp::PathState ps{};
+    // film
+    uint32_t fw = 0, fh = 0;
+    uint64_t film_count = 0;
+    float4* film_sum = nullptr;
+    float4* film_comp = nullptr;
+    float* film_out = nullptr;
+    kp_render_stats stats{};
+};
+
+namespace kp {
+
+static void free_scene(RenderState* r) {
+    if (r->stris) (void)hipFree(r->stris);
+    if (r->d_mats) (void)hipFree(r->d_mats);
+    if (r->d_lights) (void)hipFree(r->d_lights);
+    if (r->d_light_tris) (void)hipFree(r->d_light_tris);
+    if (r->d_light_cdf) (void)hipFree(r->d_light_cdf);
+    r->stris = nullptr;
+    r->d_mats = nullptr;
+    r->d_lights = nullptr;
+    r->d_light_tris = nullptr;
+    r->d_light_cdf = nullptr;
+    r->scene_ready = false;
+}
+
+void render_state_free(kp_ctx* c) {
+    RenderState* r = c->render;
+    if (!r) return;
+    free_scene(r);
+    if (r->path_mem) (void)hipFree(r->path_mem);
+    if (r->film_sum) (void)hipFree(r->film_sum);
+    if (r->film_comp) (void)hipFree(r->film_comp);
+    if (r->film_out) (void)hipFree(r->film_out);
+    delete r;
+    c->render = nullptr;
+}
+
+static RenderState* rstate(kp_ctx* c) {
+    if (!c->render) c->render = new RenderState();
+    return c->render;
+}
+
+void render_scene_dirty(kp_ctx* c) {
+    if (c->render) c->render->scene_ready = false;
+}
+
+static float tri_area(const float* t) {
+    const float e1[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+    const float e2[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+    const float cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                cz = e1[0] * e2[1] - e1[1] * e2[0];
+    return 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+// Uploads primitives, materials and the light table (one area light per
+// emissive mesh, sampled by area: TriangleMesh::setup_dpdf, shape.cpp:700-709).
+static int prepare_scene(kp_ctx* c) {
+    RenderState* r = rstate(c);
+    if (r->scene_ready) return KP_OK;
+    if (!c->built) return fail(c, KP_ERR_STATE, "render before kp_bvh_build");
+    if (r->mats.empty()) return fail(c, KP_ERR_STATE, "render without materials (kp_material_add)");
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_scene(r);
+    const uint32_t n = c->n_tris_built;
+    std::vector<float4> st(3 * (size_t)n);
+    std::vector<uint32_t> light_tris;
+    std::vector<float> cdf;
+    std::vector<LightRec> lights;
+    for (size_t mi = 0; mi < c->mesh_first.size(); ++mi) {
+        const uint32_t mat = c->mesh_material[mi];
+        if (mat >= r->mats.size()) return fail(c, KP_ERR_INVALID, "mesh material id out of range");
+        const Material& m = r->mats[mat];
+        const bool emissive = m.Le[0] != 0.0f || m.Le[1] != 0.0f || m.Le[2] != 0.0f;
+        int32_t lid = -1;
+        if (emissive) {
+            lid = (int32_t)lights.size();
+            LightRec lr{};
+            lr.first = (uint32_t)light_tris.size();
+            lr.count = c->mesh_count[mi];
+            lr.cdf_first = (uint32_t)cdf.size();
+            cdf.push_back(0.0f);
+            for (uint32_t t = 0; t < lr.count; ++t) {
+                const uint32_t p = c->mesh_first[mi] + t;
+                light_tris.push_back(p);
+                cdf.push_back(cdf.back() + tri_area(&c->soup[9 * (size_t)p]));
+            }
+            const float sum = cdf.back();
+            lr.area = sum;
+            if (!(sum > 0.0f)) return fail(c, KP_ERR_INVALID, "emissive mesh with zero area");
+            const float inv = 1.0f / sum;
+            for (uint32_t t = 1; t <= lr.count; ++t) cdf[lr.cdf_first + t] *= inv;
+            cdf[lr.cdf_first + lr.count] = 1.0f;
+            lr.Le[0] = m.Le[0];
+            lr.Le[1] = m.Le[1];
+            lr.Le[2] = m.Le[2];
+            lights.push_back(lr);
+        }
+        for (uint32_t t = 0; t < c->mesh_count[mi]; ++t) {
+            const uint32_t p = c->mesh_first[mi] + t;
+            const float* v = &c->soup[9 * (size_t)p];
+            uint32_t matbits = mat;
+            float fm, fl;
+            std::memcpy(&fm, &matbits, 4);
+            std::memcpy(&fl, &lid, 4);
+            st[3 * (size_t)p] = make_float4(v[0], v[1], v[2], fm);
+            st[3 * (size_t)p + 1] = make_float4(v[3], v[4], v[5], fl);
+            st[3 * (size_t)p + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+        }
+    }
+    KP_HIP_TRY(c, hipMalloc((void**)&r->stris, st.size() * sizeof(float4)));
+    KP_HIP_TRY(c, hipMemcpy(r->stris, st.data(), st.size() * sizeof(float4), hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMalloc((void**)&r->d_mats, r->mats.size() * sizeof(Material)));
+    KP_HIP_TRY(c, hipMemcpy(r->d_mats, r->mats.data(), r->mats.size() * sizeof(Material), hipMemcpyHostToDevice));
+    r->n_lights = (uint32_t)lights.size();
+    if (!lights.empty()) {
+        KP_HIP_TRY(c, hipMalloc((void**)&r->d_lights, lights.size() * sizeof(LightRec)));
+        KP_HIP_TRY(c, hipMemcpy(r->d_lights, lights.data(), lights.size() * sizeof(LightRec), hipMemcpyHostToDevice));
+        KP_HIP_TRY(c, hipMalloc((void**)&r->d_light_tris, light_tris.size() * 4));
+        KP_HIP_TRY(c, hipMemcpy(r->d_light_tris, light_tris.data(), light_tris.size() * 4, hipMemcpyHostToDevice));
+        KP_HIP_TRY(c, hipMalloc((void**)&r->d_light_cdf, cdf.size() * 4));
+        KP_HIP_TRY(c, hipMemcpy(r->d_light_cdf, cdf.data(), cdf.size() * 4, hipMemcpyHostToDevice));
+    }
+    r->scene_ready = true;
+    return KP_OK;
+}
+
+static int ensure_paths(kp_ctx* c, uint32_t n) {
+    RenderState* r = rstate(c);
+    if (r->cap_paths >= n && r->path_mem) return KP_OK;
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (r->path_mem) (void)hipFree(r->path_mem);
+    r->path_mem = nullptr;
+    r->cap_paths = 0;
+    // bytes per slot: 2x32 rays + 2x4 qmap + 16 hit + 16 s0 + 16 s1 + 8 rng
+    //                 + 32 shadow ray + 16 contrib + 4 occl
+    const size_t N = n;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_r0 = take(N * 32), o_r1 = take(N * 32), o_q0 = take(N * 4), o_q1 = take(N * 4);
+    const size_t o_h = take(N * 16), o_s0 = take(N * 16), o_s1 = take(N * 16), o_rng = take(N * 8);
+    const size_t o_sr = take(N * 32), o_sc = take(N * 16), o_so = take(N * 4), o_cnt = take(64),
+                 o_tot = take(64);
+    KP_HIP_TRY(c, hipMalloc(&r->path_mem, off));
+    char* b = (char*)r->path_mem;
+    PathState& ps = r->ps;
+    ps.rays[0] = (float4*)(b + o_r0);
+    ps.rays[1] = (float4*)(b + o_r1);
+    ps.qmap[0] = (uint32_t*)(b + o_q0);
+    ps.qmap[1] = (uint32_t*)(b + o_q1);
+    ps.hits = (float4*)(b + o_h);
+    ps.s0 = (float4*)(b + o_s0);
+    ps.s1 = (float4*)(b + o_s1);
+    ps.rng = (uint64_t*)(b + o_rng);
+    ps.srays = (float4*)(b + o_sr);
+    ps.scontrib = (float4*)(b + o_sc);
+    ps.socc = (uint32_t*)(b + o_so);
+    ps.cnt = (uint32_t*)(b + o_cnt);
+    ps.totals = (unsigned long long*)(b + o_tot);
+    KP_HIP_TRY(c, hipMemset(ps.cnt, 0, 64));
+    KP_HIP_TRY(c, hipMemset(ps.totals, 0, 64));
+    r->cap_paths = n;
+    return KP_OK;
+}
+
+static int film_alloc(kp_ctx* c, uint32_t w, uint32_t h) {
+    RenderState* r = rstate(c);
+    const size_t npix = (size_t)w * h;
+    if (r->fw != w || r->fh != h || !r->film_sum) {
+        KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (r->film_sum) (void)hipFree(r->film_sum);
+        if (r->film_comp) (void)hipFree(r->film_comp);
+        if (r->film_out) (void)hipFree(r->film_out);
+        r->film_sum = r->film_comp = nullptr;
+        r->film_out = nullptr;
+        KP_HIP_TRY(c, hipMalloc((void**)&r->film_sum, npix * sizeof(float4)));
+        KP_HIP_TRY(c, hipMalloc((void**)&r->film_comp, npix * sizeof(float4)));
+        KP_HIP_TRY(c, hipMalloc((void**)&r->film_out, npix * 3 * sizeof(float)));
+        r->fw = w;
+        r->fh = h;
+    }
+    const uint32_t blocks = (uint32_t)((npix + 255) / 256);
+    hipLaunchKernelGGL(k_zero_f4, dim3(blocks), dim3(256), 0, c->stream, r->film_sum, (uint32_t)npix);
+    hipLaunchKernelGGL(k_zero_f4, dim3(blocks), dim3(256), 0, c->stream, r->film_comp, (uint32_t)npix);
+    KP_HIP_TRY(c, hipGetLastError());
+    r->film_count = 0;
+    return KP_OK;
+}
+
+static void camera_basis(const kp_render_desc& d, RenderArgs& a) {
+#pragma clang fp contract(off)
+    // Same float operations as the oracle camera: fwd, right = fwd x up, upv.
+    auto nrm = [](float x, float y, float z, float* o) {
+        const float l = std::sqrt(x * x + y * y + z * z);
+        o[0] = x / l; o[1] = y / l; o[2] = z / l;
+    };
+    float f[3], rr[3], u[3];
+    nrm(d.lookat[0] - d.eye[0], d.lookat[1] - d.eye[1], d.lookat[2] - d.eye[2], f);
+    const float cx = f[1] * d.up[2] - f[2] * d.up[1], cy = f[2] * d.up[0] - f[0] * d.up[2],
+                cz = f[0] * d.up[1] - f[1] * d.up[0];
+    nrm(cx, cy, cz, rr);
+    u[0] = rr[1] * f[2] - rr[2] * f[1];
+    u[1] = rr[2] * f[0] - rr[0] * f[2];
+    u[2] = rr[0] * f[1] - rr[1] * f[0];
+    a.eye = make_float3(d.eye[0], d.eye[1], d.eye[2]);
+    a.fwd = make_float3(f[0], f[1], f[2]);
+    a.right = make_float3(rr[0], rr[1], rr[2]);
+    a.upv = make_float3(u[0], u[1], u[2]);
+    a.tan_half = d.tan_half_fov;
+    a.aspect = (float)d.width / (float)d.height;
+}
+
+static uint32_t samples_per_pass(uint32_t npix, uint32_t spp) {
+    const uint64_t target = 1u << 21;   // paths in flight per pass
+    uint32_t best = 1;
+    for (uint32_t k = 1; k <= spp; ++k)
+        if (spp % k == 0 && (uint64_t)npix * k <= target) best = k;
+    return best;
+}
+
+static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
+    RenderState* r = rstate(c);
+    if (d->width != r->fw || d->height != r->fh)
+        return fail(c, KP_ERR_STATE, "film size differs from the render size (kp_film_reset first)");
+    if (d->max_depth == 0 || d->max_depth > 64) return fail(c, KP_ERR_INVALID, "max_depth must be 1..64");
+    int rc = prepare_scene(c);
+    if (rc) return rc;
+    const uint32_t npix = d->width * d->height;
+    const uint32_t k = samples_per_pass(npix, d->spp);
+    const uint32_t n_paths = npix * k;
+    if ((rc = ensure_paths(c, n_paths))) return rc;
+    RenderArgs a{};
+    a.stris = r->stris;
+    a.mats = r->d_mats;
+    a.lights = r->d_lights;
+    a.light_tris = r->d_light_tris;
+    a.light_cdf = r->d_light_cdf;
+    a.n_lights = r->n_lights;
+    a.width = d->width;
+    a.height = d->height;
+    a.npix = npix;
+    a.k = k;
+    a.n_paths = n_paths;
+    a.seed = d->seed;
+    a.max_depth = d->max_depth;
+    a.rr_depth = d->rr_depth;
+    camera_basis(*d, a);
+    a.ps = r->ps;
+    const uint32_t gen_blocks = (n_paths + 255) / 256;
+    const uint32_t shade_blocks = (uint32_t)c->cus * 8u;
+    auto t0 = std::chrono::steady_clock::now();
+    unsigned long long zero2[2] = {0, 0};
+    KP_HIP_TRY(c, hipMemcpyAsync(r->ps.totals, zero2, 16, hipMemcpyHostToDevice, c->stream));
+    for (uint32_t pass = 0; pass < d->spp / k; ++pass) {
+        a.s0 = d->spp_offset + pass * k;
+        hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, a);
+        KP_HIP_TRY(c, hipGetLastError());
+        int cur = 0;
+        for (uint32_t depth = 1; depth <= d->max_depth + 1; ++depth) {
+            TraceArgs ta;
+            std::memset(&ta, 0, sizeof(ta));
+            ta.nodes = c->d_nodes;
+            ta.tris = c->d_tris;
+            ta.counter = c->d_words;
+            ta.err = c->d_words + 1;
+            ta.spill = c->d_spill;
+            ta.spill_cap = c->spill_cap;
+            ta.refill_below = c->refill_below;
+            ta.max_steps = c->max_steps;
+            ta.counters = c->d_counters;
+            ta.rays = a.ps.rays[cur];
+            ta.hits = a.ps.hits;
+            ta.n = n_paths;
+            ta.n_dev = &a.ps.cnt[cur];
+            KP_HIP_TRY(c, launch_closest(ta, c->grid_closest, false, c->stream));
+            hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, c->stream, a, cur);
+            hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
+            KP_HIP_TRY(c, hipGetLastError());
+            if (depth <= d->max_depth) {
+                ta.rays = a.ps.srays;
+                ta.hits = nullptr;
+                ta.occl = a.ps.socc;
+                ta.n_dev = &a.ps.cnt[2];
+                KP_HIP_TRY(c, launch_occluded(ta, c->grid_occluded, c->stream));
+                hipLaunchKernelGGL(k_count_shadow, dim3(1), dim3(64), 0, c->stream, a);
+                hipLaunchKernelGGL(k_connect, dim3(shade_blocks), dim3(256), 0, c->stream, a);
+                KP_HIP_TRY(c, hipGetLastError());
+            }
+            cur ^= 1;
+        }
+        hipLaunchKernelGGL(k_film, dim3((npix + 255) / 256), dim3(256), 0, c->stream, a, r->film_sum,
+                           r->film_comp);
+        KP_HIP_TRY(c, hipGetLastError());
+    }
+    r->film_count += d->spp;
+    if ((rc = kp_synchronize(c))) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    unsigned long long tot[2];
+    KP_HIP_TRY(c, hipMemcpy(tot, r->ps.totals, 16, hipMemcpyDeviceToHost));
+    r->stats.paths = (uint64_t)npix * d->spp;
+    r->stats.extension_rays = tot[0];
+    r->stats.shadow_rays = tot[1];
+    r->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    r->stats.passes = d->spp / k;
+    r->stats.paths_per_pass = n_paths;
+    return KP_OK;
+}
+
+}  // namespace kp
+
+extern "C" {
+
+int kp_material_add(kp_ctx* c, const kp_material* m, uint32_t* id) {
+    if (!c || !m) return KP_ERR_INVALID;
+    if (m->type < 0 || m->type > 2 || m->dist < 0 || m->dist > 1)
+        return kp::fail(c, KP_ERR_INVALID, "kp_material_add: bad closure type or distribution");
+    if (m->type != 0 && !(m->alpha > 0.0f))
+        return kp::fail(c, KP_ERR_INVALID, "kp_material_add: microfacet alpha must be > 0");
+    if (m->type == 2 && !(m->eta > 0.0f))
+        return kp::fail(c, KP_ERR_INVALID, "kp_material_add: eta must be > 0");
+    RenderState* r = kp::rstate(c);
+    kp::Material g;
+    static_assert(sizeof(kp::Material) == sizeof(kp_material), "material layout");
+    std::memcpy(&g, m, sizeof(g));
+    r->mats.push_back(g);
+    r->scene_ready = false;
+    if (id) *id = (uint32_t)(r->mats.size() - 1);
+    return KP_OK;
+}
+
+int kp_mesh_set_material(kp_ctx* c, uint32_t mesh_id, uint32_t material_id) {
+    if (!c) return KP_ERR_INVALID;
+    if (mesh_id >= c->mesh_material.size()) return kp::fail(c, KP_ERR_INVALID, "bad mesh id");
+    c->mesh_material[mesh_id] = material_id;
+    kp::render_scene_dirty(c);
+    return KP_OK;
+}
+
+int kp_film_reset(kp_ctx* c, uint32_t w, uint32_t h) {
+    if (!c) return KP_ERR_INVALID;
+    if (w == 0 || h == 0 || (uint64_t)w * h > (1u << 26)) return kp::fail(c, KP_ERR_INVALID, "bad film size");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    return kp::film_alloc(c, w, h);
+}
+
+int kp_render_accumulate(kp_ctx* c, const kp_render_desc* d) {
+    if (!c || !d) return KP_ERR_INVALID;
+    if (d->spp == 0) return KP_OK;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    return kp::render_accumulate(c, d);
+}
+
+int kp_film_read(kp_ctx* c, float* out, int kind) {
+    if (!c || !out) return KP_ERR_INVALID;
+    RenderState* r = kp::rstate(c);
+    if (!r->film_sum) return kp::fail(c, KP_ERR_STATE, "no film (kp_film_reset first)");
+    if (kind != KP_PTR_HOST && kind != KP_PTR_DEVICE) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t npix = r->fw * r->fh;
+    const float inv = r->film_count ? 1.0f / (float)r->film_count : 0.0f;
+    float* dst = kind == KP_PTR_DEVICE ? out : r->film_out;
+    hipLaunchKernelGGL(kp::k_film_resolve, dim3((npix + 255) / 256), dim3(256), 0, c->stream,
+                       r->film_sum, r->film_comp, npix, inv, dst);
+    KP_HIP_TRY(c, hipGetLastError());
+    if (kind == KP_PTR_HOST)
+        KP_HIP_TRY(c, hipMemcpyAsync(out, r->film_out, (size_t)npix * 12, hipMemcpyDeviceToHost, c->stream));
+    return kp_synchronize(c);
+}
+
+int kp_render(kp_ctx* c, const kp_render_desc* d, float* out, int kind) {
+    if (!c || !d || !out) return KP_ERR_INVALID;
+    int rc = kp_film_reset(c, d->width, d->height);
+    if (rc) return rc;
+    if ((rc = kp_render_accumulate(c, d))) return rc;
+    return kp_film_read(c, out, kind);
+}
+
+int kp_render_get_stats(kp_ctx* c, kp_render_stats* s) {
+    if (!c || !s) return KP_ERR_INVALID;
+    *s = kp::rstate(c)->stats;
+    return KP_OK;
+}
+
+}  // extern "C"

@@ -223,6 +223,7 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
     st.sp = 0;
     st.tid = tid;
     const uint4* __restrict__ nodes = args.nodes;
+    const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n;   // device-side count for queues
     const float4* __restrict__ tris = args.tris;
 
     uint32_t ray_id = KP_INVALID;
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
             base = __shfl(base, leader);
             if (need) {
                 const uint32_t my = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (my < args.n) {
+                if (my < n_rays) {
                     ray_id = my;
                     const float4 ra = args.rays[2 * (size_t)my], rb = args.rays[2 * (size_t)my + 1];
                     setup_ray(ra, rb, r);
@@ -339,6 +340,7 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
     st.sp = 0;
     st.tid = tid;
     const uint4* __restrict__ nodes = args.nodes;
+    const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n;   // device-side count for queues
     const float4* __restrict__ tris = args.tris;
 
     uint32_t ray_id = KP_INVALID;
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
             base = __shfl(base, leader);
             if (need) {
                 const uint32_t my = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (my < args.n) {
+                if (my < n_rays) {
                     ray_id = my;
                     const float4 ra = args.rays[2 * (size_t)my], rb = args.rays[2 * (size_t)my + 1];
                     setup_ray(ra, rb, r);
@@ -420,6 +422,20 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy16(void* dst, const void* src, uint64_t n16, int cus, hipStream_t s) {
+    const uint64_t want = (n16 + 255) / 256;
+    const uint64_t cap = (uint64_t)(cus > 0 ? cus : 256) * 16;
+    const uint32_t grid = (uint32_t)(want < cap ? (want ? want : 1) : cap);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16);
+    return hipGetLastError();
 }
 
 __global__ void k_reset_u32(uint32_t* p, uint32_t n) {

@@ -44,6 +44,56 @@ class Stats(ctypes.Structure):
                 ("device_cus", _u32), ("reserved", _u32 * 7)]
 
 
+class Material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("dist", ctypes.c_int32),
+                ("albedo", ctypes.c_float * 3), ("F0", ctypes.c_float * 3),
+                ("alpha", ctypes.c_float), ("eta", ctypes.c_float),
+                ("Le", ctypes.c_float * 3), ("pad", ctypes.c_float)]
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [("width", _u32), ("height", _u32), ("spp", _u32), ("max_depth", _u32),
+                ("rr_depth", _u32), ("seed", _u32), ("spp_offset", _u32), ("flags", _u32),
+                ("eye", ctypes.c_float * 3), ("lookat", ctypes.c_float * 3),
+                ("up", ctypes.c_float * 3), ("tan_half_fov", ctypes.c_float)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("paths", _u64), ("extension_rays", _u64), ("shadow_rays", _u64),
+                ("seconds", ctypes.c_double), ("passes", _u32), ("paths_per_pass", _u32),
+                ("reserved", _u32 * 6)]
+
+
+def make_material(d):
+    """Material structure from a scenes.material() dict."""
+    m = Material()
+    m.type = d["type"]
+    m.dist = d["dist"]
+    m.albedo[:] = d["albedo"]
+    m.F0[:] = d["F0"]
+    m.alpha = d["alpha"]
+    m.eta = d["eta"]
+    m.Le[:] = d["Le"]
+    return m
+
+
+def make_render_desc(scene, spp=None, spp_offset=0, width=None, height=None):
+    d = RenderDesc()
+    d.width = width or scene.width
+    d.height = height or scene.height
+    d.spp = scene.spp if spp is None else spp
+    d.max_depth = scene.max_depth
+    d.rr_depth = scene.rr_depth
+    d.seed = scene.seed
+    d.spp_offset = spp_offset
+    d.flags = 0
+    d.eye[:] = scene.eye
+    d.lookat[:] = scene.lookat
+    d.up[:] = scene.up
+    d.tan_half_fov = scene.tan_half_fov
+    return d
+
+
 # name -> (restype, argtypes); mirrors include/kp_hip.h
 SIGNATURES = {
     "kp_create": (_i32, [ctypes.POINTER(_p), _i32]),
@@ -65,10 +115,18 @@ SIGNATURES = {
     "kp_bvh8_blob_tris": (_p, [_p]),
     "kp_bvh8_blob_free": (None, [_p]),
     "kp_set_trace_tuning": (_i32, [_p, _u32]),
+    "kp_material_add": (_i32, [_p, _p, ctypes.POINTER(_u32)]),
+    "kp_mesh_set_material": (_i32, [_p, _u32, _u32]),
+    "kp_film_reset": (_i32, [_p, _u32, _u32]),
+    "kp_render_accumulate": (_i32, [_p, _p]),
+    "kp_film_read": (_i32, [_p, _p, _i32]),
+    "kp_render": (_i32, [_p, _p, _p, _i32]),
+    "kp_render_get_stats": (_i32, [_p, _p]),
     "kp_device_alloc": (_i32, [_p, _u64, ctypes.POINTER(_p)]),
     "kp_device_free": (_i32, [_p, _p]),
     "kp_copy_to_device": (_i32, [_p, _p, _p, _u64]),
     "kp_copy_to_host": (_i32, [_p, _p, _p, _u64]),
+    "kp_device_copy": (_i32, [_p, _p, _p, _u64]),
     "kp_timer_begin": (_i32, [_p]),
     "kp_timer_end": (_i32, [_p, ctypes.POINTER(ctypes.c_float)]),
 }
@@ -261,6 +319,47 @@ class Context:
             out = np.zeros(n, np.uint32)
         self._check(lib().kp_occluded(self._h, _np_ptr(rays), n, _np_ptr(out), KP_PTR_HOST))
         return out
+
+    # ------------------------------------------------------------ render --
+    def load_scene(self, scene, leaf_size=4):
+        """Adds every mesh and material of a scenes.Scene and builds the BVH."""
+        mids = []
+        for md in scene.materials:
+            mid = _u32()
+            m = make_material(md)
+            self._check(lib().kp_material_add(self._h, ctypes.byref(m), ctypes.byref(mid)))
+            mids.append(mid.value)
+        for tris, mi in zip(scene.meshes, scene.mesh_material):
+            mesh = self.mesh_add_soup(tris)
+            self._check(lib().kp_mesh_set_material(self._h, mesh, mids[mi]))
+        self.bvh_build(leaf_size=leaf_size)
+
+    def film_reset(self, width, height):
+        self._check(lib().kp_film_reset(self._h, width, height))
+
+    def render_accumulate(self, desc):
+        self._check(lib().kp_render_accumulate(self._h, ctypes.byref(desc)))
+
+    def film_read(self, width, height):
+        out = np.zeros((height, width, 3), np.float32)
+        self._check(lib().kp_film_read(self._h, _np_ptr(out), KP_PTR_HOST))
+        return out
+
+    def render(self, desc):
+        """Mean radiance image [h, w, 3] float32."""
+        out = np.zeros((desc.height, desc.width, 3), np.float32)
+        self._check(lib().kp_render(self._h, ctypes.byref(desc), _np_ptr(out), KP_PTR_HOST))
+        return out
+
+    def render_stats(self):
+        s = RenderStats()
+        self._check(lib().kp_render_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in RenderStats._fields_ if k != "reserved"}
+
+    def device_copy(self, dst, src):
+        """Kernel copy between two DeviceBuffers of the same size."""
+        assert dst.nbytes == src.nbytes
+        self._check(lib().kp_device_copy(self._h, dst.ptr, src.ptr, src.nbytes))
 
     def timer_begin(self):
         self._check(lib().kp_timer_begin(self._h))

@@ -262,6 +262,64 @@ def cornell_box(width=256, height=256, spp=16, max_depth=5, rr_depth=3, seed=7,
     return s
 
 
+def materials_scene(width=32, height=32, spp=16, max_depth=6, rr_depth=3, seed=11):
+    """Every closure of the integrator in one small room: Lambert walls, a GGX
+    conductor, a Beckmann conductor, a Beckmann dielectric (eta 1.5) and a GGX
+    dielectric sphere under two area lights (one tilted)."""
+    s = cornell_box(width=width, height=height, spp=spp, max_depth=max_depth, rr_depth=rr_depth,
+                    seed=seed, sphere_lon=24, sphere_lat=12)
+    s.name = "materials"
+    s.meshes.pop()          # drop the Lambert sphere
+    s.mesh_material.pop()
+    mats = [material(CONDUCTOR, GGX, F0=(0.95, 0.7, 0.4), alpha=0.2),
+            material(CONDUCTOR, BECKMANN, F0=(0.6, 0.9, 0.9), alpha=0.35),
+            material(DIELECTRIC, BECKMANN, alpha=0.15, eta=1.5),
+            material(DIELECTRIC, GGX, alpha=0.4, eta=1.5)]
+    centers = [(-0.5, -0.65, -0.4), (0.5, -0.65, -0.5), (-0.35, -0.1, 0.35), (0.45, 0.0, 0.3)]
+    for m, c in zip(mats, centers):
+        s.materials.append(m)
+        s.add_mesh(uv_sphere(24, 12, center=c, radius=0.3), len(s.materials) - 1)
+    lt = len(s.materials)
+    s.materials.append(material(LAMBERT, albedo=(0.2, 0.2, 0.2), Le=(4.0, 6.0, 8.0)))
+    s.add_mesh(quad((-0.95, 0.2, -0.6), (-0.95, 0.2, -0.2), (-0.95, 0.6, -0.2), (-0.95, 0.6, -0.6)), lt)
+    return s
+
+
+def orient_toward(tris9, point):
+    """Flips triangles so their winding normals face `point` (inward for a
+    closed mesh around it)."""
+    t = np.array(tris9, np.float64).reshape(-1, 3, 3)
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    c = t.mean(1)
+    away = ((np.asarray(point, np.float64) - c) * n).sum(1) < 0
+    t[away] = t[away][:, [0, 2, 1]]
+    return t.reshape(-1, 9).astype(np.float32)
+
+
+def furnace_box(albedo=0.5, Le=1.0, width=16, height=16, spp=64, max_depth=4, rr_depth=64,
+                seed=5):
+    """Closed cube [-1,1]^3, every face Lambert(albedo) emitting Le toward the
+    inside, camera at the centre.  Closed form of the estimator's mean:
+    L = Le * sum_{k=0}^{max_depth} albedo^k (one emission per vertex, NEE and
+    BSDF sampling combined by MIS)."""
+    s = Scene(name="furnace", width=width, height=height, spp=spp, max_depth=max_depth,
+              rr_depth=rr_depth, seed=seed, eye=(0.0, 0.0, 0.0), lookat=(0.3, 0.2, -1.0),
+              up=(0.0, 1.0, 0.0), fov_deg=90.0)
+    s.materials.append(material(LAMBERT, albedo=(albedo,) * 3, Le=(Le,) * 3))
+    faces = [quad((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1)),
+             quad((-1, 1, 1), (-1, 1, -1), (1, 1, -1), (1, 1, 1)),
+             quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1)),
+             quad((-1, -1, 1), (-1, 1, 1), (1, 1, 1), (1, -1, 1)),
+             quad((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1)),
+             quad((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1))]
+    s.add_mesh(orient_toward(np.concatenate(faces), (0, 0, 0)), 0)
+    return s
+
+
+def furnace_expected(albedo, Le, max_depth):
+    return Le * sum(albedo ** k for k in range(max_depth + 1))
+
+
 def small_test_scene(seed=3, n_random=400):
     """Tiny scene for brute-force parity: a 16x8 sphere, a floor quad and random
     triangles, including shared edges and vertices."""

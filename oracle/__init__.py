@@ -61,6 +61,9 @@ def lib():
             "or_mf_sample_vndf": (None, [_i32, ctypes.c_float, _p, _p, _p]),
             "or_fresnel_dielectric": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
             "or_erfinv": (ctypes.c_double, [ctypes.c_double]),
+            "or_bsdf_eval_batch": (None, [_p, _p, _p, _i32, _p, _p]),
+            "or_bsdf_sample_batch": (None, [_p, _p, _p, _i32, _p, _p, _p, _p]),
+            "or_mf_eval_batch": (None, [_i32, ctypes.c_float, _p, _i32, _p, _p]),
             "or_scene_create": (_p, [_p, _u32, _p, _p, _p, _u32]),
             "or_scene_free": (None, [_p]),
             "or_render": (_i32, [_p, _p, _p, _p, _i32]),
@@ -204,6 +207,38 @@ def bsdf_sample(mat, wo, u3):
     ok = lib().or_bsdf_sample(ctypes.byref(mat), _ptr(wo), _ptr(u3), _ptr(wi), _ptr(f),
                               ctypes.byref(pdf))
     return bool(ok), wi, f, pdf.value
+
+
+def bsdf_eval_batch(mat, wo, wi):
+    wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32), np.shape(wi)), np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    n = wi.shape[0]
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().or_bsdf_eval_batch(ctypes.byref(mat), _ptr(wo), _ptr(wi), n, _ptr(f), _ptr(pdf))
+    return f, pdf
+
+
+def bsdf_sample_batch(mat, wo, u3):
+    u3 = np.ascontiguousarray(u3, np.float32)
+    n = u3.shape[0]
+    wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32), (n, 3)), np.float32)
+    wi = np.zeros((n, 3), np.float32)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    ok = np.zeros(n, np.int32)
+    lib().or_bsdf_sample_batch(ctypes.byref(mat), _ptr(wo), _ptr(u3), n, _ptr(wi), _ptr(f),
+                               _ptr(pdf), _ptr(ok))
+    return ok.astype(bool), wi, f, pdf
+
+
+def mf_eval_batch(dist, alpha, w):
+    w = np.ascontiguousarray(w, np.float32)
+    n = w.shape[0]
+    D = np.zeros(n, np.float64)
+    lam = np.zeros(n, np.float64)
+    lib().or_mf_eval_batch(dist, alpha, _ptr(w), n, _ptr(D), _ptr(lam))
+    return D, lam
 
 
 def mf_D(dist, alpha, m):

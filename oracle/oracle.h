@@ -112,6 +112,13 @@ void or_mf_sample_vndf(int dist, float alpha, const float* wo, const float* u2,
                        float* m);
 double or_fresnel_dielectric(double cos_i, double eta);
 double or_erfinv(double x);
+/* Batched forms (n directions, 3 floats each) for quadrature tests. */
+void or_bsdf_eval_batch(const or_material* m, const float* wo, const float* wi, int n,
+                        float* f, float* pdf);
+void or_bsdf_sample_batch(const or_material* m, const float* wo, const float* u3, int n,
+                          float* wi, float* f, float* pdf, int32_t* ok);
+void or_mf_eval_batch(int dist, float alpha, const float* w, int n, double* D,
+                      double* lambda);
 
 /* ---- Path integrator ---------------------------------------------------- */
 typedef struct {

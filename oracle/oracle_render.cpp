@@ -648,6 +648,35 @@ void or_mf_sample_vndf(int dist, float alpha, const float* wo, const float* u2, 
 double or_fresnel_dielectric(double cos_i, double eta) { return fresnel_dielectric(cos_i, eta); }
 double or_erfinv(double x) { return erfinv_d(x); }
 
+void or_bsdf_eval_batch(const or_material* m, const float* wo, const float* wi, int n, float* f,
+                        float* pdf) {
+    Mat mm = load_mat(*m);
+    for (int i = 0; i < n; ++i) {
+        V3 ff;
+        bsdf_eval(mm, load3(wo + 3 * i), load3(wi + 3 * i), ff, pdf[i]);
+        f[3 * i] = ff.x; f[3 * i + 1] = ff.y; f[3 * i + 2] = ff.z;
+    }
+}
+
+void or_bsdf_sample_batch(const or_material* m, const float* wo, const float* u3, int n, float* wi,
+                          float* f, float* pdf, int32_t* ok) {
+    Mat mm = load_mat(*m);
+    for (int i = 0; i < n; ++i) {
+        V3 w, ff;
+        ok[i] = bsdf_sample(mm, load3(wo + 3 * i), u3[3 * i], u3[3 * i + 1], u3[3 * i + 2], w, ff, pdf[i]);
+        wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+        f[3 * i] = ff.x; f[3 * i + 1] = ff.y; f[3 * i + 2] = ff.z;
+    }
+}
+
+void or_mf_eval_batch(int dist, float alpha, const float* w, int n, double* D, double* lambda) {
+    for (int i = 0; i < n; ++i) {
+        V3 v = load3(w + 3 * i);
+        D[i] = mf_D(dist, alpha, v);
+        lambda[i] = mf_lambda(dist, alpha, v);
+    }
+}
+
 or_scene* or_scene_create(const float* tris, uint32_t n_tris, const int32_t* tri_material,
                           const int32_t* tri_light, const or_material* materials,
                           uint32_t n_materials) {

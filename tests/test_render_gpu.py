@@ -1,0 +1,144 @@
+"""GPU wavefront integrator against the oracle.
+
+Three levels: (1) closed-form furnace radiance; (2) per-path agreement at one
+sample per pixel (same sampling protocol and seeds: the GPU path radiance
+must equal the oracle's up to float rounding for nearly every pixel);
+(3) statistics: the GPU image's relMSE against an FP64 high-spp oracle
+render must match the error level of an independent oracle render at the
+same spp (relMSE = mean (I - R)^2 / (R^2 + 1e-2))."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from kazen_proto_amd import native, scenes
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CPU_THREADS = min(16, os.cpu_count() or 1)
+
+
+def rel_mse(img, ref):
+    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def oracle_scene(s):
+    mats = [oracle.material(m["type"], m["dist"], m["albedo"], m["F0"], m["alpha"], m["eta"], m["Le"])
+            for m in s.materials]
+    return oracle.Scene(s.soup(), s.tri_material(), s.tri_light(), mats)
+
+
+def odesc(s, spp=None, seed=None, spp_offset=0):
+    d = oracle.RenderDesc()
+    d.width, d.height = s.width, s.height
+    d.spp = s.spp if spp is None else spp
+    d.max_depth, d.rr_depth = s.max_depth, s.rr_depth
+    d.seed = s.seed if seed is None else seed
+    d.spp_offset = spp_offset
+    d.eye[:], d.lookat[:], d.up[:] = s.eye, s.lookat, s.up
+    d.tan_half_fov = s.tan_half_fov
+    return d
+
+
+def gdesc(s, spp=None, seed=None, spp_offset=0):
+    d = native.make_render_desc(s, spp=spp, spp_offset=spp_offset)
+    if seed is not None:
+        d.seed = seed
+    return d
+
+
+def gpu_ctx(s):
+    c = native.Context(0)
+    c.load_scene(s)
+    return c
+
+
+@pytest.mark.parametrize("albedo,depth,rr", [(0.5, 4, 64), (0.7, 6, 2), (1.0, 3, 64)])
+def test_furnace_closed_form(albedo, depth, rr):
+    s = scenes.furnace_box(albedo=albedo, Le=1.0, max_depth=depth, rr_depth=rr, spp=64,
+                           width=32, height=32)
+    c = gpu_ctx(s)
+    img = c.render(gdesc(s))
+    expect = scenes.furnace_expected(albedo, 1.0, depth)
+    pix = img[..., 0].astype(np.float64).ravel()
+    err = pix.std() / np.sqrt(pix.size)
+    print("furnace albedo %.2f depth %d: %.5f expected %.5f (+-%.5f)" % (albedo, depth, pix.mean(),
+                                                                         expect, err))
+    assert abs(pix.mean() - expect) < 5 * err + 1e-6
+    c.close()
+
+
+def _per_path(s):
+    c = gpu_ctx(s)
+    g = c.render(gdesc(s, spp=1)).astype(np.float64)
+    o = oracle_scene(s).render(odesc(s, spp=1), threads=CPU_THREADS)
+    c.close()
+    close = np.all(np.abs(g - o) <= 2e-3 * np.abs(o) + 1e-5, axis=2)
+    return float(close.mean()), g, o
+
+
+def test_cornell_per_path_agreement():
+    s = scenes.cornell_box(width=64, height=64)
+    frac, g, o = _per_path(s)
+    print("cornell: %.4f of pixels agree path by path" % frac)
+    assert frac >= 0.97
+
+
+def test_all_closures_per_path_agreement():
+    s = scenes.materials_scene(width=48, height=48)
+    frac, g, o = _per_path(s)
+    print("materials: %.4f of pixels agree path by path" % frac)
+    assert frac >= 0.95
+
+
+def test_cornell_matches_fp64_oracle_image():
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "cornell64_ref.npz"))
+    s = scenes.cornell_box(width=64, height=64)
+    c = gpu_ctx(s)
+    img = c.render(gdesc(s, spp=256, seed=3000))
+    st = c.render_stats()
+    c.close()
+    e = rel_mse(img, ref["ref"])
+    print("cornell 256 spp relMSE %.3e (independent oracle 256 spp: %.3e); %.1f Mpaths/s"
+          % (e, float(ref["noise256"]), st["paths"] / st["seconds"] / 1e6))
+    assert e < 1.5 * float(ref["noise256"])
+    # image mean within 3 sigma of the oracle's
+    assert abs(img.mean() - ref["ref"].mean()) < 0.01 * ref["ref"].mean()
+
+
+def test_all_closures_match_oracle_statistics():
+    s = scenes.materials_scene(width=32, height=32)
+    osc = oracle_scene(s)
+    ref = osc.render(odesc(s, spp=4096, seed=900), threads=CPU_THREADS)
+    indep = osc.render(odesc(s, spp=256, seed=901), threads=CPU_THREADS)
+    c = gpu_ctx(s)
+    img = c.render(gdesc(s, spp=256, seed=902))
+    c.close()
+    e, e0 = rel_mse(img, ref), rel_mse(indep, ref)
+    print("materials 256 spp relMSE gpu %.3e oracle %.3e" % (e, e0))
+    assert e < 1.6 * e0
+
+
+def test_film_accumulate_resumes_bit_for_bit_samples():
+    s = scenes.cornell_box(width=32, height=32, spp=8)
+    c = gpu_ctx(s)
+    full = c.render(gdesc(s, spp=8))
+    c.film_reset(32, 32)
+    c.render_accumulate(gdesc(s, spp=4, spp_offset=0))
+    c.render_accumulate(gdesc(s, spp=4, spp_offset=4))
+    two = c.film_read(32, 32)
+    c.close()
+    assert np.allclose(full, two, rtol=1e-5, atol=1e-7)
+
+
+def test_render_state_errors():
+    c = native.Context(0)
+    s = scenes.cornell_box(width=8, height=8, spp=1)
+    with pytest.raises(native.KpError):
+        c.render(gdesc(s))          # no scene
+    c.mesh_add_soup(s.soup())
+    c.bvh_build()
+    with pytest.raises(native.KpError):
+        c.render(gdesc(s))          # no materials
+    c.close()
