@@ -139,7 +139,7 @@ const void* kp_bvh8_blob_tris(const kp_bvh8_blob* b);
 void kp_bvh8_blob_free(kp_bvh8_blob* b);
 
 /* Tuning: refill a wave once fewer than `lanes` (1..64) of its lanes are
- * busy (default 40). */
+ * busy (default 4). */
 int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
 
 /* Integrator -------------------------------------------------------------- */
@@ -205,6 +205,12 @@ int kp_film_read(kp_ctx* ctx, float* rgb, int ptr_kind);
 /* kp_film_reset + kp_render_accumulate + kp_film_read. */
 int kp_render(kp_ctx* ctx, const kp_render_desc* desc, float* rgb, int ptr_kind);
 int kp_render_get_stats(kp_ctx* ctx, kp_render_stats* out);
+/* Batched Integrator::Li (src/core/integrator.h:29, PathIntegrator::Li
+ * integrator.cpp:362-561): traces one path from each caller ray and writes its
+ * radiance (rgb[3*i]). Uses desc->max_depth, rr_depth, seed and spp_offset
+ * (the sample index); path i draws from PCG32 sequence i. */
+int kp_integrate_rays(kp_ctx* ctx, const kp_ray* rays, uint32_t n, const kp_render_desc* desc,
+                      float* rgb, int ptr_kind);
 
 /* Device memory and timing helpers for callers without their own HIP
  * runtime handle (the reference has no counterpart: its callers never see

@@ -65,6 +65,7 @@ struct RenderArgs {
     const float* light_cdf;
     uint32_t n_lights;
     uint32_t width, height, npix, k, n_paths;
+    uint32_t seq_base;       // PCG sequence of slot s = seq_base + s / k
     uint32_t seed, s0, max_depth, rr_depth;
     float3 eye, fwd, right, upv;
     float tan_half, aspect;
@@ -154,6 +155,35 @@ __global__ __launch_bounds__(256) void k_generate(RenderArgs a) {
     a.ps.rng[slot] = rng.state;
 }
 
+// Starts one path per caller ray (integrator entry for arbitrary primary
+// rays): sequence = seq_base + slot, sample index = s0.
+__global__ __launch_bounds__(256) void k_generate_rays(RenderArgs a, const float4* rays) {
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot == 0) {
+        a.ps.cnt[0] = a.n_paths;
+        a.ps.cnt[1] = 0;
+        a.ps.cnt[2] = 0;
+    }
+    if (slot >= a.n_paths) return;
+    const uint64_t initstate = ((uint64_t)a.s0 << 32) ^ (uint64_t)a.seed ^ 0x9E3779B97F4A7C15ull;
+    const Pcg rng = pcg_seed(initstate, a.seq_base + slot);
+    a.ps.rays[0][2 * (size_t)slot] = rays[2 * (size_t)slot];
+    a.ps.rays[0][2 * (size_t)slot + 1] = rays[2 * (size_t)slot + 1];
+    a.ps.qmap[0][slot] = slot;
+    a.ps.s0[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.ps.s1[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a.ps.rng[slot] = rng.state;
+}
+
+__global__ void k_write_radiance(RenderArgs a, float* out) {
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot >= a.n_paths) return;
+    const float4 l = a.ps.s1[slot];
+    out[3 * (size_t)slot] = l.x;
+    out[3 * (size_t)slot + 1] = l.y;
+    out[3 * (size_t)slot + 2] = l.z;
+}
+
 // Resets the output queue counters of the coming shade, accumulating totals.
 __global__ void k_bookkeep(RenderArgs a, int cur) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -205,7 +235,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
             const float prev_pdf = st0.w;
             f3 L = mk(st1.x, st1.y, st1.z);
             const uint32_t prim = __float_as_uint(hit.y);
-            const uint32_t pixel = slot / a.k;
+            const uint32_t pixel = a.seq_base + slot / a.k;
             Pcg rng;
             rng.state = a.ps.rng[slot];
             rng.inc = ((uint64_t)pixel << 1u) | 1u;
@@ -602,6 +632,119 @@ static uint32_t samples_per_pass(uint32_t npix, uint32_t spp) {
     return best;
 }
 
+// Extend / shade / connect for max_depth + 1 bounces of the paths that the
+// generate kernel has queued (queue sizes live on the device).
+static int run_bounces(kp_ctx* c, const RenderArgs& a) {
+    const uint32_t shade_blocks = (uint32_t)c->cus * 8u;
+    int cur = 0;
+    for (uint32_t depth = 1; depth <= a.max_depth + 1; ++depth) {
+        TraceArgs ta;
+        std::memset(&ta, 0, sizeof(ta));
+        ta.nodes = c->d_nodes;
+        ta.tris = c->d_tris;
+        ta.counter = c->d_words;
+        ta.err = c->d_words + 1;
+        ta.spill = c->d_spill;
+        ta.spill_cap = c->spill_cap;
+        ta.refill_below = c->refill_below;
+        ta.max_steps = c->max_steps;
+        ta.counters = c->d_counters;
+        ta.rays = a.ps.rays[cur];
+        ta.hits = a.ps.hits;
+        ta.n = a.n_paths;
+        ta.n_dev = &a.ps.cnt[cur];
+        KP_HIP_TRY(c, launch_closest(ta, c->grid_closest, false, c->stream));
+        hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, c->stream, a, cur);
+        hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
+        KP_HIP_TRY(c, hipGetLastError());
+        if (depth <= a.max_depth) {
+            ta.rays = a.ps.srays;
+            ta.hits = nullptr;
+            ta.occl = a.ps.socc;
+            ta.n_dev = &a.ps.cnt[2];
+            KP_HIP_TRY(c, launch_occluded(ta, c->grid_occluded, c->stream));
+            hipLaunchKernelGGL(k_count_shadow, dim3(1), dim3(64), 0, c->stream, a);
+            hipLaunchKernelGGL(k_connect, dim3(shade_blocks), dim3(256), 0, c->stream, a);
+            KP_HIP_TRY(c, hipGetLastError());
+        }
+        cur ^= 1;
+    }
+    return KP_OK;
+}
+
+static RenderArgs base_render_args(kp_ctx* c) {
+    RenderState* r = rstate(c);
+    RenderArgs a{};
+    a.stris = r->stris;
+    a.mats = r->d_mats;
+    a.lights = r->d_lights;
+    a.light_tris = r->d_light_tris;
+    a.light_cdf = r->d_light_cdf;
+    a.n_lights = r->n_lights;
+    a.ps = r->ps;
+    return a;
+}
+
+// Traces one path per caller ray; writes the path radiance (RGB) per ray.
+static int integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_render_desc* d,
+                          float* out, int kind) {
+    if (d->max_depth == 0 || d->max_depth > 64) return fail(c, KP_ERR_INVALID, "max_depth must be 1..64");
+    int rc = prepare_scene(c);
+    if (rc) return rc;
+    RenderState* r = rstate(c);
+    const uint32_t chunk_cap = 1u << 21;
+    const uint32_t cap = n < chunk_cap ? n : chunk_cap;
+    if ((rc = ensure_paths(c, cap))) return rc;
+    float4* d_in = nullptr;
+    float* d_out = nullptr;
+    if (kind == KP_PTR_HOST) {
+        if ((rc = ensure_buffer(c, &c->d_stage_in, &c->stage_in_bytes, (size_t)cap * 32))) return rc;
+        if ((rc = ensure_buffer(c, &c->d_stage_out, &c->stage_out_bytes, (size_t)cap * 12))) return rc;
+        d_in = (float4*)c->d_stage_in;
+        d_out = (float*)c->d_stage_out;
+    }
+    unsigned long long zero2[2] = {0, 0};
+    KP_HIP_TRY(c, hipMemcpyAsync(r->ps.totals, zero2, 16, hipMemcpyHostToDevice, c->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t off = 0; off < n; off += cap) {
+        const uint32_t cnt = (n - off) < cap ? (n - off) : cap;
+        RenderArgs a = base_render_args(c);
+        a.k = 1;
+        a.n_paths = cnt;
+        a.seq_base = off;
+        a.s0 = d->spp_offset;
+        a.seed = d->seed;
+        a.max_depth = d->max_depth;
+        a.rr_depth = d->rr_depth;
+        const float4* src = (const float4*)(rays) + 2 * (size_t)off;
+        float* dst = out + 3 * (size_t)off;
+        if (kind == KP_PTR_HOST) {
+            KP_HIP_TRY(c, hipMemcpyAsync(d_in, src, (size_t)cnt * 32, hipMemcpyHostToDevice, c->stream));
+            src = d_in;
+            dst = d_out;
+        }
+        hipLaunchKernelGGL(k_generate_rays, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a, src);
+        KP_HIP_TRY(c, hipGetLastError());
+        if ((rc = run_bounces(c, a))) return rc;
+        hipLaunchKernelGGL(k_write_radiance, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a, dst);
+        KP_HIP_TRY(c, hipGetLastError());
+        if (kind == KP_PTR_HOST)
+            KP_HIP_TRY(c, hipMemcpyAsync(out + 3 * (size_t)off, d_out, (size_t)cnt * 12,
+                                         hipMemcpyDeviceToHost, c->stream));
+    }
+    if ((rc = kp_synchronize(c))) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    unsigned long long tot[2];
+    KP_HIP_TRY(c, hipMemcpy(tot, r->ps.totals, 16, hipMemcpyDeviceToHost));
+    r->stats.paths = n;
+    r->stats.extension_rays = tot[0];
+    r->stats.shadow_rays = tot[1];
+    r->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    r->stats.passes = (n + cap - 1) / cap;
+    r->stats.paths_per_pass = cap;
+    return KP_OK;
+}
+
 static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     RenderState* r = rstate(c);
     if (d->width != r->fw || d->height != r->fh)
@@ -631,7 +774,6 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     camera_basis(*d, a);
     a.ps = r->ps;
     const uint32_t gen_blocks = (n_paths + 255) / 256;
-    const uint32_t shade_blocks = (uint32_t)c->cus * 8u;
     auto t0 = std::chrono::steady_clock::now();
     unsigned long long zero2[2] = {0, 0};
     KP_HIP_TRY(c, hipMemcpyAsync(r->ps.totals, zero2, 16, hipMemcpyHostToDevice, c->stream));
@@ -639,39 +781,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
         a.s0 = d->spp_offset + pass * k;
         hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, a);
         KP_HIP_TRY(c, hipGetLastError());
-        int cur = 0;
-        for (uint32_t depth = 1; depth <= d->max_depth + 1; ++depth) {
-            TraceArgs ta;
-            std::memset(&ta, 0, sizeof(ta));
-            ta.nodes = c->d_nodes;
-            ta.tris = c->d_tris;
-            ta.counter = c->d_words;
-            ta.err = c->d_words + 1;
-            ta.spill = c->d_spill;
-            ta.spill_cap = c->spill_cap;
-            ta.refill_below = c->refill_below;
-            ta.max_steps = c->max_steps;
-            ta.counters = c->d_counters;
-            ta.rays = a.ps.rays[cur];
-            ta.hits = a.ps.hits;
-            ta.n = n_paths;
-            ta.n_dev = &a.ps.cnt[cur];
-            KP_HIP_TRY(c, launch_closest(ta, c->grid_closest, false, c->stream));
-            hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, c->stream, a, cur);
-            hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
-            KP_HIP_TRY(c, hipGetLastError());
-            if (depth <= d->max_depth) {
-                ta.rays = a.ps.srays;
-                ta.hits = nullptr;
-                ta.occl = a.ps.socc;
-                ta.n_dev = &a.ps.cnt[2];
-                KP_HIP_TRY(c, launch_occluded(ta, c->grid_occluded, c->stream));
-                hipLaunchKernelGGL(k_count_shadow, dim3(1), dim3(64), 0, c->stream, a);
-                hipLaunchKernelGGL(k_connect, dim3(shade_blocks), dim3(256), 0, c->stream, a);
-                KP_HIP_TRY(c, hipGetLastError());
-            }
-            cur ^= 1;
-        }
+        if ((rc = run_bounces(c, a))) return rc;
         hipLaunchKernelGGL(k_film, dim3((npix + 255) / 256), dim3(256), 0, c->stream, a, r->film_sum,
                            r->film_comp);
         KP_HIP_TRY(c, hipGetLastError());
@@ -757,6 +867,16 @@ int kp_render(kp_ctx* c, const kp_render_desc* d, float* out, int kind) {
     if (rc) return rc;
     if ((rc = kp_render_accumulate(c, d))) return rc;
     return kp_film_read(c, out, kind);
+}
+
+int kp_integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_render_desc* d,
+                      float* rgb, int kind) {
+    if (!c || !d) return KP_ERR_INVALID;
+    if (kind != KP_PTR_HOST && kind != KP_PTR_DEVICE) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
+    if (n == 0) return KP_OK;
+    if (!rays || !rgb) return kp::fail(c, KP_ERR_INVALID, "NULL rays or output");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    return kp::integrate_rays(c, rays, n, d, rgb, kind);
 }
 
 int kp_render_get_stats(kp_ctx* c, kp_render_stats* s) {

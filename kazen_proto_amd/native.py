@@ -122,6 +122,7 @@ SIGNATURES = {
     "kp_film_read": (_i32, [_p, _p, _i32]),
     "kp_render": (_i32, [_p, _p, _p, _i32]),
     "kp_render_get_stats": (_i32, [_p, _p]),
+    "kp_integrate_rays": (_i32, [_p, _p, _u32, _p, _p, _i32]),
     "kp_device_alloc": (_i32, [_p, _u64, ctypes.POINTER(_p)]),
     "kp_device_free": (_i32, [_p, _p]),
     "kp_copy_to_device": (_i32, [_p, _p, _p, _u64]),
@@ -349,6 +350,14 @@ class Context:
         """Mean radiance image [h, w, 3] float32."""
         out = np.zeros((desc.height, desc.width, 3), np.float32)
         self._check(lib().kp_render(self._h, ctypes.byref(desc), _np_ptr(out), KP_PTR_HOST))
+        return out
+
+    def integrate_rays(self, rays, desc):
+        """Radiance [n,3] of one path per ray (batched Integrator::Li)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 3), np.float32)
+        self._check(lib().kp_integrate_rays(self._h, _np_ptr(rays), rays.shape[0],
+                                            ctypes.byref(desc), _np_ptr(out), KP_PTR_HOST))
         return out
 
     def render_stats(self):

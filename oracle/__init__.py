@@ -67,6 +67,7 @@ def lib():
             "or_scene_create": (_p, [_p, _u32, _p, _p, _p, _u32]),
             "or_scene_free": (None, [_p]),
             "or_render": (_i32, [_p, _p, _p, _p, _i32]),
+            "or_integrate_rays": (_i32, [_p, _p, _u32, _p, _p, _i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -290,3 +291,12 @@ class Scene:
         if rc != 0:
             raise RuntimeError("or_render failed (%d)" % rc)
         return (out, s0) if want_sample0 else out
+
+    def integrate_rays(self, rays, desc, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 3), np.float32)
+        rc = lib().or_integrate_rays(self._h, _ptr(rays), rays.shape[0], ctypes.byref(desc),
+                                     _ptr(out), threads)
+        if rc != 0:
+            raise RuntimeError("or_integrate_rays failed (%d)" % rc)
+        return out

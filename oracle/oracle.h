@@ -143,6 +143,10 @@ void or_scene_free(or_scene* s);
  * may be NULL): radiance of the first sample of every pixel, FP32. */
 int or_render(const or_scene* s, const or_render_desc* d, double* out,
               float* sample0, int threads);
+/* One path per caller ray (rays: 8 floats each, tmin/tmax ignored): path i
+ * uses PCG32 sequence i and sample index d->spp_offset; out[3*i] radiance. */
+int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n,
+                      const or_render_desc* d, float* out, int threads);
 
 #ifdef __cplusplus
 }

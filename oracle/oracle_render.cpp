@@ -523,6 +523,8 @@ Hit trace(const or_scene& s, V3 o, V3 d, float tmax) {
     return r;
 }
 
+V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir);
+
 V3 path_radiance(const or_scene& s, const or_render_desc& d, const Camera& cam, uint32_t px,
                  uint32_t py, uint32_t sample) {
     Pcg32 rng;
@@ -530,6 +532,11 @@ V3 path_radiance(const or_scene& s, const or_render_desc& d, const Camera& cam, 
     float ux = rng.nextf(), uy = rng.nextf();
     V3 o = cam.eye;
     V3 dir = camera_dir(cam, (float)px + ux, (float)py + uy);
+    return trace_path(s, d, rng, o, dir);
+}
+
+// PathIntegrator::Li restated: radiance along the ray (o, dir).
+V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir) {
     V3 beta = v3(1, 1, 1), L = v3(0, 0, 0);
     float prev_pdf = 0.0f;
     const uint32_t nl = (uint32_t)s.lights.size();
@@ -711,6 +718,30 @@ void or_scene_free(or_scene* s) {
     if (!s) return;
     or_bvh_free(s->bvh);
     delete s;
+}
+
+int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n, const or_render_desc* d,
+                      float* out, int threads) {
+    if (!s || !d || (!rays && n)) return 1;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            uint32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            Pcg32 rng;
+            rng.seed(path_initstate(d->seed, d->spp_offset), i);
+            const float* r = rays + 8 * (size_t)i;
+            V3 L = trace_path(*s, *d, rng, load3(r), load3(r + 4));
+            out[3 * (size_t)i] = L.x;
+            out[3 * (size_t)i + 1] = L.y;
+            out[3 * (size_t)i + 2] = L.z;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    return 0;
 }
 
 int or_render(const or_scene* s, const or_render_desc* d, double* out, float* sample0,

@@ -1,0 +1,67 @@
+"""The C++ adapters (kazen_proto_amd/cpp/kazen/*.h) used the way reference
+callers use Accelerator / Integrator / Renderer, checked against the oracle.
+The program is tests/cpp/adapter_check.cpp, built by __graft_entry__.build()."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "bin", "adapter_check")
+
+
+def run_program(tmp_path):
+    assert os.path.exists(EXE), "adapter test program not built (run __graft_entry__.build())"
+    env = dict(os.environ, KP_ADAPTER_PFM=str(tmp_path / "out.pfm"))
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120, env=env)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    out = {}
+    for line in r.stdout.splitlines():
+        k, *v = line.split()
+        out.setdefault(k, []).append(v)
+    return out
+
+
+def test_adapter_program(tmp_path):
+    out = run_program(tmp_path)
+    # Accelerator::intersect on the reference's Embree single-triangle test
+    hit, geom, prim, t, u, v = out["kat_hit"][0]
+    ok, t0, u0, v0 = oracle.intersect_triangle(np.array([0, 1, 0, 0, 0, -1, 0, np.finfo(np.float32).max],
+                                                        np.float32), (-1, 0, 1), (2, 0, 1), (-1, 0, -2))
+    assert hit == "1" and geom == "0" and prim == "0" and ok
+    assert np.float32(t) == np.float32(t0) and np.float32(u) == np.float32(u0) and np.float32(v) == np.float32(v0)
+    assert out["kat_t"][0] == ["1", "1"]
+    assert out["kat_occluded"][0] == ["1", "0"]
+
+    # Integrator::Li: same rays through the oracle integrator, path by path
+    rays = np.array([[float(x) for x in r[1:]] for r in out["ray"]], np.float32)
+    r8 = np.zeros((rays.shape[0], 8), np.float32)
+    r8[:, 0:3] = rays[:, 0:3]
+    r8[:, 4:7] = rays[:, 3:6]
+    r8[:, 7] = np.inf
+    li = np.array([[float(x) for x in r[1:]] for r in out["li"]], np.float64)
+    floor = np.array([[-2, 0, -2, -2, 0, 2, 2, 0, 2], [-2, 0, -2, 2, 0, 2, 2, 0, -2]], np.float32)
+    light = np.array([[-0.5, 2, -0.5, 0.5, 2, -0.5, 0.5, 2, 0.5],
+                      [-0.5, 2, -0.5, 0.5, 2, 0.5, -0.5, 2, 0.5]], np.float32)
+    sc = oracle.Scene(np.concatenate([floor, light]), np.array([0, 0, 1, 1], np.int32),
+                      np.array([-1, -1, 0, 0], np.int32),
+                      [oracle.material(0, albedo=(0.5, 0.5, 0.5)),
+                       oracle.material(0, albedo=(0, 0, 0), Le=(5, 5, 5))])
+    d = oracle.RenderDesc()
+    d.max_depth, d.rr_depth, d.seed, d.spp_offset = 3, 3, 1, 0
+    ref = sc.integrate_rays(r8, d)
+    print("li gpu", li[:4], "oracle", ref[:4])
+    assert np.allclose(li, ref, rtol=2e-3, atol=1e-5)
+    single = np.array([float(x) for x in out["li_single"][0]])
+    assert np.allclose(single, li[0], rtol=1e-6)
+
+    # Renderer::render(SceneGPU&) wrote a film and called the tile callback
+    ok, calls, mean = out["render"][0]
+    assert ok == "1" and calls == "1" and float(mean) > 0.0
+    pfm = (tmp_path / "out.pfm").read_bytes()
+    assert pfm.startswith(b"PF\n16 16\n") and len(pfm) >= 16 * 16 * 12
