@@ -212,6 +212,30 @@ int kp_render_get_stats(kp_ctx* ctx, kp_render_stats* out);
 int kp_integrate_rays(kp_ctx* ctx, const kp_ray* rays, uint32_t n, const kp_render_desc* desc,
                       float* rgb, int ptr_kind);
 
+/* Path guiding -------------------------------------------------------------
+ * No counterpart in the reference (kazen_proto has no path guiding). A small
+ * MLP maps a frequency-encoded position (normalised to [0,1]^3) and normal to
+ * an 8-lobe von Mises-Fisher mixture over directions; forward, backward and
+ * Adam run on the matrix cores in bf16 with FP32 accumulation. in6 per sample:
+ * {px, py, pz, nx, ny, nz}. Parameters: 17,696 FP32 values in the order
+ * W0[48x64] b0 W1[64x64] b1 W2 b2 W3 b3 W4[64x32] b4 (row-major, input-major). */
+#define KP_GUIDE_PARAMS 17696
+int kp_guiding_init(kp_ctx* ctx, uint64_t seed);
+int kp_guiding_set_params(kp_ctx* ctx, const float* params, uint32_t n);
+int kp_guiding_get_params(kp_ctx* ctx, float* params, uint32_t n);
+/* Network outputs (32 per sample); host pointers. */
+int kp_guiding_eval(kp_ctx* ctx, const float* in6, uint32_t n, float* out32);
+/* Mixture density of unit directions (host pointers). */
+int kp_guiding_pdf(kp_ctx* ctx, const float* in6, const float* dirs, uint32_t n, float* pdf);
+/* Mixture samples from uniforms u3 (lobe, two direction), with their density. */
+int kp_guiding_sample(kp_ctx* ctx, const float* in6, const float* u3, uint32_t n, float* dirs,
+                      float* pdf);
+/* Gradient of mean_i(-w_i log q_i(dir_i)) over n <= 262144 samples; with
+ * apply != 0 also one Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8).
+ * grad (optional) receives the gradient in the parameter order above. */
+int kp_guiding_train_step(kp_ctx* ctx, const float* in6, const float* dirs, const float* weights,
+                          uint32_t n, float lr, int apply, float* grad);
+
 /* Device memory and timing helpers for callers without their own HIP
  * runtime handle (the reference has no counterpart: its callers never see
  * device memory). Buffers belong to the caller and must be freed with

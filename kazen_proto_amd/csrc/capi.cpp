@@ -130,6 +130,7 @@ void kp_destroy(kp_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     kp::render_state_free(c);
+    kp::guide_state_free(c);
     kp::free_bvh(c);
     if (c->d_words) (void)hipFree(c->d_words);
     if (c->d_counters) (void)hipFree(c->d_counters);

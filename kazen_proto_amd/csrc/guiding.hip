@@ -1,0 +1,503 @@
+// Path-guiding network kernels: weight packing, inference, training
+// (forward + back-propagation on the matrix cores, deterministic gradient
+// reduction) and Adam.  No reference counterpart (kazen_proto has no path
+// guiding); the methods are named in kp_guide.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/kp_hip.h"
+#include "kp_ctx.h"
+#include "kp_guide.h"
+
+namespace kp {
+
+// ------------------------------------------------------------ packing ----
+// Converts the FP32 masters into the per-lane bf16 matrix-core fragments.
+__global__ void k_guide_pack(const float* __restrict__ P, u32x4* wf, u32x4* wb) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total_f = KP_G_WF_FRAGS * 64, total_b = KP_G_WB_FRAGS * 64;
+    if (tid < total_f) {
+        const int frag = tid / 64, lane = tid % 64, g = lane >> 4, r = lane & 15;
+        int l = frag < 32 ? frag / 8 : 4;
+        const int local = frag - g_wf_base(l);
+        const int t = local / 2, c = local % 2;
+        const int din = g_dim_in(l), dout = g_dim_out(l);
+        const float* W = P + g_layer_offset(l);
+        float v[8];
+        for (int j = 0; j < 8; ++j) {
+            const int k = 32 * c + 8 * g + j, n = 16 * t + r;
+            v[j] = k < din ? W[k * dout + n] : 0.0f;
+        }
+        wf[tid] = u32x4{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7])};
+    } else if (tid < total_f + total_b) {
+        const int id = tid - total_f;
+        const int frag = id / 64, lane = id % 64, g = lane >> 4, r = lane & 15;
+        const int l = 1 + frag / 8;
+        const int local = frag - g_wb_base(l);
+        const int nchunks = g_dim_out(l) / 32;
+        const int kt = local / nchunks, c = local % nchunks;
+        const int dout = g_dim_out(l);
+        const float* W = P + g_layer_offset(l);
+        float v[8];
+        for (int j = 0; j < 8; ++j) v[j] = W[(16 * kt + r) * dout + 32 * c + 8 * g + j];
+        wb[id] = u32x4{pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7])};
+    }
+}
+
+// Writes the 48 encoded features (bf16) of a sample into an LDS row; columns
+// 48..63 are zero.
+__device__ __forceinline__ void g_write_features(uint32_t* row, const float* in6, bool valid) {
+    float f[48];
+    if (valid) g_encode(in6[0], in6[1], in6[2], in6[3], in6[4], in6[5], f);
+#pragma unroll
+    for (int w = 0; w < 24; ++w) row[w] = valid ? pack_bf2(f[2 * w], f[2 * w + 1]) : 0u;
+#pragma unroll
+    for (int w = 24; w < 32; ++w) row[w] = 0u;
+}
+
+// ---------------------------------------------------------- inference ----
+// mode 0: outputs; 1: mixture pdf of dirs; 2: mixture sample from u3.
+__global__ __launch_bounds__(256) void k_guide_eval(GuideNet net, const float* __restrict__ in6, uint32_t n,
+                                                    float* out32, const float* dirs, const float* u3,
+                                                    float* dirs_out, float* pdf_out, int mode) {
+    __shared__ uint32_t lds[4][96 * KP_G_ROW];
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    uint32_t* feat = lds[wave];
+    uint32_t* ping = feat + 64 * KP_G_ROW;
+    uint32_t* pong = ping + 16 * KP_G_ROW;
+    const uint32_t s = (blockIdx.x * 4 + wave) * 64 + lane;
+    const bool valid = s < n;
+    g_write_features(feat + lane * KP_G_ROW, in6 + 6 * (size_t)(valid ? s : 0), valid);
+    uint32_t* act[4] = {ping, pong, ping, pong};
+    for (int q = 0; q < 4; ++q) {
+        uint32_t* rows = feat + 16 * q * KP_G_ROW;
+        g_forward_chunk(net, rows, act, (float*)rows, KP_G_ROW);
+    }
+    const float* o = (const float*)(feat + lane * KP_G_ROW);
+    if (!valid) return;
+    if (mode == 0) {
+        for (int i = 0; i < KP_G_OUT; ++i) out32[(size_t)s * KP_G_OUT + i] = o[i];
+        return;
+    }
+    float ov[KP_G_OUT];
+    for (int i = 0; i < KP_G_OUT; ++i) ov[i] = o[i];
+    Lobe L[KP_G_LOBES];
+    g_mixture(ov, L);
+    if (mode == 1) {
+        pdf_out[s] = g_pdf(L, dirs[3 * (size_t)s], dirs[3 * (size_t)s + 1], dirs[3 * (size_t)s + 2]);
+    } else {
+        float wx, wy, wz;
+        g_sample(L, u3[3 * (size_t)s], u3[3 * (size_t)s + 1], u3[3 * (size_t)s + 2], wx, wy, wz);
+        dirs_out[3 * (size_t)s] = wx;
+        dirs_out[3 * (size_t)s + 1] = wy;
+        dirs_out[3 * (size_t)s + 2] = wz;
+        pdf_out[s] = g_pdf(L, wx, wy, wz);
+    }
+}
+
+// ----------------------------------------------------------- training ----
+// d(-scale * w * log q)/d outputs, same formulas as the oracle.
+__device__ inline void g_output_grad(const float* o, float wx, float wy, float wz, float weight,
+                                     float* gout) {
+    Lobe L[KP_G_LOBES];
+    g_mixture(o, L);
+    float v[KP_G_LOBES];
+    float q = 0.0f;
+#pragma unroll
+    for (int i = 0; i < KP_G_LOBES; ++i) {
+        const float c = L[i].mx * wx + L[i].my * wy + L[i].mz * wz;
+        v[i] = L[i].norm * expf(L[i].kappa * (c - 1.0f));
+        q += L[i].pi * v[i];
+    }
+    q = fmaxf(q, 1e-30f);
+#pragma unroll
+    for (int i = 0; i < KP_G_LOBES; ++i) {
+        const float k = L[i].kappa;
+        const float c = L[i].mx * wx + L[i].my * wy + L[i].mz * wz;
+        gout[4 * i] = -weight * L[i].pi * (v[i] - q) / q;
+        const float dlogc = 1.0f / k - 2.0f / expm1f(2.0f * k);
+        const float dv_dk = v[i] * (dlogc + c - 1.0f);
+        const float b = o[4 * i + 1];
+        const bool clamped = g_softplus(b) + 0.01f > 1.0e4f;
+        const float sig = 1.0f / (1.0f + expf(-b));
+        gout[4 * i + 1] = clamped ? 0.0f : -weight * L[i].pi * dv_dk / q * sig;
+        const float s = -weight * L[i].pi * v[i] * k / q;
+        const float z = L[i].my;
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        const float td = tanhf(o[4 * i + 3]);
+        const float dphi = 3.14159265358979f * (1.0f - td * td);
+        // cos/sin phi from mu
+        const float cp = rr > 0.0f ? L[i].mx / rr : 1.0f, sp = rr > 0.0f ? L[i].mz / rr : 0.0f;
+        gout[4 * i + 2] = s * (-z * rr * cp * wx + (1.0f - z * z) * wy - z * rr * sp * wz);
+        gout[4 * i + 3] = s * (-rr * sp * dphi * wx + rr * cp * dphi * wz);
+    }
+}
+
+__device__ __forceinline__ uint32_t lds_bf16(const uint32_t* buf, int row, int col) {
+    const uint32_t w = buf[row * KP_G_ROW + (col >> 1)];
+    return (col & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// One block = 64 samples (4 waves x 16).  Writes this block's gradient
+// contribution (canonical layout) to partial[blockIdx.x].
+__global__ __launch_bounds__(256) void k_guide_train(GuideNet net, const float* __restrict__ in6,
+                                                     const float* __restrict__ dirs,
+                                                     const float* __restrict__ weights,
+                                                     const uint32_t* n_dev, const float* scale_dev,
+                                                     float* __restrict__ partial) {
+    __shared__ uint32_t act[KP_G_LAYERS][64 * KP_G_ROW];   // layer inputs (bf16)
+    __shared__ uint32_t dyb[2][64 * KP_G_ROW];             // gradients (bf16)
+    __shared__ float outs[64 * KP_G_ROW];                  // network outputs (f32)
+    const uint32_t n = *n_dev;
+    const uint32_t base = blockIdx.x * 64;
+    if (base >= n) return;   // uniform per block
+    const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
+    const int g = lane >> 4, r = lane & 15;
+    float* part = partial + (size_t)blockIdx.x * KP_G_PARAMS;
+    const float scale = *scale_dev;
+
+    if (tid < 64) {
+        const bool valid = base + tid < n;
+        g_write_features(act[0] + tid * KP_G_ROW, in6 + 6 * (size_t)(valid ? base + tid : 0), valid);
+    }
+    __syncthreads();
+    {
+        uint32_t* a[4] = {act[1] + 16 * wave * KP_G_ROW, act[2] + 16 * wave * KP_G_ROW,
+                          act[3] + 16 * wave * KP_G_ROW, act[4] + 16 * wave * KP_G_ROW};
+        g_forward_chunk(net, act[0] + 16 * wave * KP_G_ROW, a, outs + 16 * wave * KP_G_ROW, KP_G_ROW);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t s = base + tid;
+        float gout[KP_G_OUT];
+        if (s < n) {
+            float o[KP_G_OUT];
+            for (int i = 0; i < KP_G_OUT; ++i) o[i] = outs[tid * KP_G_ROW + i];
+            g_output_grad(o, dirs[3 * (size_t)s], dirs[3 * (size_t)s + 1], dirs[3 * (size_t)s + 2],
+                          weights[s] * scale, gout);
+        } else {
+            for (int i = 0; i < KP_G_OUT; ++i) gout[i] = 0.0f;
+        }
+        uint32_t* row = dyb[0] + tid * KP_G_ROW;
+        for (int w = 0; w < 16; ++w) row[w] = pack_bf2(gout[2 * w], gout[2 * w + 1]);
+        for (int w = 16; w < 32; ++w) row[w] = 0u;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int l = KP_G_LAYERS - 1; l >= 0; --l) {
+        const int din = g_dim_in(l), dout = g_dim_out(l);
+        const uint32_t* X = act[l];
+        const uint32_t* dY = dyb[cur];
+        float* gW = part + g_layer_offset(l);
+        float* gb = gW + din * dout;
+        // weight gradient tiles: dW[k][n] = sum_s X[s][k] dY[s][n]
+        const int ntn = dout / 16, tiles = 4 * ntn;
+        for (int tile = wave; tile < tiles; tile += 4) {
+            const int kt = tile / ntn, nt = tile % ntn;
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int sc = 0; sc < 2; ++sc) {
+                uint32_t av[4], bv[4];
+                for (int w = 0; w < 4; ++w) {
+                    const int s0 = 32 * sc + 8 * g + 2 * w;
+                    av[w] = lds_bf16(X, s0, 16 * kt + r) | (lds_bf16(X, s0 + 1, 16 * kt + r) << 16);
+                    bv[w] = lds_bf16(dY, s0, 16 * nt + r) | (lds_bf16(dY, s0 + 1, 16 * nt + r) << 16);
+                }
+                acc = mfma16(u32x4{av[0], av[1], av[2], av[3]}, u32x4{bv[0], bv[1], bv[2], bv[3]}, acc);
+            }
+            for (int j = 0; j < 4; ++j) {
+                const int k = 16 * kt + 4 * g + j;
+                if (k < din) gW[k * dout + 16 * nt + r] = acc[j];
+            }
+        }
+        if (tid < dout) {
+            float sb = 0.0f;
+            for (int s = 0; s < 64; ++s) {
+                const uint32_t w = dY[s * KP_G_ROW + (tid >> 1)];
+                sb += (tid & 1) ? bf_hi(w) : bf_lo(w);
+            }
+            gb[tid] = sb;
+        }
+        if (l > 0) {
+            // dX^T = W dY^T for this wave's 16 samples, masked by the ReLU
+            uint32_t* dX = dyb[cur ^ 1];
+            const int nch = dout / 32;
+            const uint32_t* dyrow = dY + (16 * wave + r) * KP_G_ROW;
+            const uint32_t* xrow = X + (16 * wave + r) * KP_G_ROW;
+            uint32_t* dxrow = dX + (16 * wave + r) * KP_G_ROW;
+            for (int kt = 0; kt < 4; ++kt) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int c = 0; c < nch; ++c) {
+                    const u32x4 b = *(const u32x4*)(dyrow + 16 * c + 4 * g);
+                    acc = mfma16(net.wb[(g_wb_base(l) + kt * nch + c) * 64 + lane], b, acc);
+                }
+                const int k0 = 16 * kt + 4 * g;
+                float v[4];
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t xw = xrow[(k0 + j) >> 1];
+                    const float xv = ((k0 + j) & 1) ? bf_hi(xw) : bf_lo(xw);
+                    v[j] = xv > 0.0f ? acc[j] : 0.0f;
+                }
+                dxrow[k0 / 2] = pack_bf2(v[0], v[1]);
+                dxrow[k0 / 2 + 1] = pack_bf2(v[2], v[3]);
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+// grad = sum of the block contributions (fixed order); optional Adam step.
+__global__ void k_guide_reduce_adam(const float* __restrict__ partial, const uint32_t* n_dev, float* grad,
+                                    float* params, float* m, float* v, int step, float lr, float beta1,
+                                    float beta2, float eps, int apply) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= KP_G_PARAMS) return;
+    const uint32_t nb = (*n_dev + 63) / 64;
+    float gsum = 0.0f;
+    for (uint32_t b = 0; b < nb; ++b) gsum += partial[(size_t)b * KP_G_PARAMS + p];
+    grad[p] = gsum;
+    if (!apply) return;
+    const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+    const float mm = beta1 * m[p] + (1.0f - beta1) * gsum;
+    const float vv = beta2 * v[p] + (1.0f - beta2) * gsum * gsum;
+    m[p] = mm;
+    v[p] = vv;
+    params[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+}  // namespace kp
+
+// =================================================================== host ==
+struct GuideState {
+    float* params = nullptr;   // FP32 masters, canonical layout
+    float* m = nullptr;        // Adam moments
+    float* v = nullptr;
+    float* grad = nullptr;     // last reduced gradient
+    kp::u32x4* wf = nullptr;   // bf16 fragments
+    kp::u32x4* wb = nullptr;
+    float* partial = nullptr;  // per-block gradient contributions
+    uint32_t max_batch = 1u << 18;
+    // staging of a training batch
+    float* in6 = nullptr;
+    float* dirs = nullptr;
+    float* weights = nullptr;
+    uint32_t* count = nullptr;
+    float* scale = nullptr;
+    int step = 0;
+    float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    float3 bmin{0, 0, 0}, binv{1, 1, 1};
+};
+
+namespace kp {
+
+void guide_state_free(kp_ctx* c) {
+    GuideState* g = c->guide;
+    if (!g) return;
+    for (void* p : {(void*)g->params, (void*)g->m, (void*)g->v, (void*)g->grad, (void*)g->wf, (void*)g->wb,
+                    (void*)g->partial, (void*)g->in6, (void*)g->dirs, (void*)g->weights, (void*)g->count,
+                    (void*)g->scale})
+        if (p) (void)hipFree(p);
+    delete g;
+    c->guide = nullptr;
+}
+
+GuideNet guide_net(kp_ctx* c) {
+    GuideNet n;
+    n.wf = c->guide->wf;
+    n.wb = c->guide->wb;
+    n.params = c->guide->params;
+    n.bmin = c->guide->bmin;
+    n.binv = c->guide->binv;
+    return n;
+}
+
+static int guide_pack(kp_ctx* c) {
+    const int total = (KP_G_WF_FRAGS + KP_G_WB_FRAGS) * 64;
+    hipLaunchKernelGGL(k_guide_pack, dim3((total + 255) / 256), dim3(256), 0, c->stream, c->guide->params,
+                       c->guide->wf, c->guide->wb);
+    KP_HIP_TRY(c, hipGetLastError());
+    return KP_OK;
+}
+
+static int guide_alloc(kp_ctx* c) {
+    if (c->guide) return KP_OK;
+    GuideState* g = new GuideState();
+    c->guide = g;
+    const size_t P = KP_G_PARAMS * sizeof(float);
+    KP_HIP_TRY(c, hipMalloc((void**)&g->params, P));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->m, P));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->v, P));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->grad, P));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->wf, KP_G_WF_FRAGS * 64 * sizeof(u32x4)));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->wb, KP_G_WB_FRAGS * 64 * sizeof(u32x4)));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->partial, (size_t)(g->max_batch / 64) * P));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->in6, (size_t)g->max_batch * 6 * sizeof(float)));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->dirs, (size_t)g->max_batch * 3 * sizeof(float)));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->weights, (size_t)g->max_batch * sizeof(float)));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->count, 64));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->scale, 64));
+    return KP_OK;
+}
+
+static int guide_set_params(kp_ctx* c, const float* host) {
+    GuideState* g = c->guide;
+    const size_t P = KP_G_PARAMS * sizeof(float);
+    KP_HIP_TRY(c, hipMemcpyAsync(g->params, host, P, hipMemcpyHostToDevice, c->stream));
+    KP_HIP_TRY(c, hipMemsetAsync(g->m, 0, P, c->stream));
+    KP_HIP_TRY(c, hipMemsetAsync(g->v, 0, P, c->stream));
+    g->step = 0;
+    int rc = guide_pack(c);
+    if (rc) return rc;
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return KP_OK;
+}
+
+// One training step on the batch staged in g->in6/dirs/weights, with the
+// sample count and loss scale read from device memory.
+int guide_train_device(kp_ctx* c, float lr, bool apply) {
+    GuideState* g = c->guide;
+    GuideNet net = guide_net(c);
+    const uint32_t blocks = g->max_batch / 64;
+    hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), 0, c->stream, net, g->in6, g->dirs, g->weights,
+                       g->count, g->scale, g->partial);
+    KP_HIP_TRY(c, hipGetLastError());
+    if (apply) ++g->step;
+    hipLaunchKernelGGL(k_guide_reduce_adam, dim3((KP_G_PARAMS + 255) / 256), dim3(256), 0, c->stream,
+                       g->partial, g->count, g->grad, g->params, g->m, g->v, g->step > 0 ? g->step : 1, lr,
+                       g->beta1, g->beta2, g->eps, apply ? 1 : 0);
+    KP_HIP_TRY(c, hipGetLastError());
+    if (apply) return guide_pack(c);
+    return KP_OK;
+}
+
+static int guide_ready(kp_ctx* c) {
+    if (!c->guide) return fail(c, KP_ERR_STATE, "guiding network not initialised (kp_guiding_init)");
+    return KP_OK;
+}
+
+static int guide_eval_common(kp_ctx* c, const float* in6, uint32_t n, float* out32, const float* dirs,
+                             const float* u3, float* dirs_out, float* pdf_out, int mode) {
+    int rc = guide_ready(c);
+    if (rc) return rc;
+    if (n == 0) return KP_OK;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    float *d_in = nullptr, *d_out = nullptr, *d_dirs = nullptr, *d_u = nullptr, *d_do = nullptr, *d_pdf = nullptr;
+    auto cleanup = [&]() {
+        for (float* p : {d_in, d_out, d_dirs, d_u, d_do, d_pdf})
+            if (p) (void)hipFree(p);
+    };
+    auto up = [&](float** d, const float* h, size_t count) -> hipError_t {
+        hipError_t e = hipMalloc((void**)d, count * sizeof(float));
+        if (e == hipSuccess && h) e = hipMemcpy(*d, h, count * sizeof(float), hipMemcpyHostToDevice);
+        return e;
+    };
+    hipError_t e = up(&d_in, in6, 6 * (size_t)n);
+    if (e == hipSuccess && mode == 0) e = up(&d_out, nullptr, 32 * (size_t)n);
+    if (e == hipSuccess && mode == 1) e = up(&d_dirs, dirs, 3 * (size_t)n);
+    if (e == hipSuccess && mode == 2) e = up(&d_u, u3, 3 * (size_t)n);
+    if (e == hipSuccess && mode == 2) e = up(&d_do, nullptr, 3 * (size_t)n);
+    if (e == hipSuccess && mode >= 1) e = up(&d_pdf, nullptr, (size_t)n);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_guide_eval, dim3((n + 255) / 256), dim3(256), 0, c->stream, guide_net(c), d_in, n,
+                           d_out, d_dirs, d_u, d_do, d_pdf, mode);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && mode == 0) e = hipMemcpy(out32, d_out, 32 * (size_t)n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mode == 2) e = hipMemcpy(dirs_out, d_do, 3 * (size_t)n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mode >= 1) e = hipMemcpy(pdf_out, d_pdf, (size_t)n * 4, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return hip_fail(c, e, "guiding evaluation");
+    return KP_OK;
+}
+
+}  // namespace kp
+
+extern "C" {
+
+int kp_guiding_init(kp_ctx* c, uint64_t seed) {
+    if (!c) return KP_ERR_INVALID;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    int rc = kp::guide_alloc(c);
+    if (rc) return rc;
+    // Glorot-uniform weights from a PCG32 stream, zero biases; the output
+    // layer starts 10x smaller so the first mixture is nearly uniform.
+    std::vector<float> P(KP_G_PARAMS, 0.0f);
+    uint64_t state = 0, inc = (seed << 1u) | 1u;
+    auto next = [&]() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    };
+    next();
+    state += 0x853c49e6748fea9bULL;
+    next();
+    for (int l = 0; l < KP_G_LAYERS; ++l) {
+        const int din = kp::g_dim_in(l), dout = kp::g_dim_out(l);
+        const float lim = std::sqrt(6.0f / (float)(din + dout)) * (l == KP_G_LAYERS - 1 ? 0.1f : 1.0f);
+        float* W = P.data() + kp::g_layer_offset(l);
+        for (int i = 0; i < din * dout; ++i) W[i] = lim * (2.0f * ((float)(next() >> 8) * (1.0f / 16777216.0f)) - 1.0f);
+    }
+    return kp::guide_set_params(c, P.data());
+}
+
+int kp_guiding_set_params(kp_ctx* c, const float* params, uint32_t n) {
+    if (!c || !params) return KP_ERR_INVALID;
+    if (n != KP_G_PARAMS) return kp::fail(c, KP_ERR_INVALID, "guiding parameter count must be 17696");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    int rc = kp::guide_alloc(c);
+    if (rc) return rc;
+    return kp::guide_set_params(c, params);
+}
+
+int kp_guiding_get_params(kp_ctx* c, float* params, uint32_t n) {
+    if (!c || !params) return KP_ERR_INVALID;
+    int rc = kp::guide_ready(c);
+    if (rc) return rc;
+    if (n != KP_G_PARAMS) return kp::fail(c, KP_ERR_INVALID, "guiding parameter count must be 17696");
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    KP_HIP_TRY(c, hipMemcpy(params, c->guide->params, KP_G_PARAMS * 4, hipMemcpyDeviceToHost));
+    return KP_OK;
+}
+
+int kp_guiding_eval(kp_ctx* c, const float* in6, uint32_t n, float* out32) {
+    if (!c || (n && (!in6 || !out32))) return KP_ERR_INVALID;
+    return kp::guide_eval_common(c, in6, n, out32, nullptr, nullptr, nullptr, nullptr, 0);
+}
+
+int kp_guiding_pdf(kp_ctx* c, const float* in6, const float* dirs, uint32_t n, float* pdf) {
+    if (!c || (n && (!in6 || !dirs || !pdf))) return KP_ERR_INVALID;
+    return kp::guide_eval_common(c, in6, n, nullptr, dirs, nullptr, nullptr, pdf, 1);
+}
+
+int kp_guiding_sample(kp_ctx* c, const float* in6, const float* u3, uint32_t n, float* dirs, float* pdf) {
+    if (!c || (n && (!in6 || !u3 || !dirs || !pdf))) return KP_ERR_INVALID;
+    return kp::guide_eval_common(c, in6, n, nullptr, nullptr, u3, dirs, pdf, 2);
+}
+
+int kp_guiding_train_step(kp_ctx* c, const float* in6, const float* dirs, const float* weights, uint32_t n,
+                          float lr, int apply, float* grad_out) {
+    if (!c || (n && (!in6 || !dirs || !weights))) return KP_ERR_INVALID;
+    int rc = kp::guide_ready(c);
+    if (rc) return rc;
+    GuideState* g = c->guide;
+    if (n == 0 || n > g->max_batch) return kp::fail(c, KP_ERR_INVALID, "batch must hold 1..262144 samples");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipMemcpyAsync(g->in6, in6, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    KP_HIP_TRY(c, hipMemcpyAsync(g->dirs, dirs, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    KP_HIP_TRY(c, hipMemcpyAsync(g->weights, weights, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    const float scale = 1.0f / (float)n;   // mean over the batch
+    KP_HIP_TRY(c, hipMemcpyAsync(g->count, &n, 4, hipMemcpyHostToDevice, c->stream));
+    KP_HIP_TRY(c, hipMemcpyAsync(g->scale, &scale, 4, hipMemcpyHostToDevice, c->stream));
+    if ((rc = kp::guide_train_device(c, lr, apply != 0))) return rc;
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (grad_out) KP_HIP_TRY(c, hipMemcpy(grad_out, g->grad, KP_G_PARAMS * 4, hipMemcpyDeviceToHost));
+    return KP_OK;
+}
+
+}  // extern "C"

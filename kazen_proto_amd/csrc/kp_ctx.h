@@ -46,8 +46,9 @@ struct kp_ctx {
     void* d_stage_out = nullptr;
     size_t stage_out_bytes = 0;
 
-    // render state lives in kp_render.cpp
+    // integrator state (render.hip) and guiding network state (guiding.hip)
     struct RenderState* render = nullptr;
+    struct GuideState* guide = nullptr;
 };
 
 namespace kp {
@@ -60,6 +61,7 @@ int ensure_buffer(kp_ctx* c, void** buf, size_t* cap, size_t bytes);
 int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, bool count);
 int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out);
 void render_state_free(kp_ctx* c);
+void guide_state_free(kp_ctx* c);
 // Marks the integrator's device copy of the scene stale (meshes changed).
 void render_scene_dirty(kp_ctx* c);
 }  // namespace kp

@@ -123,6 +123,13 @@ SIGNATURES = {
     "kp_render": (_i32, [_p, _p, _p, _i32]),
     "kp_render_get_stats": (_i32, [_p, _p]),
     "kp_integrate_rays": (_i32, [_p, _p, _u32, _p, _p, _i32]),
+    "kp_guiding_init": (_i32, [_p, _u64]),
+    "kp_guiding_set_params": (_i32, [_p, _p, _u32]),
+    "kp_guiding_get_params": (_i32, [_p, _p, _u32]),
+    "kp_guiding_eval": (_i32, [_p, _p, _u32, _p]),
+    "kp_guiding_pdf": (_i32, [_p, _p, _p, _u32, _p]),
+    "kp_guiding_sample": (_i32, [_p, _p, _p, _u32, _p, _p]),
+    "kp_guiding_train_step": (_i32, [_p, _p, _p, _p, _u32, ctypes.c_float, _i32, _p]),
     "kp_device_alloc": (_i32, [_p, _u64, ctypes.POINTER(_p)]),
     "kp_device_free": (_i32, [_p, _p]),
     "kp_copy_to_device": (_i32, [_p, _p, _p, _u64]),
@@ -364,6 +371,52 @@ class Context:
         s = RenderStats()
         self._check(lib().kp_render_get_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in RenderStats._fields_ if k != "reserved"}
+
+    # ----------------------------------------------------------- guiding --
+    GUIDE_PARAMS = 17696
+
+    def guiding_init(self, seed=0):
+        self._check(lib().kp_guiding_init(self._h, seed))
+
+    def guiding_set_params(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        self._check(lib().kp_guiding_set_params(self._h, _np_ptr(params), params.size))
+
+    def guiding_get_params(self):
+        p = np.zeros(self.GUIDE_PARAMS, np.float32)
+        self._check(lib().kp_guiding_get_params(self._h, _np_ptr(p), p.size))
+        return p
+
+    def guiding_eval(self, in6):
+        in6 = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)
+        out = np.zeros((in6.shape[0], 32), np.float32)
+        self._check(lib().kp_guiding_eval(self._h, _np_ptr(in6), in6.shape[0], _np_ptr(out)))
+        return out
+
+    def guiding_pdf(self, in6, dirs):
+        in6 = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        pdf = np.zeros(in6.shape[0], np.float32)
+        self._check(lib().kp_guiding_pdf(self._h, _np_ptr(in6), _np_ptr(dirs), in6.shape[0], _np_ptr(pdf)))
+        return pdf
+
+    def guiding_sample(self, in6, u3):
+        in6 = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)
+        u3 = np.ascontiguousarray(u3, np.float32).reshape(-1, 3)
+        d = np.zeros((in6.shape[0], 3), np.float32)
+        pdf = np.zeros(in6.shape[0], np.float32)
+        self._check(lib().kp_guiding_sample(self._h, _np_ptr(in6), _np_ptr(u3), in6.shape[0], _np_ptr(d),
+                                            _np_ptr(pdf)))
+        return d, pdf
+
+    def guiding_train_step(self, in6, dirs, weights, lr=1e-3, apply=True):
+        in6 = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        weights = np.ascontiguousarray(weights, np.float32)
+        g = np.zeros(self.GUIDE_PARAMS, np.float32)
+        self._check(lib().kp_guiding_train_step(self._h, _np_ptr(in6), _np_ptr(dirs), _np_ptr(weights),
+                                                in6.shape[0], lr, int(apply), _np_ptr(g)))
+        return g
 
     def device_copy(self, dst, src):
         """Kernel copy between two DeviceBuffers of the same size."""

@@ -64,6 +64,14 @@ def lib():
             "or_bsdf_eval_batch": (None, [_p, _p, _p, _i32, _p, _p]),
             "or_bsdf_sample_batch": (None, [_p, _p, _p, _i32, _p, _p, _p, _p]),
             "or_mf_eval_batch": (None, [_i32, ctypes.c_float, _p, _i32, _p, _p]),
+            "or_mlp_param_count": (_i32, []),
+            "or_mlp_encode": (None, [_p, _p]),
+            "or_mlp_forward": (None, [_p, _i32, _p, _i32, _p]),
+            "or_vmf_pdf": (ctypes.c_double, [_p, _p]),
+            "or_vmf_sample": (None, [_p, _p, _p, _p]),
+            "or_mlp_grad": (ctypes.c_double, [_p, _i32, _p, _p, _p, _i32, _p]),
+            "or_adam_step": (None, [_p, _p, _p, _p, _i32, _i32, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float]),
             "or_scene_create": (_p, [_p, _u32, _p, _p, _p, _u32]),
             "or_scene_free": (None, [_p]),
             "or_render": (_i32, [_p, _p, _p, _p, _i32]),
@@ -263,6 +271,58 @@ def fresnel_dielectric(cos_i, eta):
 
 def erfinv(x):
     return lib().or_erfinv(x)
+
+
+# ---- path-guiding network ----------------------------------------------------
+def mlp_param_count():
+    return int(lib().or_mlp_param_count())
+
+
+def mlp_encode(in6):
+    in6 = np.ascontiguousarray(in6, np.float32)
+    out = np.zeros(48, np.float64)
+    lib().or_mlp_encode(_ptr(in6), _ptr(out))
+    return out
+
+
+def mlp_forward(params, in6, emulate_bf16=True):
+    params = np.ascontiguousarray(params, np.float32)
+    in6 = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)
+    out = np.zeros((in6.shape[0], 32), np.float64)
+    lib().or_mlp_forward(_ptr(params), in6.shape[0], _ptr(in6), int(emulate_bf16), _ptr(out))
+    return out
+
+
+def mlp_grad(params, in6, dirs, weights, emulate_bf16=True):
+    params = np.ascontiguousarray(params, np.float32)
+    in6 = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    weights = np.ascontiguousarray(weights, np.float32)
+    g = np.zeros(params.size, np.float64)
+    loss = lib().or_mlp_grad(_ptr(params), in6.shape[0], _ptr(in6), _ptr(dirs), _ptr(weights),
+                             int(emulate_bf16), _ptr(g))
+    return loss, g
+
+
+def vmf_pdf(out32, d):
+    return lib().or_vmf_pdf(_ptr(np.ascontiguousarray(out32, np.float64)),
+                            _ptr(np.ascontiguousarray(d, np.float32)))
+
+
+def vmf_sample(out32, u3):
+    d = np.zeros(3, np.float32)
+    pdf = ctypes.c_double()
+    lib().or_vmf_sample(_ptr(np.ascontiguousarray(out32, np.float64)),
+                        _ptr(np.ascontiguousarray(u3, np.float32)), _ptr(d), ctypes.byref(pdf))
+    return d, pdf.value
+
+
+def adam_step(params, m, v, grad, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    grad = np.ascontiguousarray(grad, np.float32)
+    lib().or_adam_step(_ptr(params), _ptr(m), _ptr(v), _ptr(grad), params.size, t, lr, beta1,
+                       beta2, eps)
 
 
 class Scene:

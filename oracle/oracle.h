@@ -148,6 +148,23 @@ int or_render(const or_scene* s, const or_render_desc* d, double* out,
 int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n,
                       const or_render_desc* d, float* out, int threads);
 
+/* ---- Path-guiding network (oracle_guide.cpp) ---------------------------- */
+/* in6 per sample: position normalised to [0,1]^3, unit normal. Canonical
+ * parameter layout: W0[48x64] b0 W1[64x64] b1 W2 b2 W3 b3 W4[64x32] b4. With
+ * emulate_bf16 the weights, features, hidden activations and back-propagated
+ * gradients are rounded to bfloat16 the way the GPU kernels feed them to the
+ * matrix cores (accumulation in double). */
+int or_mlp_param_count(void);
+void or_mlp_encode(const float* in6, double* feat48);
+void or_mlp_forward(const float* params, int n, const float* in6, int emulate_bf16,
+                    double* out32);
+double or_vmf_pdf(const double* out32, const float* dir);
+void or_vmf_sample(const double* out32, const float* u3, float* dir, double* pdf);
+double or_mlp_grad(const float* params, int n, const float* in6, const float* dirs,
+                   const float* weights, int emulate_bf16, double* grad);
+void or_adam_step(float* params, float* m, float* v, const float* grad, int n, int t,
+                  float lr, float beta1, float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif
