@@ -34,7 +34,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--leaf", type=int, default=4)
-    ap.add_argument("--refill", type=int, default=4)
+    ap.add_argument("--refill", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy kernel of MIB MiB (reference kernel of "
                          "known traffic for HBM counter calibration)")

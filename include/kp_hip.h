@@ -139,7 +139,7 @@ const void* kp_bvh8_blob_tris(const kp_bvh8_blob* b);
 void kp_bvh8_blob_free(kp_bvh8_blob* b);
 
 /* Tuning: refill a wave once fewer than `lanes` (1..64) of its lanes are
- * busy (default 4). */
+ * busy (default 48). */
 int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
 
 /* Integrator -------------------------------------------------------------- */

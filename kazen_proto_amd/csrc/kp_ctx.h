@@ -33,7 +33,7 @@ struct kp_ctx {
     uint32_t spill_cap = 0;
     size_t spill_lanes = 0;
     uint32_t grid_closest = 0, grid_occluded = 0;
-    uint32_t refill_below = 4;   // measured best for incoherent rays (leaf 4)
+    uint32_t refill_below = 48;   // measured best for incoherent rays (leaf 4)
     uint32_t max_steps = 64;
 
     // small device words: [0] work counter, [1] error flags

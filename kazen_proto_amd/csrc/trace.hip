@@ -210,6 +210,60 @@ struct Stack {
 
 constexpr int kStack = KP_TRACE_STACK;
 
+// Work distribution of the persistent kernels.  A wave takes chunks of
+// kChunk consecutive rays from the global counter (one atomic per chunk) and
+// hands them to its lanes from a wave-uniform cursor; every lane keeps its
+// next ray prefetched in registers, so replacing a finished ray costs no
+// memory round trip.  All members except the prefetch are wave-uniform.
+constexpr uint32_t kChunk = 256;
+
+struct RayFeeder {
+    uint32_t next, end, n;
+    bool drained;
+    uint32_t* counter;
+    const float4* rays;
+    bool pending;       // this lane holds a prefetched ray
+    uint32_t pid;
+    float4 p0, p1;
+
+    __device__ __forceinline__ void init(uint32_t* ctr, const float4* rs, uint32_t count) {
+        counter = ctr;
+        rays = rs;
+        n = count;
+        next = end = 0;
+        drained = false;
+        pending = false;
+        pid = KP_INVALID;
+        assign();
+    }
+    // Gives a prefetched ray to every lane without one, as far as rays last.
+    __device__ __forceinline__ void assign() {
+        const uint64_t m = __ballot(!pending);
+        if (m == 0 || drained) return;
+        if (next >= end) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(counter, kChunk);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base >= n) {
+                drained = true;
+                return;
+            }
+            next = base;
+            end = base + kChunk < n ? base + kChunk : n;
+        }
+        const uint32_t avail = end - next;
+        const uint32_t want = (uint32_t)__popcll(m);
+        const uint32_t prefix = (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        if (!pending && prefix < avail) {
+            pid = next + prefix;
+            p0 = rays[2 * (size_t)pid];
+            p1 = rays[2 * (size_t)pid + 1];
+            pending = true;
+        }
+        next += want < avail ? want : avail;
+    }
+};
+
 template <bool kCount>
 __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
     __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
@@ -227,38 +281,28 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
     const float4* __restrict__ tris = args.tris;
 
     uint32_t ray_id = KP_INVALID;
-    bool exhausted = false;
     RayPre r;
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     uint32_t best_p = KP_INVALID, cur = KP_INVALID, steps = 0;
     uint64_t cnt_nodes = 0, cnt_tris = 0;
+    (void)lane;
+    RayFeeder feed;
+    feed.init(args.counter, args.rays, n_rays);
 
     for (;;) {
-        const bool need = (ray_id == KP_INVALID) && !exhausted;
-        const uint64_t m = __ballot(need);
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(args.counter, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (need) {
-                const uint32_t my = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (my < n_rays) {
-                    ray_id = my;
-                    const float4 ra = args.rays[2 * (size_t)my], rb = args.rays[2 * (size_t)my + 1];
-                    setup_ray(ra, rb, r);
-                    best_t = r.tmax;
-                    best_p = KP_INVALID;
-                    bu = bv = 0.0f;
-                    cur = ray_valid(ra, rb) ? 0u : KP_INVALID;   // root
-                    st.sp = 0;
-                    steps = 0;
-                } else {
-                    exhausted = true;
-                }
-            }
+        if (ray_id == KP_INVALID && feed.pending) {
+            ray_id = feed.pid;
+            setup_ray(feed.p0, feed.p1, r);
+            best_t = r.tmax;
+            best_p = KP_INVALID;
+            bu = bv = 0.0f;
+            cur = ray_valid(feed.p0, feed.p1) ? 0u : KP_INVALID;   // root
+            st.sp = 0;
+            steps = 0;
+            feed.pending = false;
         }
-        if (__ballot(ray_id != KP_INVALID) == 0) break;
+        feed.assign();
+        if (__ballot(ray_id != KP_INVALID || feed.pending) == 0) break;
         if (ray_id != KP_INVALID) {
             for (;;) {
                 bool done = false;
@@ -344,32 +388,22 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
     const float4* __restrict__ tris = args.tris;
 
     uint32_t ray_id = KP_INVALID;
-    bool exhausted = false;
     RayPre r;
     uint32_t cur = KP_INVALID, steps = 0;
+    (void)lane;
+    RayFeeder feed;
+    feed.init(args.counter, args.rays, n_rays);
     for (;;) {
-        const bool need = (ray_id == KP_INVALID) && !exhausted;
-        const uint64_t m = __ballot(need);
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(args.counter, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (need) {
-                const uint32_t my = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (my < n_rays) {
-                    ray_id = my;
-                    const float4 ra = args.rays[2 * (size_t)my], rb = args.rays[2 * (size_t)my + 1];
-                    setup_ray(ra, rb, r);
-                    cur = ray_valid(ra, rb) ? 0u : KP_INVALID;
-                    st.sp = 0;
-                    steps = 0;
-                } else {
-                    exhausted = true;
-                }
-            }
+        if (ray_id == KP_INVALID && feed.pending) {
+            ray_id = feed.pid;
+            setup_ray(feed.p0, feed.p1, r);
+            cur = ray_valid(feed.p0, feed.p1) ? 0u : KP_INVALID;
+            st.sp = 0;
+            steps = 0;
+            feed.pending = false;
         }
-        if (__ballot(ray_id != KP_INVALID) == 0) break;
+        feed.assign();
+        if (__ballot(ray_id != KP_INVALID || feed.pending) == 0) break;
         if (ray_id != KP_INVALID) {
             for (;;) {
                 bool hit = false, done = false;

@@ -30,7 +30,8 @@ int main() {
         std::printf("kat_hit %d %zu %zu %.9g %.9g %.9g\n", hit ? 1 : 0, is.geom_id, is.prim_id, is.ray_t,
                     is.uv[0], is.uv[1]);
         float t = 0;
-        std::printf("kat_t %d %.9g\n", kat.accelerator->intersect(Ray(Vec3f(0, 1, 0), Vec3f(0, -1, 0), 0.0f), t) ? 1 : 0, t);
+        const bool hit_t = kat.accelerator->intersect(Ray(Vec3f(0, 1, 0), Vec3f(0, -1, 0), 0.0f), t);
+        std::printf("kat_t %d %.9g\n", hit_t ? 1 : 0, t);
         std::printf("kat_occluded %d %d\n", kat.accelerator->occluded(Vec3f(0, 1, 0), Vec3f(0, -1, 0)) ? 1 : 0,
                     kat.accelerator->occluded(Vec3f(0, 1, 0), Vec3f(0, 0.5f, 0)) ? 1 : 0);
 
