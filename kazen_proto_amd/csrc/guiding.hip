@@ -70,31 +70,42 @@ __global__ __launch_bounds__(256) void k_guide_eval(GuideNet net, const float* _
     uint32_t* pong = ping + 16 * KP_G_ROW;
     const uint32_t s = (blockIdx.x * 4 + wave) * 64 + lane;
     const bool valid = s < n;
-    g_write_features(feat + lane * KP_G_ROW, in6 + 6 * (size_t)(valid ? s : 0), valid);
-    uint32_t* act[4] = {ping, pong, ping, pong};
-    for (int q = 0; q < 4; ++q) {
-        uint32_t* rows = feat + 16 * q * KP_G_ROW;
-        g_forward_chunk(net, rows, act, (float*)rows, KP_G_ROW);
-    }
-    const float* o = (const float*)(feat + lane * KP_G_ROW);
-    if (!valid) return;
     if (mode == 0) {
-        for (int i = 0; i < KP_G_OUT; ++i) out32[(size_t)s * KP_G_OUT + i] = o[i];
+        g_write_features(feat + lane * KP_G_ROW, in6 + 6 * (size_t)(valid ? s : 0), valid);
+        uint32_t* act[4] = {ping, pong, ping, pong};
+        for (int q = 0; q < 4; ++q) {
+            uint32_t* rows = feat + 16 * q * KP_G_ROW;
+            g_forward_chunk(net, rows, act, (float*)rows, KP_G_ROW);
+        }
+        const float* o = (const float*)(feat + lane * KP_G_ROW);
+        if (valid)
+            for (int i = 0; i < KP_G_OUT; ++i) out32[(size_t)s * KP_G_OUT + i] = o[i];
         return;
     }
-    float ov[KP_G_OUT];
-    for (int i = 0; i < KP_G_OUT; ++i) ov[i] = o[i];
-    Lobe L[KP_G_LOBES];
-    g_mixture(ov, L);
+    // the wave-level query routine the guided shade kernel uses
+    const size_t si = valid ? s : 0;
+    GuideQuery q;
+    q.px = in6[6 * si]; q.py = in6[6 * si + 1]; q.pz = in6[6 * si + 2];
+    q.nx = in6[6 * si + 3]; q.ny = in6[6 * si + 4]; q.nz = in6[6 * si + 5];
+    q.sample = mode == 2;
     if (mode == 1) {
-        pdf_out[s] = g_pdf(L, dirs[3 * (size_t)s], dirs[3 * (size_t)s + 1], dirs[3 * (size_t)s + 2]);
+        q.ax = dirs[3 * si]; q.ay = dirs[3 * si + 1]; q.az = dirs[3 * si + 2];
+        q.u0 = q.u1 = q.u2 = 0.0f;
     } else {
-        float wx, wy, wz;
-        g_sample(L, u3[3 * (size_t)s], u3[3 * (size_t)s + 1], u3[3 * (size_t)s + 2], wx, wy, wz);
-        dirs_out[3 * (size_t)s] = wx;
-        dirs_out[3 * (size_t)s + 1] = wy;
-        dirs_out[3 * (size_t)s + 2] = wz;
-        pdf_out[s] = g_pdf(L, wx, wy, wz);
+        q.ax = q.ay = q.az = 0.0f;
+        q.u0 = u3[3 * si]; q.u1 = u3[3 * si + 1]; q.u2 = u3[3 * si + 2];
+    }
+    q.bx = q.ax; q.by = q.ay; q.bz = q.az;
+    GuideAnswer a;
+    g_wave_guide(net, feat, q, a);
+    if (!valid) return;
+    if (mode == 1) {
+        pdf_out[s] = a.pdf_a;
+    } else {
+        dirs_out[3 * (size_t)s] = a.dx;
+        dirs_out[3 * (size_t)s + 1] = a.dy;
+        dirs_out[3 * (size_t)s + 2] = a.dz;
+        pdf_out[s] = a.pdf_b;
     }
 }
 
@@ -102,35 +113,36 @@ __global__ __launch_bounds__(256) void k_guide_eval(GuideNet net, const float* _
 // d(-scale * w * log q)/d outputs, same formulas as the oracle.
 __device__ inline void g_output_grad(const float* o, float wx, float wy, float wz, float weight,
                                      float* gout) {
-    Lobe L[KP_G_LOBES];
-    g_mixture(o, L);
+    const MixNorm nrm = g_mix_norm(o);
     float v[KP_G_LOBES];
     float q = 0.0f;
 #pragma unroll
     for (int i = 0; i < KP_G_LOBES; ++i) {
-        const float c = L[i].mx * wx + L[i].my * wy + L[i].mz * wz;
-        v[i] = L[i].norm * expf(L[i].kappa * (c - 1.0f));
-        q += L[i].pi * v[i];
+        const LobeP L = g_lobe(o, nrm, i);
+        const float c = L.mx * wx + L.my * wy + L.mz * wz;
+        v[i] = L.norm * expf(L.kappa * (c - 1.0f));
+        q += L.pi * v[i];
     }
     q = fmaxf(q, 1e-30f);
 #pragma unroll
     for (int i = 0; i < KP_G_LOBES; ++i) {
-        const float k = L[i].kappa;
-        const float c = L[i].mx * wx + L[i].my * wy + L[i].mz * wz;
-        gout[4 * i] = -weight * L[i].pi * (v[i] - q) / q;
+        const LobeP L = g_lobe(o, nrm, i);
+        const float k = L.kappa;
+        const float c = L.mx * wx + L.my * wy + L.mz * wz;
+        gout[4 * i] = -weight * L.pi * (v[i] - q) / q;
         const float dlogc = 1.0f / k - 2.0f / expm1f(2.0f * k);
         const float dv_dk = v[i] * (dlogc + c - 1.0f);
         const float b = o[4 * i + 1];
         const bool clamped = g_softplus(b) + 0.01f > 1.0e4f;
         const float sig = 1.0f / (1.0f + expf(-b));
-        gout[4 * i + 1] = clamped ? 0.0f : -weight * L[i].pi * dv_dk / q * sig;
-        const float s = -weight * L[i].pi * v[i] * k / q;
-        const float z = L[i].my;
+        gout[4 * i + 1] = clamped ? 0.0f : -weight * L.pi * dv_dk / q * sig;
+        const float s = -weight * L.pi * v[i] * k / q;
+        const float z = L.my;
         const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
         const float td = tanhf(o[4 * i + 3]);
         const float dphi = 3.14159265358979f * (1.0f - td * td);
         // cos/sin phi from mu
-        const float cp = rr > 0.0f ? L[i].mx / rr : 1.0f, sp = rr > 0.0f ? L[i].mz / rr : 0.0f;
+        const float cp = rr > 0.0f ? L.mx / rr : 1.0f, sp = rr > 0.0f ? L.mz / rr : 0.0f;
         gout[4 * i + 2] = s * (-z * rr * cp * wx + (1.0f - z * z) * wy - z * rr * sp * wz);
         gout[4 * i + 3] = s * (-rr * sp * dphi * wx + rr * cp * dphi * wz);
     }

@@ -105,10 +105,14 @@ __device__ __forceinline__ void g_encode(float px, float py, float pz, float nx,
 // bf16 activations to act[l] (same shape); the 32 outputs of sample r are
 // written to out[r*out_stride + o] (f32).  Every lane of the wave must call it.
 __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin, uint32_t* const* act,
-                                       float* out, int out_stride) {
+                                       float* out, int out_stride, float* last8 = nullptr) {
     const int lane = __lane_id();
     const int g = lane >> 4, r = lane & 15;
     const uint32_t* src = xin;
+    // Launder the fragment pointer so the weight loads are not hoisted out of
+    // a caller's chunk loop (they would pin 144 registers per lane).
+    const u32x4* wf = net.wf;
+    asm volatile("" : "+s"(wf));
 #pragma unroll
     for (int l = 0; l < KP_G_LAYERS; ++l) {
         const int ntiles = g_dim_out(l) / 16;
@@ -117,10 +121,11 @@ __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin,
         const u32x4 b0 = *(const u32x4*)(src + r * KP_G_ROW + 4 * g);
         const u32x4 b1 = *(const u32x4*)(src + r * KP_G_ROW + 16 + 4 * g);
         uint32_t* dst = l < KP_G_LAYERS - 1 ? act[l] : nullptr;
+#pragma unroll
         for (int t = 0; t < ntiles; ++t) {
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma16(net.wf[(g_wf_base(l) + 2 * t) * 64 + lane], b0, acc);
-            acc = mfma16(net.wf[(g_wf_base(l) + 2 * t + 1) * 64 + lane], b1, acc);
+            acc = mfma16(wf[(g_wf_base(l) + 2 * t) * 64 + lane], b0, acc);
+            acc = mfma16(wf[(g_wf_base(l) + 2 * t + 1) * 64 + lane], b1, acc);
             const int n0 = 16 * t + 4 * g;
             float v[4];
 #pragma unroll
@@ -130,9 +135,13 @@ __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin,
                 for (int j = 0; j < 4; ++j) v[j] = fmaxf(v[j], 0.0f);
                 dst[r * KP_G_ROW + n0 / 2] = pack_bf2(v[0], v[1]);
                 dst[r * KP_G_ROW + n0 / 2 + 1] = pack_bf2(v[2], v[3]);
-            } else {
+            } else if (out) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) out[r * out_stride + n0 + j] = v[j];
+            } else {
+                // outputs 16t + 4g + j: this lane holds lobes g (t = 0) and g + 4 (t = 1)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) last8[4 * t + j] = v[j];
             }
         }
         src = dst;
@@ -140,67 +149,77 @@ __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin,
 }
 
 // ------------------------------------------------------------ mixture ----
-struct Lobe {
-    float pi, kappa, norm;
-    float mx, my, mz;
-};
+// The mixture is evaluated straight from the 32 network outputs `o` (which
+// may live in LDS), one lobe at a time, so no per-lane lobe array is needed.
+// Lobe i: pi_i = softmax(o[4i]) (normalised in FP64), kappa_i =
+// min(softplus(o[4i+1]) + 0.01, 1e4), mu_i = (r cos phi, z, r sin phi) with
+// z = tanh(o[4i+2]), phi = pi tanh(o[4i+3]).
 
 __device__ __forceinline__ float g_softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
 
-// Mixture from the 32 outputs of a sample; the softmax normalisation is done
-// in FP64.
-__device__ inline void g_mixture(const float* o, Lobe* L) {
-    float mx = o[0];
+struct MixNorm {
+    float mx;        // max logit
+    double inv_sum;  // 1 / sum exp(logit - mx)
+};
+
+__device__ __forceinline__ MixNorm g_mix_norm(const float* o) {
+    MixNorm n;
+    n.mx = o[0];
 #pragma unroll
-    for (int i = 1; i < KP_G_LOBES; ++i) mx = fmaxf(mx, o[4 * i]);
-    double e[KP_G_LOBES], sum = 0.0;
+    for (int i = 1; i < KP_G_LOBES; ++i) n.mx = fmaxf(n.mx, o[4 * i]);
+    double sum = 0.0;
 #pragma unroll
-    for (int i = 0; i < KP_G_LOBES; ++i) {
-        e[i] = exp((double)(o[4 * i] - mx));
-        sum += e[i];
-    }
-#pragma unroll
-    for (int i = 0; i < KP_G_LOBES; ++i) {
-        L[i].pi = (float)(e[i] / sum);
-        const float k = fminf(g_softplus(o[4 * i + 1]) + 0.01f, 1.0e4f);
-        L[i].kappa = k;
-        L[i].norm = k / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * k)));
-        const float z = tanhf(o[4 * i + 2]);
-        const float phi = 3.14159265358979f * tanhf(o[4 * i + 3]);
-        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        float sp, cp;
-        sincosf(phi, &sp, &cp);
-        L[i].mx = rr * cp;
-        L[i].my = z;
-        L[i].mz = rr * sp;
-    }
+    for (int i = 0; i < KP_G_LOBES; ++i) sum += exp((double)(o[4 * i] - n.mx));
+    n.inv_sum = 1.0 / sum;
+    return n;
 }
 
-__device__ __forceinline__ float g_pdf(const Lobe* L, float wx, float wy, float wz) {
+struct LobeP {
+    float pi, kappa, norm, mx, my, mz;
+};
+
+__device__ __forceinline__ LobeP g_lobe(const float* o, const MixNorm& n, int i) {
+    LobeP L;
+    L.pi = (float)(exp((double)(o[4 * i] - n.mx)) * n.inv_sum);
+    const float k = fminf(g_softplus(o[4 * i + 1]) + 0.01f, 1.0e4f);
+    L.kappa = k;
+    L.norm = k / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * k)));
+    const float z = tanhf(o[4 * i + 2]);
+    const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sp, cp;
+    sincosf(3.14159265358979f * tanhf(o[4 * i + 3]), &sp, &cp);
+    L.mx = rr * cp;
+    L.my = z;
+    L.mz = rr * sp;
+    return L;
+}
+
+// Mixture density of the unit direction w.
+__device__ inline float g_mix_pdf(const float* o, float wx, float wy, float wz) {
+    const MixNorm n = g_mix_norm(o);
     float q = 0.0f;
-#pragma unroll
     for (int i = 0; i < KP_G_LOBES; ++i) {
-        const float c = L[i].mx * wx + L[i].my * wy + L[i].mz * wz;
-        q += L[i].pi * L[i].norm * expf(L[i].kappa * (c - 1.0f));
+        const LobeP L = g_lobe(o, n, i);
+        const float c = L.mx * wx + L.my * wy + L.mz * wz;
+        q += L.pi * L.norm * expf(L.kappa * (c - 1.0f));
     }
     return q;
 }
 
-// Lobe by inverse CDF of pi, then the vMF sample (W. Jakob 2012).
-__device__ inline void g_sample(const Lobe* L, float u0, float u1, float u2, float& wx, float& wy,
-                                float& wz) {
+// Lobe by inverse CDF of pi (u0), then the vMF sample (W. Jakob 2012).
+__device__ inline void g_mix_sample(const float* o, float u0, float u1, float u2, float& wx, float& wy,
+                                    float& wz) {
+    const MixNorm n = g_mix_norm(o);
     int lobe = KP_G_LOBES - 1;
-    float acc = 0.0f;
-#pragma unroll
-    for (int i = 0; i < KP_G_LOBES; ++i) {
-        acc += L[i].pi;
-        if (u0 < acc && lobe == KP_G_LOBES - 1 && i < KP_G_LOBES - 1) lobe = i;
+    double acc = 0.0;
+    for (int i = 0; i < KP_G_LOBES - 1; ++i) {
+        acc += exp((double)(o[4 * i] - n.mx)) * n.inv_sum;
+        if ((double)u0 < acc) {
+            lobe = i;
+            break;
+        }
     }
-    // (the loop keeps the first i with u0 < acc; lobe 7 otherwise)
-    Lobe l = L[0];
-#pragma unroll
-    for (int i = 1; i < KP_G_LOBES; ++i)
-        if (i == lobe) l = L[i];
+    const LobeP l = g_lobe(o, n, lobe);
     const float k = l.kappa;
     const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
     const float sr = sqrtf(fmaxf(0.0f, 1.0f - w * w));
@@ -216,13 +235,168 @@ __device__ inline void g_sample(const Lobe* L, float u0, float u1, float u2, flo
         tx = 0.0f; ty = l.mz * inv; tz = -l.my * inv;
     }
     const float bx = ty * l.mz - tz * l.my, by = tz * l.mx - tx * l.mz, bz = tx * l.my - ty * l.mx;
-    float dx = lx * tx + ly * bx + w * l.mx;
-    float dy = lx * ty + ly * by + w * l.my;
-    float dz = lx * tz + ly * bz + w * l.mz;
+    const float dx = lx * tx + ly * bx + w * l.mx;
+    const float dy = lx * ty + ly * by + w * l.my;
+    const float dz = lx * tz + ly * bz + w * l.mz;
     const float len = sqrtf(dx * dx + dy * dy + dz * dz);
     wx = dx / len;
     wy = dy / len;
     wz = dz / len;
+}
+
+// ---------------------------------------------- wave-level guide queries --
+// Features of one sample written straight into an LDS row as bf16 pairs
+// (sin, cos), columns 48..63 zero.
+__device__ __forceinline__ void g_encode_row(uint32_t* row, float px, float py, float pz, float nx, float ny,
+                                             float nz) {
+    const float pc[3] = {px, py, pz};
+    const float nc[3] = {nx, ny, nz};
+    int w = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            float s, co;
+            sincospif(pc[c] * (float)(1 << k), &s, &co);
+            row[w++] = pack_bf2(s, co);
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            float s, co;
+            sincospif(nc[c] * (float)(1 << k), &s, &co);
+            row[w++] = pack_bf2(s, co);
+        }
+#pragma unroll
+    for (; w < 32; ++w) row[w] = 0u;
+}
+
+// Per-lane query of the guiding distribution at one shading point.
+struct GuideQuery {
+    float px, py, pz, nx, ny, nz;   // position in [0,1]^3, unit normal
+    float ax, ay, az;               // direction A: density wanted (zero = none)
+    float bx, by, bz;               // direction B: density wanted, unless sampling
+    bool sample;                    // draw a direction (u0: lobe, u1, u2) instead of B
+    float u0, u1, u2;
+};
+
+struct GuideAnswer {
+    float pdf_a, pdf_b;             // densities at A and at B (or at the sample)
+    float dx, dy, dz;               // the sampled direction
+};
+
+// Density contribution of lobe (raw outputs o4) for the unit direction w.
+__device__ __forceinline__ float g_lobe_term(const float* o4, float pi, float wx, float wy, float wz) {
+    const float k = fminf(g_softplus(o4[1]) + 0.01f, 1.0e4f);
+    const float norm = k / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * k)));
+    const float z = tanhf(o4[2]);
+    const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sp, cp;
+    sincosf(3.14159265358979f * tanhf(o4[3]), &sp, &cp);
+    const float c = rr * cp * wx + z * wy + rr * sp * wz;
+    return pi * norm * expf(k * (c - 1.0f));
+}
+
+__device__ __forceinline__ float g_sum4(float v) {
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+// Answers every lane's query.  Each 16-lane group's samples go through the
+// network together (4 chunks); after the last layer lane (g, r) of chunk q
+// holds lobes g and g+4 of the query of lane 16q + r, and the mixture sums
+// are reduced over the four lanes of a sample with xor-shuffles.  `lds` is
+// this wave's 96 rows (KP_G_ROW words each).  Every lane must call it.
+__device__ inline void g_wave_guide(const GuideNet& net, uint32_t* lds, const GuideQuery& q, GuideAnswer& ans) {
+    const int lane = __lane_id();
+    const int g = lane >> 4, r = lane & 15;
+    uint32_t* feat = lds;
+    uint32_t* ping = lds + 64 * KP_G_ROW;
+    uint32_t* pong = ping + 16 * KP_G_ROW;
+    g_encode_row(feat + lane * KP_G_ROW, q.px, q.py, q.pz, q.nx, q.ny, q.nz);
+    uint32_t* act[4] = {ping, pong, ping, pong};
+    ans.pdf_a = ans.pdf_b = 0.0f;
+    ans.dx = ans.dy = 0.0f;
+    ans.dz = 1.0f;
+    for (int c = 0; c < 4; ++c) {
+        float o[8];
+        g_forward_chunk(net, feat + 16 * c * KP_G_ROW, act, nullptr, 0, o);
+        // the query of sample r of this chunk lives in lane 16c + r
+        const int owner = 16 * c + r;
+        const float ax = __shfl(q.ax, owner), ay = __shfl(q.ay, owner), az = __shfl(q.az, owner);
+        const float bx = __shfl(q.bx, owner), by = __shfl(q.by, owner), bz = __shfl(q.bz, owner);
+        const int smp = __shfl((int)q.sample, owner);
+        const float u0 = __shfl(q.u0, owner), u1 = __shfl(q.u1, owner), u2 = __shfl(q.u2, owner);
+        // softmax over the 8 logits of the sample (sum in FP64)
+        float mx = fmaxf(o[0], o[4]);
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        const double e0 = exp((double)(o[0] - mx)), e1 = exp((double)(o[4] - mx));
+        double sum = e0 + e1;
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        const float pi0 = (float)(e0 / sum), pi1 = (float)(e1 / sum);
+        const float pa = g_sum4(g_lobe_term(o, pi0, ax, ay, az) + g_lobe_term(o + 4, pi1, ax, ay, az));
+        float sx = bx, sy = by, sz = bz;
+        if (smp) {
+            // lobe i lives in lane 16*(i%4) + r, slot i/4: gather all weights
+            float pis[8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                pis[i] = __shfl(pi0, 16 * i + r);
+                pis[4 + i] = __shfl(pi1, 16 * i + r);
+            }
+            int sel = KP_G_LOBES - 1;
+            float acc = 0.0f;
+#pragma unroll
+            for (int i = 0; i < KP_G_LOBES - 1; ++i) {
+                acc += pis[i];
+                if (sel == KP_G_LOBES - 1 && u0 < acc) sel = i;
+            }
+            const int src = 16 * (sel & 3) + r;
+            const bool hi = sel >= 4;
+            float ob[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) ob[j] = __shfl(hi ? o[4 + j] : o[j], src);
+            const float k = fminf(g_softplus(ob[1]) + 0.01f, 1.0e4f);
+            const float z = tanhf(ob[2]);
+            const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            float sp, cp;
+            sincosf(3.14159265358979f * tanhf(ob[3]), &sp, &cp);
+            const float mxx = rr * cp, myy = z, mzz = rr * sp;
+            const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
+            const float srr = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+            float s2, c2;
+            sincosf(2.0f * 3.14159265358979f * u2, &s2, &c2);
+            const float lx = srr * c2, ly = srr * s2;
+            float tx, ty, tz;
+            if (fabsf(mxx) > fabsf(myy)) {
+                const float inv = 1.0f / sqrtf(mxx * mxx + mzz * mzz);
+                tx = mzz * inv; ty = 0.0f; tz = -mxx * inv;
+            } else {
+                const float inv = 1.0f / sqrtf(myy * myy + mzz * mzz);
+                tx = 0.0f; ty = mzz * inv; tz = -myy * inv;
+            }
+            const float bxx = ty * mzz - tz * myy, byy = tz * mxx - tx * mzz, bzz = tx * myy - ty * mxx;
+            const float dx = lx * tx + ly * bxx + w * mxx;
+            const float dy = lx * ty + ly * byy + w * myy;
+            const float dz = lx * tz + ly * bzz + w * mzz;
+            const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+            sx = dx / len;
+            sy = dy / len;
+            sz = dz / len;
+        }
+        const float pb = g_sum4(g_lobe_term(o, pi0, sx, sy, sz) + g_lobe_term(o + 4, pi1, sx, sy, sz));
+        if (g == c) {   // this lane owns sample r of chunk c
+            ans.pdf_a = pa;
+            ans.pdf_b = pb;
+            ans.dx = sx;
+            ans.dy = sy;
+            ans.dz = sz;
+        }
+    }
 }
 
 }  // namespace kp
