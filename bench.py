@@ -26,6 +26,98 @@ HBM_PEAK_GBS = 8000.0
 BASELINE_METRIC = "incoherent Mrays/s (1 GPU); equal-relMSE time, guided vs plain GPU tracer"
 
 
+def rel_mse(img, ref):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)"""
+    img = img.astype(np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def guiding_equal_error(args):
+    """Wall time for the plain and the guided GPU tracer to reach relMSE
+    args.guiding_target on the slit-lit glossy room (64 GGX spheres, depth 10)
+    against a plain-tracer reference of args.guiding_ref_spp samples per pixel.
+    Both tracers accumulate chunks of samples into the film; only the
+    kp_render_accumulate wall time counts (network training included for the
+    guided one), the film reads and error evaluations between chunks do not."""
+    from kazen_proto_amd import native, scenes
+    res = args.guiding_res
+    s = scenes.guiding_scene(width=res, height=res, spp=1, max_depth=10)
+    c = native.Context(0)
+    c.load_scene(s)
+    t0 = time.time()
+    c.film_reset(res, res)
+    chunk = 256
+    done = 0
+    ref_sec = 0.0
+    paths = rays = 0
+    while done < args.guiding_ref_spp:
+        n = min(chunk, args.guiding_ref_spp - done)
+        d = native.make_render_desc(s, spp=n, spp_offset=done)
+        d.seed = s.seed + 7919
+        c.render_accumulate(d)
+        st = c.render_stats()
+        ref_sec += st["seconds"]
+        paths += st["paths"]
+        rays += st["extension_rays"] + st["shadow_rays"]
+        done += n
+        print("[bench] guiding reference %d/%d spp (%.1fs)" % (done, args.guiding_ref_spp, ref_sec),
+              file=sys.stderr, flush=True)
+    ref = c.film_read(res, res).astype(np.float64)
+    plain_rate = {"samples_per_s": round(paths / ref_sec, 1), "mrays_per_s": round(rays / ref_sec / 1e6, 2)}
+
+    def converge(guided):
+        if guided:
+            c.guiding_init(args.seed_guide)
+            c.guiding_configure(bsdf_fraction=0.5, min_alpha=0.1, lr=args.guiding_lr,
+                                train_steps=args.guiding_steps, guide_after=args.guiding_after,
+                                max_records=1 << 18, pass_paths=1 << 18)
+        c.film_reset(res, res)
+        spp = 0
+        sec = 0.0
+        steps = 0
+        curve = []
+        step_spp = 4
+        while True:
+            d = native.make_render_desc(s, spp=step_spp, spp_offset=spp, guided=guided)
+            c.render_accumulate(d)
+            st = c.render_stats()
+            sec += st["seconds"]
+            steps += st["train_steps"]
+            spp += step_spp
+            e = rel_mse(c.film_read(res, res), ref)
+            curve.append((spp, sec, e))
+            if e <= args.guiding_target or sec > args.guiding_max_seconds:
+                break
+            if spp >= 64:
+                step_spp = 16
+        t_hit = None
+        if curve[-1][2] <= args.guiding_target:
+            if len(curve) == 1:
+                t_hit = curve[0][1]
+            else:
+                (_, t1, e1), (_, t2, e2) = curve[-2], curve[-1]
+                # log-log interpolation between the two chunks around the target
+                f = (np.log(e1) - np.log(args.guiding_target)) / max(np.log(e1) - np.log(e2), 1e-12)
+                t_hit = float(np.exp(np.log(t1) + f * (np.log(t2) - np.log(t1))))
+        return {"seconds_to_target": None if t_hit is None else round(t_hit, 4), "spp": spp,
+                "final_relmse": round(curve[-1][2], 5), "render_seconds": round(sec, 4),
+                "train_steps": steps}
+
+    plain = converge(False)
+    print("[bench] guiding plain: %s" % plain, file=sys.stderr, flush=True)
+    guided = converge(True)
+    print("[bench] guiding guided: %s" % guided, file=sys.stderr, flush=True)
+    c.close()
+    ratio = None
+    if plain["seconds_to_target"] and guided["seconds_to_target"]:
+        ratio = round(guided["seconds_to_target"] / plain["seconds_to_target"], 4)
+    return {"workload": "slit-lit room, 64 GGX spheres, %dx%d, depth 10" % (res, res),
+            "target_relmse": args.guiding_target, "reference_spp": args.guiding_ref_spp,
+            "reference": "plain GPU tracer (FP32 film, Kahan-compensated)",
+            "plain": plain, "guided": guided, "time_ratio_guided_over_plain": ratio,
+            "plain_throughput": plain_rate, "wall_seconds": round(time.time() - t0, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -33,11 +125,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--skip-primary", action="store_true",
+                    help="launch only the incoherent set (profiling runs: one kernel population)")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--refill", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy kernel of MIB MiB (reference kernel of "
                          "known traffic for HBM counter calibration)")
+    ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain equal-error leg")
+    ap.add_argument("--guiding-res", type=int, default=512)
+    ap.add_argument("--guiding-ref-spp", type=int, default=16384)
+    ap.add_argument("--guiding-target", type=float, default=0.01)
+    ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
+    ap.add_argument("--guiding-lr", type=float, default=1e-3)
+    ap.add_argument("--guiding-steps", type=int, default=4, help="Adam steps per pass")
+    ap.add_argument("--guiding-after", type=int, default=4, help="training passes before guided sampling")
+    ap.add_argument("--seed-guide", type=int, default=17)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -84,7 +187,8 @@ def main():
 
     for _ in range(args.warmup):
         step(d_inc, h_inc)
-        step(d_prim, h_prim)
+        if not args.skip_primary:
+            step(d_prim, h_prim)
     ctx.synchronize()
 
     def timed(r, h, k):
@@ -98,11 +202,13 @@ def main():
         return ms / k
 
     ms_inc = timed(d_inc, h_inc, args.steps)
-    ms_prim = timed(d_prim, h_prim, max(1, args.steps // 2))
     mrays_inc = inc.shape[0] / (ms_inc * 1e-3) / 1e6
-    mrays_prim = prim.shape[0] / (ms_prim * 1e-3) / 1e6
-    print("[bench] incoherent %.3f ms/launch %.1f Mrays/s; primary %.3f ms %.1f Mrays/s"
-          % (ms_inc, mrays_inc, ms_prim, mrays_prim), file=sys.stderr, flush=True)
+    mrays_prim = None
+    if not args.skip_primary:
+        ms_prim = timed(d_prim, h_prim, max(1, args.steps // 2))
+        mrays_prim = round(prim.shape[0] / (ms_prim * 1e-3) / 1e6, 2)
+    print("[bench] incoherent %.3f ms/launch %.1f Mrays/s; primary %s Mrays/s"
+          % (ms_inc, mrays_inc, mrays_prim), file=sys.stderr, flush=True)
 
     # algorithmic bytes per ray from the instrumented launch (untimed)
     _, nv, nt = ctx.intersect_closest(d_inc, out=h_inc, counted=True)
@@ -136,6 +242,10 @@ def main():
         mm = int(np.count_nonzero(np.any(got.view(np.uint32) != hits_cpu.view(np.uint32), axis=1)))
         parity = {"incoherent_mismatches": mm, "rays": int(n)}
 
+    guiding = None
+    if not args.no_guiding:
+        guiding = guiding_equal_error(args)
+
     line = {
         "metric": BASELINE_METRIC,
         "value": round(mrays_inc, 2),
@@ -154,9 +264,10 @@ def main():
                    "bvh": "BVH8, 128-byte nodes, leaf<=%d" % args.leaf, "tmax": "inf"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "primary_mrays_per_s": round(mrays_prim, 2),
+        "primary_mrays_per_s": mrays_prim,
         "parity": parity,
         "bvh_build_seconds": round(st["build_seconds"], 3),
+        "guiding": guiding,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -176,10 +176,13 @@ typedef struct {
     uint32_t width, height, spp, max_depth, rr_depth;
     uint32_t seed;
     uint32_t spp_offset;
-    uint32_t flags;      /* reserved, 0 */
+    uint32_t flags;      /* KP_RENDER_* bits */
     float eye[3], lookat[3], up[3];
     float tan_half_fov;
 } kp_render_desc;
+
+/* flags: sample with the path-guiding network (see kp_guiding_configure). */
+#define KP_RENDER_GUIDED 1u
 
 typedef struct {
     uint64_t paths;           /* camera paths traced by the last accumulate */
@@ -188,7 +191,10 @@ typedef struct {
     double seconds;           /* wall time of the last accumulate */
     uint32_t passes;
     uint32_t paths_per_pass;
-    uint32_t reserved[6];
+    uint32_t train_steps;     /* guiding: Adam steps taken by the last accumulate */
+    uint32_t guided_passes;   /* guiding: passes that sampled from the network */
+    uint32_t train_records;   /* guiding: valid training samples of the last pass */
+    uint32_t reserved[3];
 } kp_render_stats;
 
 int kp_material_add(kp_ctx* ctx, const kp_material* m, uint32_t* material_id);
@@ -235,6 +241,38 @@ int kp_guiding_sample(kp_ctx* ctx, const float* in6, const float* u3, uint32_t n
  * grad (optional) receives the gradient in the parameter order above. */
 int kp_guiding_train_step(kp_ctx* ctx, const float* in6, const float* dirs, const float* weights,
                           uint32_t n, float lr, int apply, float* grad);
+
+/* Guided rendering (desc->flags & KP_RENDER_GUIDED). At a guided vertex (any
+ * Lambert material; conductors with alpha >= min_alpha) the next direction is
+ * drawn from the BSDF with probability bsdf_fraction and from the network
+ * otherwise, and weighted by the one-sample MIS density
+ * bsdf_fraction * p_bsdf + (1 - bsdf_fraction) * q; next-event estimation and
+ * emission use the same density in their power heuristic, so the estimator
+ * stays unbiased for any network. Guided paths draw 12 PCG32 numbers per
+ * vertex: the 8 of the plain protocol, then strategy, lobe and two direction
+ * numbers.
+ * kp_render_accumulate then also trains the network online: each pass records
+ * up to max_records guided-vertex samples (position, normal, sampled
+ * direction, weight = luminance of the radiance gathered beyond the vertex
+ * divided by the path throughput before it) and takes train_steps Adam steps
+ * of the loss of kp_guiding_train_step on them, weights normalised to sum 1.
+ * Sampling from the network starts once guide_after passes have trained it.
+ * Guided renders run at most pass_paths paths per pass. The position is
+ * normalised with the box [bmin, bmax] (bmin == bmax: the scene's bounds).
+ * kp_integrate_rays with the flag samples from the network without training. */
+typedef struct {
+    float bsdf_fraction;   /* (0, 1]; 0.5 */
+    float min_alpha;       /* 0.1 */
+    float lr;              /* Adam learning rate; 1e-3 */
+    uint32_t train_steps;  /* per pass; 0 = frozen network */
+    uint32_t guide_after;  /* training passes before sampling from the network */
+    uint32_t max_records;  /* 1 .. 262144 */
+    uint32_t pass_paths;   /* >= 1; 262144 */
+    uint32_t reserved;
+    float bmin[3], bmax[3];
+} kp_guiding_desc;
+/* Needs kp_guiding_init first; a later kp_guiding_init keeps the settings. */
+int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);
 
 /* Device memory and timing helpers for callers without their own HIP
  * runtime handle (the reference has no counterpart: its callers never see

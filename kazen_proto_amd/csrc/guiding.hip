@@ -11,6 +11,7 @@
 #include "../../include/kp_hip.h"
 #include "kp_ctx.h"
 #include "kp_guide.h"
+#include "kp_guide_host.h"
 
 namespace kp {
 
@@ -72,6 +73,9 @@ __global__ __launch_bounds__(256) void k_guide_eval(GuideNet net, const float* _
     const bool valid = s < n;
     if (mode == 0) {
         g_write_features(feat + lane * KP_G_ROW, in6 + 6 * (size_t)(valid ? s : 0), valid);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         uint32_t* act[4] = {ping, pong, ping, pong};
         for (int q = 0; q < 4; ++q) {
             uint32_t* rows = feat + 16 * q * KP_G_ROW;
@@ -261,6 +265,37 @@ __global__ __launch_bounds__(256) void k_guide_train(GuideNet net, const float* 
     }
 }
 
+// One block: sums weights[0..n) in a fixed order (FP64), then writes the
+// batch count, the loss scale 1 / sum and the number of positive weights.
+__global__ __launch_bounds__(1024) void k_guide_normalise(const float* __restrict__ weights, uint32_t n,
+                                                          uint32_t* count, float* scale, uint32_t* valid) {
+    __shared__ double ssum[1024];
+    __shared__ uint32_t scnt[1024];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    uint32_t nv = 0;
+    for (uint32_t i = t; i < n; i += 1024) {
+        const float w = weights[i];
+        acc += (double)w;
+        nv += w > 0.0f ? 1u : 0u;
+    }
+    ssum[t] = acc;
+    scnt[t] = nv;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) {
+            ssum[t] += ssum[t + h];
+            scnt[t] += scnt[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        *count = n;
+        *scale = ssum[0] > 0.0 ? (float)(1.0 / ssum[0]) : 0.0f;
+        *valid = scnt[0];
+    }
+}
+
 // grad = sum of the block contributions (fixed order); optional Adam step.
 __global__ void k_guide_reduce_adam(const float* __restrict__ partial, const uint32_t* n_dev, float* grad,
                                     float* params, float* m, float* v, int step, float lr, float beta1,
@@ -283,26 +318,6 @@ __global__ void k_guide_reduce_adam(const float* __restrict__ partial, const uin
 }  // namespace kp
 
 // =================================================================== host ==
-struct GuideState {
-    float* params = nullptr;   // FP32 masters, canonical layout
-    float* m = nullptr;        // Adam moments
-    float* v = nullptr;
-    float* grad = nullptr;     // last reduced gradient
-    kp::u32x4* wf = nullptr;   // bf16 fragments
-    kp::u32x4* wb = nullptr;
-    float* partial = nullptr;  // per-block gradient contributions
-    uint32_t max_batch = 1u << 18;
-    // staging of a training batch
-    float* in6 = nullptr;
-    float* dirs = nullptr;
-    float* weights = nullptr;
-    uint32_t* count = nullptr;
-    float* scale = nullptr;
-    int step = 0;
-    float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    float3 bmin{0, 0, 0}, binv{1, 1, 1};
-};
-
 namespace kp {
 
 void guide_state_free(kp_ctx* c) {
@@ -310,7 +325,7 @@ void guide_state_free(kp_ctx* c) {
     if (!g) return;
     for (void* p : {(void*)g->params, (void*)g->m, (void*)g->v, (void*)g->grad, (void*)g->wf, (void*)g->wb,
                     (void*)g->partial, (void*)g->in6, (void*)g->dirs, (void*)g->weights, (void*)g->count,
-                    (void*)g->scale})
+                    (void*)g->scale, (void*)g->rec_aux, (void*)g->rec_valid})
         if (p) (void)hipFree(p);
     delete g;
     c->guide = nullptr;
@@ -351,6 +366,8 @@ static int guide_alloc(kp_ctx* c) {
     KP_HIP_TRY(c, hipMalloc((void**)&g->weights, (size_t)g->max_batch * sizeof(float)));
     KP_HIP_TRY(c, hipMalloc((void**)&g->count, 64));
     KP_HIP_TRY(c, hipMalloc((void**)&g->scale, 64));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->rec_aux, (size_t)g->max_batch * 2 * sizeof(float4)));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->rec_valid, 64));
     return KP_OK;
 }
 
@@ -361,6 +378,7 @@ static int guide_set_params(kp_ctx* c, const float* host) {
     KP_HIP_TRY(c, hipMemsetAsync(g->m, 0, P, c->stream));
     KP_HIP_TRY(c, hipMemsetAsync(g->v, 0, P, c->stream));
     g->step = 0;
+    g->train_passes = 0;
     int rc = guide_pack(c);
     if (rc) return rc;
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -383,6 +401,25 @@ int guide_train_device(kp_ctx* c, float lr, bool apply) {
     KP_HIP_TRY(c, hipGetLastError());
     if (apply) return guide_pack(c);
     return KP_OK;
+}
+
+int guide_normalise_records(kp_ctx* c, uint32_t n) {
+    GuideState* g = c->guide;
+    hipLaunchKernelGGL(k_guide_normalise, dim3(1), dim3(1024), 0, c->stream, g->weights, n, g->count, g->scale,
+                       g->rec_valid);
+    KP_HIP_TRY(c, hipGetLastError());
+    return KP_OK;
+}
+
+void guide_set_box(kp_ctx* c, const float* lo, const float* hi) {
+    GuideState* g = c->guide;
+    float inv[3];
+    for (int i = 0; i < 3; ++i) {
+        const float ext = hi[i] - lo[i];
+        inv[i] = ext > 0.0f ? 1.0f / ext : 1.0f;
+    }
+    g->bmin = make_float3(lo[0], lo[1], lo[2]);
+    g->binv = make_float3(inv[0], inv[1], inv[2]);
 }
 
 static int guide_ready(kp_ctx* c) {
@@ -509,6 +546,36 @@ int kp_guiding_train_step(kp_ctx* c, const float* in6, const float* dirs, const 
     if ((rc = kp::guide_train_device(c, lr, apply != 0))) return rc;
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (grad_out) KP_HIP_TRY(c, hipMemcpy(grad_out, g->grad, KP_G_PARAMS * 4, hipMemcpyDeviceToHost));
+    return KP_OK;
+}
+
+int kp_guiding_configure(kp_ctx* c, const kp_guiding_desc* d) {
+    if (!c || !d) return KP_ERR_INVALID;
+    int rc = kp::guide_ready(c);
+    if (rc) return rc;
+    if (!(d->bsdf_fraction > 0.0f && d->bsdf_fraction <= 1.0f))
+        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: bsdf_fraction must be in (0, 1]");
+    if (!(d->min_alpha >= 0.0f) || !(d->lr >= 0.0f))
+        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: min_alpha and lr must be >= 0");
+    if (d->max_records == 0 || d->max_records > c->guide->max_batch || d->pass_paths == 0)
+        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: max_records must be 1..262144, pass_paths >= 1");
+    for (int i = 0; i < 3; ++i)
+        if (!std::isfinite(d->bmin[i]) || !std::isfinite(d->bmax[i]) || d->bmax[i] < d->bmin[i])
+            return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: bad box");
+    GuideState* g = c->guide;
+    g->bsdf_fraction = d->bsdf_fraction;
+    g->min_alpha = d->min_alpha;
+    g->lr = d->lr;
+    g->train_steps = d->train_steps;
+    g->guide_after = d->guide_after;
+    g->max_records = d->max_records;
+    g->pass_paths = d->pass_paths;
+    g->auto_box = d->bmin[0] == d->bmax[0] && d->bmin[1] == d->bmax[1] && d->bmin[2] == d->bmax[2];
+    for (int i = 0; i < 3; ++i) {
+        g->box_min[i] = d->bmin[i];
+        g->box_max[i] = d->bmax[i];
+    }
+    if (!g->auto_box) kp::guide_set_box(c, d->bmin, d->bmax);
     return KP_OK;
 }
 

@@ -144,6 +144,10 @@ __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin,
                 for (int j = 0; j < 4; ++j) last8[4 * t + j] = v[j];
             }
         }
+        // the next layer reads rows written by other lanes of the wave
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         src = dst;
     }
 }
@@ -316,6 +320,9 @@ __device__ inline void g_wave_guide(const GuideNet& net, uint32_t* lds, const Gu
     uint32_t* ping = lds + 64 * KP_G_ROW;
     uint32_t* pong = ping + 16 * KP_G_ROW;
     g_encode_row(feat + lane * KP_G_ROW, q.px, q.py, q.pz, q.nx, q.ny, q.nz);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     uint32_t* act[4] = {ping, pong, ping, pong};
     ans.pdf_a = ans.pdf_b = 0.0f;
     ans.dx = ans.dy = 0.0f;

@@ -23,6 +23,7 @@
 // light pick, light triangle, two barycentric, roulette, lobe, two direction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -31,6 +32,8 @@
 #include "../../include/kp_hip.h"
 #include "kp_closures.h"
 #include "kp_ctx.h"
+#include "kp_guide.h"
+#include "kp_guide_host.h"
 
 namespace kp {
 
@@ -70,6 +73,14 @@ struct RenderArgs {
     float3 eye, fwd, right, upv;
     float tan_half, aspect;
     PathState ps;
+    // guided shading (k_shade<true>)
+    GuideNet net;
+    float bsdf_frac, min_alpha;
+    uint32_t guide_on;                      // sample from the network
+    uint32_t rec_cap, rec_stride, rec_phase;   // training records (rec_cap 0: none)
+    float* rec_in6;
+    float* rec_dirs;
+    float4* rec_aux;
 };
 
 // ------------------------------------------------------------------ RNG --
@@ -209,7 +220,13 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
+// Shades the current queue.  kGuided adds the guiding network: 12 draws per
+// vertex (8 plain + strategy, lobe, two direction), one-sample MIS between
+// the BSDF and the network at guided vertices, and training records.  The
+// network query runs once per wave after every lane has prepared its vertex.
+template <bool kGuided>
 __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t depth) {
+    __shared__ uint32_t glds[kGuided ? 4 * 96 * KP_G_ROW : 1];
     const uint32_t n = a.ps.cnt[cur];
     const uint32_t total = gridDim.x * 256;
     const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
@@ -219,35 +236,61 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
     float4* __restrict__ rout = a.ps.rays[cur ^ 1];
     uint32_t* __restrict__ qout = a.ps.qmap[cur ^ 1];
     const float inv_nl = a.n_lights ? 1.0f / (float)a.n_lights : 0.0f;
+    constexpr int kDraws = kGuided ? 12 : 8;
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t q = it * total + gid;
         const bool active = q < n;
         bool cont = false, shad = false;
         float4 nr0 = make_float4(0, 0, 0, 0), nr1 = nr0, sr0 = nr0, sr1 = nr0, sc = nr0;
         uint32_t slot = 0;
+        // ---- per-lane vertex set-up --------------------------------------
+        bool alive = false, guided = false, recorded = false;
+        uint32_t rec = 0;
+        f3 beta = mk(0, 0, 0), beta_pre = beta, L = beta, P = beta, ng = beta, nrm = beta, wo = beta;
+        uint32_t mat_id = 0;
+        Pcg rng;
+        rng.state = 0;
+        rng.inc = 1;
+        float u[kDraws];
+        bool nee = false;
+        f3 nee_org = beta, nee_dir = beta, nee_c = beta;
+        float nee_dist = 0.0f, nee_cos = 0.0f, pdf_l = 0.0f, pdf_bl = 0.0f;
+        bool bs_ok = false;
+        f3 bs_wi = beta, bs_f = beta;
+        float bs_pdf = 0.0f;
+        bool use_guide = false;
         if (active) {
             slot = qin[q];
             const float4 hit = a.ps.hits[q];
             const float4 r1 = rin[2 * (size_t)q + 1];
-            float4 st0 = a.ps.s0[slot];
-            float4 st1 = a.ps.s1[slot];
-            f3 beta = mk(st0.x, st0.y, st0.z);
+            const float4 st0 = a.ps.s0[slot];
+            const float4 st1 = a.ps.s1[slot];
+            beta = mk(st0.x, st0.y, st0.z);
             const float prev_pdf = st0.w;
-            f3 L = mk(st1.x, st1.y, st1.z);
+            L = mk(st1.x, st1.y, st1.z);
             const uint32_t prim = __float_as_uint(hit.y);
             const uint32_t pixel = a.seq_base + slot / a.k;
-            Pcg rng;
             rng.state = a.ps.rng[slot];
             rng.inc = ((uint64_t)pixel << 1u) | 1u;
+            if constexpr (kGuided) {
+                if (a.rec_cap && slot % a.rec_stride == a.rec_phase) {
+                    recorded = true;
+                    rec = (slot / a.rec_stride) * a.max_depth;
+                    // radiance gathered before the previous vertex's direction
+                    // (its emission and next-event estimate included)
+                    if (depth >= 2) a.rec_aux[2 * (size_t)(rec + depth - 2)] = make_float4(L.x, L.y, L.z, 1.0f);
+                }
+            }
             if (prim != 0xFFFFFFFFu) {
                 const float4 t0 = a.stris[3 * (size_t)prim], t1 = a.stris[3 * (size_t)prim + 1],
                              t2 = a.stris[3 * (size_t)prim + 2];
                 const f3 v0 = ld3(t0), v1 = ld3(t1), v2 = ld3(t2);
-                const Material& m = a.mats[__float_as_uint(t0.w)];
+                mat_id = __float_as_uint(t0.w);
+                const Material& m = a.mats[mat_id];
                 const int lid = __float_as_int(t1.w);
-                const f3 ng = normalize(cross(v1 - v0, v2 - v0));
+                ng = normalize(cross(v1 - v0, v2 - v0));
                 const float b0 = 1.0f - hit.z - hit.w;
-                const f3 P = v0 * b0 + v1 * hit.z + v2 * hit.w;
+                P = v0 * b0 + v1 * hit.z + v2 * hit.w;
                 const f3 dir = mk(r1.x, r1.y, r1.z);
                 const f3 Le = mk(m.Le[0], m.Le[1], m.Le[2]);
                 const float cos_hit = dot(dir, ng);
@@ -261,13 +304,17 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                     L = L + mul(beta, Le) * w;
                 }
                 if (depth <= a.max_depth) {
-                    float u[8];
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) u[i] = rng.nextf();
+                    for (int i = 0; i < kDraws; ++i) u[i] = rng.nextf();
                     const f3 wo_w = -dir;
-                    const f3 nrm = (m.type == 2) ? ng : (dot(wo_w, ng) < 0.0f ? -ng : ng);
+                    nrm = (m.type == 2) ? ng : (dot(wo_w, ng) < 0.0f ? -ng : ng);
                     const Frame fr(nrm);
-                    const f3 wo = fr.to_local(wo_w);
+                    wo = fr.to_local(wo_w);
+                    if constexpr (kGuided) {
+                        const bool eligible = m.type == 0 || (m.type == 1 && m.alpha >= a.min_alpha);
+                        guided = eligible && a.guide_on;
+                        recorded = recorded && eligible;
+                    }
                     // next-event estimation (GeometryLight::sample, light.cpp:52-77)
                     if (a.n_lights > 0) {
                         uint32_t li = (uint32_t)(u[0] * (float)a.n_lights);
@@ -292,19 +339,20 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                                 float pdf_b;
                                 bsdf_eval(m, wo, wi, f, pdf_b);
                                 if (!is_zero(f)) {
-                                    const float pdf_l = inv_nl * (1.0f / lr.area) * (dist * dist) / cos_l;
-                                    const float w = pdf_l * pdf_l / (pdf_l * pdf_l + pdf_b * pdf_b);
-                                    const f3 c = mul(mul(beta, f), mk(lr.Le[0], lr.Le[1], lr.Le[2])) *
-                                                 (fabsf(wi.y) * w / pdf_l);
-                                    shad = true;
-                                    sr0 = make_float4(org.x, org.y, org.z, 0.0f);
-                                    sr1 = make_float4(wi_w.x, wi_w.y, wi_w.z, dist * 0.9999f);
-                                    sc = make_float4(c.x, c.y, c.z, __uint_as_float(slot));
+                                    nee = true;
+                                    pdf_l = inv_nl * (1.0f / lr.area) * (dist * dist) / cos_l;
+                                    pdf_bl = pdf_b;
+                                    nee_c = mul(mul(beta, f), mk(lr.Le[0], lr.Le[1], lr.Le[2]));
+                                    nee_org = org;
+                                    nee_dir = wi_w;
+                                    nee_dist = dist;
+                                    nee_cos = fabsf(wi.y);
                                 }
                             }
                         }
                     }
-                    bool alive = true;
+                    beta_pre = beta;
+                    alive = true;
                     // Russian roulette (integrator.cpp:491-497)
                     if (depth >= a.rr_depth) {
                         const float eta_rr = 0.95f;
@@ -313,23 +361,96 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                         else beta = beta * (1.0f / qs);
                     }
                     if (alive) {
-                        f3 wi, f;
-                        float pdf;
-                        if (bsdf_sample(m, wo, u[5], u[6], u[7], wi, f, pdf)) {
-                            beta = mul(beta, f) * (fabsf(wi.y) / pdf);
-                            if (!is_zero(beta)) {
-                                const f3 wi_w = fr.to_world(wi);
-                                const f3 o = offset_ray(P, dot(wi_w, ng) > 0.0f ? ng : -ng);
-                                nr0 = make_float4(o.x, o.y, o.z, 0.0f);
-                                nr1 = make_float4(wi_w.x, wi_w.y, wi_w.z, __int_as_float(0x7f800000));
-                                st0 = make_float4(beta.x, beta.y, beta.z, pdf);
-                                cont = true;
+                        if constexpr (kGuided) use_guide = guided && !(u[8] < a.bsdf_frac);
+                        if (!use_guide) bs_ok = bsdf_sample(m, wo, u[5], u[6], u[7], bs_wi, bs_f, bs_pdf);
+                    }
+                }
+            }
+        }
+        // ---- guiding network: one query per wave ---------------------------
+        GuideAnswer ga;
+        ga.pdf_a = ga.pdf_b = 0.0f;
+        ga.dx = ga.dy = 0.0f;
+        ga.dz = 1.0f;
+        if constexpr (kGuided) {
+            const bool want = guided && (nee || use_guide || bs_ok);
+            if (__ballot(want)) {
+                GuideQuery gq;
+                const f3 p01 = mk((P.x - a.net.bmin.x) * a.net.binv.x, (P.y - a.net.bmin.y) * a.net.binv.y,
+                                  (P.z - a.net.bmin.z) * a.net.binv.z);
+                gq.px = want ? p01.x : 0.0f;
+                gq.py = want ? p01.y : 0.0f;
+                gq.pz = want ? p01.z : 0.0f;
+                gq.nx = want ? nrm.x : 0.0f;
+                gq.ny = want ? nrm.y : 1.0f;
+                gq.nz = want ? nrm.z : 0.0f;
+                gq.ax = nee_dir.x;
+                gq.ay = nee_dir.y;
+                gq.az = nee_dir.z;
+                f3 bw = mk(0, 0, 0);
+                if (bs_ok) bw = Frame(nrm).to_world(bs_wi);
+                gq.bx = bw.x;
+                gq.by = bw.y;
+                gq.bz = bw.z;
+                gq.sample = want && use_guide;
+                gq.u0 = gq.sample ? u[9] : 0.5f;
+                gq.u1 = gq.sample ? u[10] : 0.5f;
+                gq.u2 = gq.sample ? u[11] : 0.5f;
+                g_wave_guide(a.net, glds + (threadIdx.x >> 6) * 96 * KP_G_ROW, gq, ga);
+            }
+        }
+        // ---- per-lane finish ---------------------------------------------
+        if (active) {
+            if (nee) {
+                float pdf_b = pdf_bl;
+                if (guided) pdf_b = a.bsdf_frac * pdf_bl + (1.0f - a.bsdf_frac) * ga.pdf_a;
+                const float w = pdf_l * pdf_l / (pdf_l * pdf_l + pdf_b * pdf_b);
+                const f3 c = nee_c * (nee_cos * w / pdf_l);
+                shad = true;
+                sr0 = make_float4(nee_org.x, nee_org.y, nee_org.z, 0.0f);
+                sr1 = make_float4(nee_dir.x, nee_dir.y, nee_dir.z, nee_dist * 0.9999f);
+                sc = make_float4(c.x, c.y, c.z, __uint_as_float(slot));
+            }
+            if (alive) {
+                const Frame fr(nrm);
+                f3 wi = bs_wi, f = bs_f;
+                float pdf = bs_pdf;
+                bool ok = bs_ok;
+                if (use_guide) {
+                    wi = fr.to_local(mk(ga.dx, ga.dy, ga.dz));
+                    float pdf_bsdf;
+                    bsdf_eval(a.mats[mat_id], wo, wi, f, pdf_bsdf);
+                    pdf = a.bsdf_frac * pdf_bsdf + (1.0f - a.bsdf_frac) * ga.pdf_b;
+                    ok = !is_zero(f) && pdf > 0.0f;
+                } else if (guided && ok) {
+                    pdf = a.bsdf_frac * bs_pdf + (1.0f - a.bsdf_frac) * ga.pdf_b;
+                }
+                if (ok) {
+                    beta = mul(beta, f) * (fabsf(wi.y) / pdf);
+                    if (!is_zero(beta)) {
+                        const f3 wi_w = fr.to_world(wi);
+                        const f3 o = offset_ray(P, dot(wi_w, ng) > 0.0f ? ng : -ng);
+                        nr0 = make_float4(o.x, o.y, o.z, 0.0f);
+                        nr1 = make_float4(wi_w.x, wi_w.y, wi_w.z, __int_as_float(0x7f800000));
+                        a.ps.s0[slot] = make_float4(beta.x, beta.y, beta.z, pdf);
+                        cont = true;
+                        if constexpr (kGuided) {
+                            if (recorded) {
+                                const uint32_t r = rec + depth - 1;
+                                const f3 p01 = mk((P.x - a.net.bmin.x) * a.net.binv.x,
+                                                  (P.y - a.net.bmin.y) * a.net.binv.y,
+                                                  (P.z - a.net.bmin.z) * a.net.binv.z);
+                                float* in6 = a.rec_in6 + 6 * (size_t)r;
+                                in6[0] = p01.x; in6[1] = p01.y; in6[2] = p01.z;
+                                in6[3] = nrm.x; in6[4] = nrm.y; in6[5] = nrm.z;
+                                float* dr = a.rec_dirs + 3 * (size_t)r;
+                                dr[0] = wi_w.x; dr[1] = wi_w.y; dr[2] = wi_w.z;
+                                a.rec_aux[2 * (size_t)r + 1] = make_float4(beta_pre.x, beta_pre.y, beta_pre.z, 1.0f);
                             }
                         }
                     }
                 }
             }
-            if (cont) a.ps.s0[slot] = st0;
             a.ps.s1[slot] = make_float4(L.x, L.y, L.z, 0.0f);
             a.ps.rng[slot] = rng.state;
         }
@@ -361,6 +482,36 @@ __global__ __launch_bounds__(256) void k_connect(RenderArgs a) {
         l.z += c.z;
         a.ps.s1[slot] = l;
     }
+}
+
+// Turns a pass's records into training samples.  Record i belongs to slot
+// (i / max_depth) * rec_stride + rec_phase and vertex i % max_depth + 1; its
+// weight is the luminance of (radiance at the end of the path - radiance
+// before the vertex's sampled direction) / throughput before the vertex, an
+// estimate of f cos L_i / p for the sampled direction.  Records that were not
+// completed get weight 0 and finite inputs.
+__global__ __launch_bounds__(256) void k_guide_records(RenderArgs a, uint32_t n_rec, float* weights) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_rec) return;
+    const uint32_t slot = (i / a.max_depth) * a.rec_stride + a.rec_phase;
+    const float4 x0 = a.rec_aux[2 * (size_t)i], x1 = a.rec_aux[2 * (size_t)i + 1];
+    float w = 0.0f;
+    const bool made = x0.w == 1.0f && x1.w == 1.0f && slot < a.n_paths;
+    if (made) {
+        const float4 le = a.ps.s1[slot];
+        if (x1.x > 0.0f) w += 0.2126f * (le.x - x0.x) / x1.x;
+        if (x1.y > 0.0f) w += 0.7152f * (le.y - x0.y) / x1.y;
+        if (x1.z > 0.0f) w += 0.0722f * (le.z - x0.z) / x1.z;
+        if (!(w > 0.0f) || !isfinite(w)) w = 0.0f;
+    } else {
+        float* in6 = a.rec_in6 + 6 * (size_t)i;
+        in6[0] = in6[1] = in6[2] = in6[3] = in6[5] = 0.0f;
+        in6[4] = 1.0f;
+        float* dr = a.rec_dirs + 3 * (size_t)i;
+        dr[0] = dr[2] = 0.0f;
+        dr[1] = 1.0f;
+    }
+    weights[i] = w;
 }
 
 // Folds the k path radiances of every pixel into the film, in sample order,
@@ -409,6 +560,7 @@ struct RenderState {
     uint32_t* d_light_tris = nullptr;
     float* d_light_cdf = nullptr;
     uint32_t n_lights = 0;
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
     // paths
     uint32_t cap_paths = 0;
     void* path_mem = nullptr;
@@ -522,6 +674,15 @@ static int prepare_scene(kp_ctx* c) {
             st[3 * (size_t)p + 2] = make_float4(v[6], v[7], v[8], 0.0f);
         }
     }
+    for (int a = 0; a < 3; ++a) {
+        r->scene_lo[a] = INFINITY;
+        r->scene_hi[a] = -INFINITY;
+    }
+    for (size_t i = 0; i < 3 * (size_t)n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            r->scene_lo[a] = std::fmin(r->scene_lo[a], c->soup[3 * i + a]);
+            r->scene_hi[a] = std::fmax(r->scene_hi[a], c->soup[3 * i + a]);
+        }
     KP_HIP_TRY(c, hipMalloc((void**)&r->stris, st.size() * sizeof(float4)));
     KP_HIP_TRY(c, hipMemcpy(r->stris, st.data(), st.size() * sizeof(float4), hipMemcpyHostToDevice));
     KP_HIP_TRY(c, hipMalloc((void**)&r->d_mats, r->mats.size() * sizeof(Material)));
@@ -624,8 +785,7 @@ static void camera_basis(const kp_render_desc& d, RenderArgs& a) {
     a.aspect = (float)d.width / (float)d.height;
 }
 
-static uint32_t samples_per_pass(uint32_t npix, uint32_t spp) {
-    const uint64_t target = 1u << 21;   // paths in flight per pass
+static uint32_t samples_per_pass(uint32_t npix, uint32_t spp, uint64_t target) {
     uint32_t best = 1;
     for (uint32_t k = 1; k <= spp; ++k)
         if (spp % k == 0 && (uint64_t)npix * k <= target) best = k;
@@ -634,7 +794,7 @@ static uint32_t samples_per_pass(uint32_t npix, uint32_t spp) {
 
 // Extend / shade / connect for max_depth + 1 bounces of the paths that the
 // generate kernel has queued (queue sizes live on the device).
-static int run_bounces(kp_ctx* c, const RenderArgs& a) {
+static int run_bounces(kp_ctx* c, const RenderArgs& a, bool guided) {
     const uint32_t shade_blocks = (uint32_t)c->cus * 8u;
     int cur = 0;
     for (uint32_t depth = 1; depth <= a.max_depth + 1; ++depth) {
@@ -655,7 +815,10 @@ static int run_bounces(kp_ctx* c, const RenderArgs& a) {
         ta.n_dev = &a.ps.cnt[cur];
         KP_HIP_TRY(c, launch_closest(ta, c->grid_closest, false, c->stream));
         hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, c->stream, a, cur);
-        hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
+        if (guided)
+            hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
+        else
+            hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
         KP_HIP_TRY(c, hipGetLastError());
         if (depth <= a.max_depth) {
             ta.rays = a.ps.srays;
@@ -685,10 +848,32 @@ static RenderArgs base_render_args(kp_ctx* c) {
     return a;
 }
 
+// Guided-shading fields of the render arguments: network, position box,
+// strategy fractions; sampling on, no records.
+static int guide_args(kp_ctx* c, RenderArgs& a) {
+    GuideState* g = c->guide;
+    if (!g) return fail(c, KP_ERR_STATE, "guided render before kp_guiding_init");
+    RenderState* r = rstate(c);
+    if (g->auto_box) guide_set_box(c, r->scene_lo, r->scene_hi);
+    a.net = guide_net(c);
+    a.bsdf_frac = g->bsdf_fraction;
+    a.min_alpha = g->min_alpha;
+    a.guide_on = 1;
+    a.rec_cap = 0;
+    a.rec_stride = 1;
+    a.rec_phase = 0;
+    a.rec_in6 = g->in6;
+    a.rec_dirs = g->dirs;
+    a.rec_aux = g->rec_aux;
+    return KP_OK;
+}
+
 // Traces one path per caller ray; writes the path radiance (RGB) per ray.
 static int integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_render_desc* d,
                           float* out, int kind) {
     if (d->max_depth == 0 || d->max_depth > 64) return fail(c, KP_ERR_INVALID, "max_depth must be 1..64");
+    const bool guided = (d->flags & KP_RENDER_GUIDED) != 0;
+    if (guided && !c->guide) return fail(c, KP_ERR_STATE, "guided render before kp_guiding_init");
     int rc = prepare_scene(c);
     if (rc) return rc;
     RenderState* r = rstate(c);
@@ -716,6 +901,7 @@ static int integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_re
         a.seed = d->seed;
         a.max_depth = d->max_depth;
         a.rr_depth = d->rr_depth;
+        if (guided && (rc = guide_args(c, a))) return rc;
         const float4* src = (const float4*)(rays) + 2 * (size_t)off;
         float* dst = out + 3 * (size_t)off;
         if (kind == KP_PTR_HOST) {
@@ -725,7 +911,7 @@ static int integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_re
         }
         hipLaunchKernelGGL(k_generate_rays, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a, src);
         KP_HIP_TRY(c, hipGetLastError());
-        if ((rc = run_bounces(c, a))) return rc;
+        if ((rc = run_bounces(c, a, guided))) return rc;
         hipLaunchKernelGGL(k_write_radiance, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a, dst);
         KP_HIP_TRY(c, hipGetLastError());
         if (kind == KP_PTR_HOST)
@@ -742,6 +928,9 @@ static int integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_re
     r->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     r->stats.passes = (n + cap - 1) / cap;
     r->stats.paths_per_pass = cap;
+    r->stats.train_steps = 0;
+    r->stats.guided_passes = guided ? r->stats.passes : 0;
+    r->stats.train_records = 0;
     return KP_OK;
 }
 
@@ -750,10 +939,15 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     if (d->width != r->fw || d->height != r->fh)
         return fail(c, KP_ERR_STATE, "film size differs from the render size (kp_film_reset first)");
     if (d->max_depth == 0 || d->max_depth > 64) return fail(c, KP_ERR_INVALID, "max_depth must be 1..64");
+    const bool guided = (d->flags & KP_RENDER_GUIDED) != 0;
+    GuideState* g = c->guide;
+    if (guided && !g) return fail(c, KP_ERR_STATE, "guided render before kp_guiding_init");
+    if (guided && g->train_steps && d->max_depth > g->max_records)
+        return fail(c, KP_ERR_INVALID, "guiding max_records must be >= max_depth");
     int rc = prepare_scene(c);
     if (rc) return rc;
     const uint32_t npix = d->width * d->height;
-    const uint32_t k = samples_per_pass(npix, d->spp);
+    const uint32_t k = samples_per_pass(npix, d->spp, guided ? g->pass_paths : (1u << 21));
     const uint32_t n_paths = npix * k;
     if ((rc = ensure_paths(c, n_paths))) return rc;
     RenderArgs a{};
@@ -773,18 +967,54 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     a.rr_depth = d->rr_depth;
     camera_basis(*d, a);
     a.ps = r->ps;
+    if (guided && (rc = guide_args(c, a))) return rc;
+    // training records: every rec_stride-th path (phase rotating per pass),
+    // max_depth records each, at most max_records per pass
+    uint32_t stride = 1, rec_slots_max = 0;
+    const bool train = guided && g->train_steps > 0;
+    if (train) {
+        const uint64_t D = d->max_depth;
+        stride = (uint32_t)std::max<uint64_t>(1, ((uint64_t)n_paths * D + g->max_records - 1) / g->max_records);
+        while (((uint64_t)n_paths + stride - 1) / stride * D > g->max_records) ++stride;
+        rec_slots_max = (uint32_t)(((uint64_t)n_paths + stride - 1) / stride);
+    }
     const uint32_t gen_blocks = (n_paths + 255) / 256;
+    uint32_t steps = 0, guided_passes = 0;
     auto t0 = std::chrono::steady_clock::now();
     unsigned long long zero2[2] = {0, 0};
     KP_HIP_TRY(c, hipMemcpyAsync(r->ps.totals, zero2, 16, hipMemcpyHostToDevice, c->stream));
     for (uint32_t pass = 0; pass < d->spp / k; ++pass) {
         a.s0 = d->spp_offset + pass * k;
+        uint32_t n_rec = 0;
+        if (guided) {
+            a.guide_on = g->train_passes >= g->guide_after ? 1u : 0u;
+            guided_passes += a.guide_on;
+            if (train) {
+                a.rec_stride = stride;
+                a.rec_phase = g->train_passes % stride;
+                const uint32_t slots = (n_paths - a.rec_phase + stride - 1) / stride;
+                n_rec = std::min(slots, rec_slots_max) * d->max_depth;
+                a.rec_cap = n_rec;
+                hipLaunchKernelGGL(k_zero_f4, dim3((2 * n_rec + 255) / 256), dim3(256), 0, c->stream, a.rec_aux,
+                                   2 * n_rec);
+            }
+        }
         hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, a);
         KP_HIP_TRY(c, hipGetLastError());
-        if ((rc = run_bounces(c, a))) return rc;
+        if ((rc = run_bounces(c, a, guided))) return rc;
         hipLaunchKernelGGL(k_film, dim3((npix + 255) / 256), dim3(256), 0, c->stream, a, r->film_sum,
                            r->film_comp);
         KP_HIP_TRY(c, hipGetLastError());
+        if (train) {
+            hipLaunchKernelGGL(k_guide_records, dim3((n_rec + 255) / 256), dim3(256), 0, c->stream, a, n_rec,
+                               g->weights);
+            KP_HIP_TRY(c, hipGetLastError());
+            if ((rc = guide_normalise_records(c, n_rec))) return rc;
+            for (uint32_t sidx = 0; sidx < g->train_steps; ++sidx)
+                if ((rc = guide_train_device(c, g->lr, true))) return rc;
+            steps += g->train_steps;
+            ++g->train_passes;
+        }
     }
     r->film_count += d->spp;
     if ((rc = kp_synchronize(c))) return rc;
@@ -797,6 +1027,10 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     r->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     r->stats.passes = d->spp / k;
     r->stats.paths_per_pass = n_paths;
+    r->stats.train_steps = steps;
+    r->stats.guided_passes = guided_passes;
+    r->stats.train_records = 0;
+    if (train) KP_HIP_TRY(c, hipMemcpy(&r->stats.train_records, g->rec_valid, 4, hipMemcpyDeviceToHost));
     return KP_OK;
 }
 

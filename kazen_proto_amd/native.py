@@ -61,7 +61,18 @@ class RenderDesc(ctypes.Structure):
 class RenderStats(ctypes.Structure):
     _fields_ = [("paths", _u64), ("extension_rays", _u64), ("shadow_rays", _u64),
                 ("seconds", ctypes.c_double), ("passes", _u32), ("paths_per_pass", _u32),
-                ("reserved", _u32 * 6)]
+                ("train_steps", _u32), ("guided_passes", _u32), ("train_records", _u32),
+                ("reserved", _u32 * 3)]
+
+
+RENDER_GUIDED = 1
+
+
+class GuidingDesc(ctypes.Structure):
+    _fields_ = [("bsdf_fraction", ctypes.c_float), ("min_alpha", ctypes.c_float),
+                ("lr", ctypes.c_float), ("train_steps", _u32), ("guide_after", _u32),
+                ("max_records", _u32), ("pass_paths", _u32), ("reserved", _u32),
+                ("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3)]
 
 
 def make_material(d):
@@ -77,7 +88,7 @@ def make_material(d):
     return m
 
 
-def make_render_desc(scene, spp=None, spp_offset=0, width=None, height=None):
+def make_render_desc(scene, spp=None, spp_offset=0, width=None, height=None, guided=False):
     d = RenderDesc()
     d.width = width or scene.width
     d.height = height or scene.height
@@ -86,7 +97,7 @@ def make_render_desc(scene, spp=None, spp_offset=0, width=None, height=None):
     d.rr_depth = scene.rr_depth
     d.seed = scene.seed
     d.spp_offset = spp_offset
-    d.flags = 0
+    d.flags = RENDER_GUIDED if guided else 0
     d.eye[:] = scene.eye
     d.lookat[:] = scene.lookat
     d.up[:] = scene.up
@@ -130,6 +141,7 @@ SIGNATURES = {
     "kp_guiding_pdf": (_i32, [_p, _p, _p, _u32, _p]),
     "kp_guiding_sample": (_i32, [_p, _p, _p, _u32, _p, _p]),
     "kp_guiding_train_step": (_i32, [_p, _p, _p, _p, _u32, ctypes.c_float, _i32, _p]),
+    "kp_guiding_configure": (_i32, [_p, _p]),
     "kp_device_alloc": (_i32, [_p, _u64, ctypes.POINTER(_p)]),
     "kp_device_free": (_i32, [_p, _p]),
     "kp_copy_to_device": (_i32, [_p, _p, _p, _u64]),
@@ -417,6 +429,22 @@ class Context:
         self._check(lib().kp_guiding_train_step(self._h, _np_ptr(in6), _np_ptr(dirs), _np_ptr(weights),
                                                 in6.shape[0], lr, int(apply), _np_ptr(g)))
         return g
+
+    def guiding_configure(self, bsdf_fraction=0.5, min_alpha=0.1, lr=1e-3, train_steps=1, guide_after=1,
+                          max_records=1 << 18, pass_paths=1 << 18, box=None):
+        """Guided-render settings (kp_guiding_configure); box = (lo, hi) or None for the scene's bounds."""
+        d = GuidingDesc()
+        d.bsdf_fraction = bsdf_fraction
+        d.min_alpha = min_alpha
+        d.lr = lr
+        d.train_steps = train_steps
+        d.guide_after = guide_after
+        d.max_records = max_records
+        d.pass_paths = pass_paths
+        if box is not None:
+            d.bmin[:] = [float(x) for x in box[0]]
+            d.bmax[:] = [float(x) for x in box[1]]
+        self._check(lib().kp_guiding_configure(self._h, ctypes.byref(d)))
 
     def device_copy(self, dst, src):
         """Kernel copy between two DeviceBuffers of the same size."""

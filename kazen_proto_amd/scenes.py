@@ -330,3 +330,58 @@ def small_test_scene(seed=3, n_random=400):
     v = c[:, None, :] + rng.normal(scale=0.08, size=(n_random, 3, 3))
     parts.append(v.reshape(-1, 9).astype(np.float32))
     return np.concatenate(parts, 0).astype(np.float32)
+
+
+def guiding_scene(width=512, height=512, spp=16, max_depth=10, rr_depth=3, seed=5, n_spheres=64,
+                  sphere_lon=16, sphere_lat=8):
+    """Glossy room lit through a slit (the neural path-guiding workload).
+
+    Box room [-1,1]^3 (12 triangles: Lambert 0.7 walls and ceiling, Lambert
+    0.6 floor).  An occluding wall at x = 0.5 (4 triangles, Lambert 0.7) has a
+    0.1-wide horizontal slit at y in [0.2, 0.3] across the whole room; behind
+    it a 0.2 x 0.2 area light (radiance 500) at x = 0.95, y in [0.75, 0.95],
+    faces the wall, so the only light reaching the camera's side comes through
+    the slit (a stripe on the floor near x in [-0.8, -0.2], then indirect).
+    64 GGX conductor spheres (radius U[0.05, 0.1], alpha U[0.02, 0.3], F0
+    U[0.6, 0.95] per channel) stand on the floor of the camera's side,
+    non-overlapping.  Camera in the corner (-0.95, 0.5, 0.95) looking at
+    (0.3, -0.6, -0.5), 65 degree vertical field of view."""
+    rng = np.random.default_rng(seed)
+    s = Scene(name="guiding", width=width, height=height, spp=spp, max_depth=max_depth,
+              rr_depth=rr_depth, seed=seed, eye=(-0.95, 0.5, 0.95), lookat=(0.3, -0.6, -0.5),
+              up=(0.0, 1.0, 0.0), fov_deg=65.0)
+    wall = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0.7, 0.7, 0.7)))
+    floor = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0.6, 0.6, 0.6)))
+    light = len(s.materials); s.materials.append(material(LAMBERT, albedo=(0, 0, 0), Le=(500, 500, 500)))
+    s.add_mesh(quad((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1)), floor)
+    s.add_mesh(np.concatenate([
+        quad((-1, 1, 1), (-1, 1, -1), (1, 1, -1), (1, 1, 1)),        # ceiling
+        quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1)),    # back (z = -1)
+        quad((-1, -1, 1), (-1, 1, 1), (1, 1, 1), (1, -1, 1)),        # front (z = +1)
+        quad((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1)),    # left (x = -1)
+        quad((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1)),        # right (x = +1)
+    ]), wall)
+    xw = 0.5
+    s.add_mesh(np.concatenate([
+        quad((xw, -1, -1), (xw, -1, 1), (xw, 0.2, 1), (xw, 0.2, -1)),
+        quad((xw, 0.3, -1), (xw, 0.3, 1), (xw, 1, 1), (xw, 1, -1)),
+    ]), wall)
+    xl = 0.95
+    s.add_mesh(quad((xl, 0.75, -0.1), (xl, 0.75, 0.1), (xl, 0.95, 0.1), (xl, 0.95, -0.1)), light)  # faces -x
+    placed = []
+    tries = 0
+    while len(placed) < n_spheres:
+        tries += 1
+        if tries > 100000:
+            raise RuntimeError("could not place the spheres")
+        r = rng.uniform(0.05, 0.1)
+        c = np.array([rng.uniform(-0.9 + r, 0.45 - r), -1.0 + r, rng.uniform(-0.9 + r, 0.9 - r)])
+        if any(np.linalg.norm(c - pc) < r + pr + 0.01 for pc, pr in placed):
+            continue
+        placed.append((c, r))
+        alpha = rng.uniform(0.02, 0.3)
+        F0 = rng.uniform(0.6, 0.95, 3)
+        m = len(s.materials)
+        s.materials.append(material(CONDUCTOR, GGX, F0=F0, alpha=alpha))
+        s.add_mesh(uv_sphere(sphere_lon, sphere_lat, center=tuple(c), radius=r), m)
+    return s

@@ -34,6 +34,29 @@ class RenderDesc(ctypes.Structure):
                 ("up", ctypes.c_float * 3), ("tan_half_fov", ctypes.c_float)]
 
 
+class GuideCfg(ctypes.Structure):
+    _fields_ = [("params", ctypes.c_void_p), ("bmin", ctypes.c_float * 3), ("binv", ctypes.c_float * 3),
+                ("bsdf_fraction", ctypes.c_float), ("min_alpha", ctypes.c_float), ("sample", _i32)]
+
+
+def make_guide(params, lo, hi, bsdf_fraction=0.5, min_alpha=0.1, sample=True):
+    """Guide settings for the guided oracle paths; the box inverse is formed
+    in FP32 as the HIP library forms it (1 / (hi - lo), 1 where the extent is 0)."""
+    params = np.ascontiguousarray(params, np.float32)
+    lo = np.asarray(lo, np.float32)
+    ext = np.asarray(hi, np.float32) - lo
+    inv = np.where(ext > 0, np.float32(1.0) / np.where(ext > 0, ext, np.float32(1.0)), np.float32(1.0))
+    g = GuideCfg()
+    g.params = params.ctypes.data
+    g.bmin[:] = [float(x) for x in lo]
+    g.binv[:] = [float(x) for x in inv.astype(np.float32)]
+    g.bsdf_fraction = bsdf_fraction
+    g.min_alpha = min_alpha
+    g.sample = 1 if sample else 0
+    g._keep = params   # the parameters must outlive the calls
+    return g
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -76,6 +99,8 @@ def lib():
             "or_scene_free": (None, [_p]),
             "or_render": (_i32, [_p, _p, _p, _p, _i32]),
             "or_integrate_rays": (_i32, [_p, _p, _u32, _p, _p, _i32]),
+            "or_render_guided": (_i32, [_p, _p, _p, _p, _p, _i32]),
+            "or_integrate_rays_guided": (_i32, [_p, _p, _u32, _p, _p, _p, _i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -343,20 +368,22 @@ class Scene:
             lib().or_scene_free(self._h)
             self._h = None
 
-    def render(self, desc, threads=0, want_sample0=False):
+    def render(self, desc, threads=0, want_sample0=False, guide=None):
         out = np.zeros((desc.height, desc.width, 3), np.float64)
         s0 = np.zeros((desc.height, desc.width, 3), np.float32) if want_sample0 else None
-        rc = lib().or_render(self._h, ctypes.byref(desc), _ptr(out),
-                             _ptr(s0) if s0 is not None else None, threads)
+        rc = lib().or_render_guided(self._h, ctypes.byref(desc),
+                                    ctypes.byref(guide) if guide is not None else None, _ptr(out),
+                                    _ptr(s0) if s0 is not None else None, threads)
         if rc != 0:
             raise RuntimeError("or_render failed (%d)" % rc)
         return (out, s0) if want_sample0 else out
 
-    def integrate_rays(self, rays, desc, threads=0):
+    def integrate_rays(self, rays, desc, threads=0, guide=None):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         out = np.zeros((rays.shape[0], 3), np.float32)
-        rc = lib().or_integrate_rays(self._h, _ptr(rays), rays.shape[0], ctypes.byref(desc),
-                                     _ptr(out), threads)
+        rc = lib().or_integrate_rays_guided(self._h, _ptr(rays), rays.shape[0], ctypes.byref(desc),
+                                            ctypes.byref(guide) if guide is not None else None,
+                                            _ptr(out), threads)
         if rc != 0:
             raise RuntimeError("or_integrate_rays failed (%d)" % rc)
         return out

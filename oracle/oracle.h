@@ -148,6 +148,23 @@ int or_render(const or_scene* s, const or_render_desc* d, double* out,
 int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n,
                       const or_render_desc* d, float* out, int threads);
 
+/* Guided variants (path-guiding network; no reference counterpart). Every
+ * vertex draws 12 numbers; with sample != 0, guided vertices (Lambert, or
+ * conductor with alpha >= min_alpha) sample the network with probability
+ * 1 - bsdf_fraction and weight by the one-sample MIS density. The network
+ * position input is (P - bmin) * binv in FP32. */
+typedef struct {
+    const float* params;
+    float bmin[3], binv[3];
+    float bsdf_fraction, min_alpha;
+    int32_t sample;
+} or_guide_cfg;
+int or_render_guided(const or_scene* s, const or_render_desc* d, const or_guide_cfg* g,
+                     double* out, float* sample0, int threads);
+int or_integrate_rays_guided(const or_scene* s, const float* rays, uint32_t n,
+                             const or_render_desc* d, const or_guide_cfg* g, float* out,
+                             int threads);
+
 /* ---- Path-guiding network (oracle_guide.cpp) ---------------------------- */
 /* in6 per sample: position normalised to [0,1]^3, unit normal. Canonical
  * parameter layout: W0[48x64] b0 W1[64x64] b1 W2 b2 W3 b3 W4[64x32] b4. With

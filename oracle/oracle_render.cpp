@@ -523,20 +523,37 @@ Hit trace(const or_scene& s, V3 o, V3 d, float tmax) {
     return r;
 }
 
-V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir);
+// Guided sampling set-up (see trace_path).
+struct Guide {
+    const float* params;
+    V3 bmin, binv;
+    float frac, min_alpha;
+    bool sample;   // sample from the network (else BSDF only, 12 draws per vertex)
+};
+
+V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir,
+              const Guide* g = nullptr);
 
 V3 path_radiance(const or_scene& s, const or_render_desc& d, const Camera& cam, uint32_t px,
-                 uint32_t py, uint32_t sample) {
+                 uint32_t py, uint32_t sample, const Guide* g = nullptr) {
     Pcg32 rng;
     rng.seed(path_initstate(d.seed, sample), (uint64_t)py * d.width + px);
     float ux = rng.nextf(), uy = rng.nextf();
     V3 o = cam.eye;
     V3 dir = camera_dir(cam, (float)px + ux, (float)py + uy);
-    return trace_path(s, d, rng, o, dir);
+    return trace_path(s, d, rng, o, dir, g);
 }
 
 // PathIntegrator::Li restated: radiance along the ray (o, dir).
-V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir) {
+//
+// With a guide (no reference counterpart) every vertex draws 12 numbers: the
+// 8 of the plain protocol, then u8 (strategy), u9..u11 (network lobe and
+// direction).  At a guided vertex (Lambert, or conductor with alpha >=
+// min_alpha) the direction comes from the BSDF (u5..u7) if u8 < frac, else
+// from the network's mixture (u9..u11); either way the sample is weighted by
+// the one-sample MIS density frac * p_bsdf + (1 - frac) * q, and that density
+// also stands in for p_bsdf in the next-event and emission power heuristics.
+V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir, const Guide* g) {
     V3 beta = v3(1, 1, 1), L = v3(0, 0, 0);
     float prev_pdf = 0.0f;
     const uint32_t nl = (uint32_t)s.lights.size();
@@ -564,12 +581,24 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
             L = L + mul(beta, m.Le) * w;
         }
         if (depth > d.max_depth) break;
-        float u[8];
-        for (int i = 0; i < 8; ++i) u[i] = rng.nextf();
+        float u[12];
+        const int draws = g ? 12 : 8;
+        for (int i = 0; i < draws; ++i) u[i] = rng.nextf();
         V3 wo_w = -dir;
         V3 n = (m.type == 2) ? ng : (dot(wo_w, ng) < 0.0f ? -ng : ng);
         Frame fr(n);
         V3 wo = fr.to_local(wo_w);
+        const bool guided = g && g->sample && (m.type == 0 || (m.type == 1 && m.alpha >= g->min_alpha));
+        double gout[32];
+        if (guided) {
+            const float in6[6] = {(P.x - g->bmin.x) * g->binv.x, (P.y - g->bmin.y) * g->binv.y,
+                                  (P.z - g->bmin.z) * g->binv.z, n.x, n.y, n.z};
+            or_mlp_forward(g->params, 1, in6, 1, gout);
+        }
+        auto guide_pdf = [&](V3 w) {
+            const float wd[3] = {w.x, w.y, w.z};
+            return (float)or_vmf_pdf(gout, wd);
+        };
         // next-event estimation (GeometryLight::sample, light.cpp:52-77)
         if (nl > 0) {
             uint32_t li = std::min((uint32_t)(u[0] * (float)nl), nl - 1);
@@ -594,6 +623,7 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
                     bsdf_eval(m, wo, wi, f, pdf_b);
                     if (!is_zero(f)) {
                         float pdf_l = (1.0f / (float)nl) * (1.0f / lt.area) * (dist * dist) / cos_l;
+                        if (guided) pdf_b = g->frac * pdf_b + (1.0f - g->frac) * guide_pdf(wi_w);
                         float w = pdf_l * pdf_l / (pdf_l * pdf_l + pdf_b * pdf_b);
                         V3 contrib = mul(mul(beta, f), lt.Le) * (std::fabs(wi.y) * w / pdf_l);
                         float sray[8] = {org.x, org.y, org.z, 0.0f, wi_w.x, wi_w.y, wi_w.z, dist * 0.9999f};
@@ -614,7 +644,21 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
         }
         V3 wi, f;
         float pdf;
-        if (!bsdf_sample(m, wo, u[5], u[6], u[7], wi, f, pdf)) break;
+        if (guided && !(u[8] < g->frac)) {
+            const float u3[3] = {u[9], u[10], u[11]};
+            float wd[3];
+            double q;
+            or_vmf_sample(gout, u3, wd, &q);
+            const V3 ww = v3(wd[0], wd[1], wd[2]);
+            wi = fr.to_local(ww);
+            float pdf_bsdf;
+            bsdf_eval(m, wo, wi, f, pdf_bsdf);
+            pdf = g->frac * pdf_bsdf + (1.0f - g->frac) * guide_pdf(ww);
+            if (is_zero(f) || !(pdf > 0.0f)) break;
+        } else {
+            if (!bsdf_sample(m, wo, u[5], u[6], u[7], wi, f, pdf)) break;
+            if (guided) pdf = g->frac * pdf + (1.0f - g->frac) * guide_pdf(fr.to_world(wi));
+        }
         beta = mul(beta, f) * (std::fabs(wi.y) / pdf);
         if (is_zero(beta)) break;
         prev_pdf = pdf;
@@ -720,9 +764,31 @@ void or_scene_free(or_scene* s) {
     delete s;
 }
 
+static Guide make_guide(const or_guide_cfg* c) {
+    Guide g;
+    g.params = c->params;
+    g.bmin = load3(c->bmin);
+    g.binv = load3(c->binv);
+    g.frac = c->bsdf_fraction;
+    g.min_alpha = c->min_alpha;
+    g.sample = c->sample != 0;
+    return g;
+}
+
 int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n, const or_render_desc* d,
                       float* out, int threads) {
+    return or_integrate_rays_guided(s, rays, n, d, nullptr, out, threads);
+}
+
+int or_integrate_rays_guided(const or_scene* s, const float* rays, uint32_t n, const or_render_desc* d,
+                             const or_guide_cfg* gc, float* out, int threads) {
     if (!s || !d || (!rays && n)) return 1;
+    Guide gd{};
+    const Guide* g = nullptr;
+    if (gc) {
+        gd = make_guide(gc);
+        g = &gd;
+    }
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<uint32_t> next{0};
     auto worker = [&]() {
@@ -732,7 +798,7 @@ int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n, const or
             Pcg32 rng;
             rng.seed(path_initstate(d->seed, d->spp_offset), i);
             const float* r = rays + 8 * (size_t)i;
-            V3 L = trace_path(*s, *d, rng, load3(r), load3(r + 4));
+            V3 L = trace_path(*s, *d, rng, load3(r), load3(r + 4), g);
             out[3 * (size_t)i] = L.x;
             out[3 * (size_t)i + 1] = L.y;
             out[3 * (size_t)i + 2] = L.z;
@@ -746,7 +812,18 @@ int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n, const or
 
 int or_render(const or_scene* s, const or_render_desc* d, double* out, float* sample0,
               int threads) {
+    return or_render_guided(s, d, nullptr, out, sample0, threads);
+}
+
+int or_render_guided(const or_scene* s, const or_render_desc* d, const or_guide_cfg* gc, double* out,
+                     float* sample0, int threads) {
     if (!s || !d || !out || d->width == 0 || d->height == 0) return 1;
+    Guide gd{};
+    const Guide* g = nullptr;
+    if (gc) {
+        gd = make_guide(gc);
+        g = &gd;
+    }
     Camera cam = make_camera(*d);
     const uint32_t npix = d->width * d->height;
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
@@ -758,7 +835,7 @@ int or_render(const or_scene* s, const or_render_desc* d, double* out, float* sa
             uint32_t px = p % d->width, py = p / d->width;
             double acc[3] = {0, 0, 0};
             for (uint32_t si = 0; si < d->spp; ++si) {
-                V3 L = path_radiance(*s, *d, cam, px, py, d->spp_offset + si);
+                V3 L = path_radiance(*s, *d, cam, px, py, d->spp_offset + si, g);
                 if (si == 0 && sample0) {
                     sample0[3 * (size_t)p] = L.x;
                     sample0[3 * (size_t)p + 1] = L.y;

@@ -1,0 +1,134 @@
+"""Guided wavefront integrator (path-guiding network in the shade stage).
+
+(1) Per path: with a fixed network the GPU guided paths equal the oracle's
+guided restatement (oracle/oracle_render.cpp trace_path with a guide) up to
+float rounding, for nearly every path.  (2) Unbiasedness: guided renders that
+train the network while they render still reproduce the furnace closed form.
+(3) Gain: on the slit-lit glossy room, the network trained online lowers the
+error at equal sample count relative to plain BSDF sampling (the ablation of
+the learned part).  The high-spp reference of (3) is the plain GPU tracer,
+checked against the FP64 oracle on the same scene in (4)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from kazen_proto_amd import native, scenes
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from test_oracle_integrator import random_guide_params  # noqa: E402
+from test_render_gpu import CPU_THREADS, odesc, oracle_scene, rel_mse  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def scene_box(s):
+    v = s.soup().reshape(-1, 3)
+    return v.min(0), v.max(0)
+
+
+def guided_ctx(s, params=None, seed=3, **cfg):
+    c = native.Context(0)
+    c.load_scene(s)
+    c.guiding_init(seed)
+    if params is not None:
+        c.guiding_set_params(params)
+    c.guiding_configure(**cfg)
+    return c
+
+
+def desc(s, spp=None, guided=False, seed=None):
+    d = native.make_render_desc(s, spp=spp, guided=guided)
+    if seed is not None:
+        d.seed = seed
+    return d
+
+
+def camera_rays(s):
+    r = scenes.pinhole_rays(s.eye, s.lookat, s.up, s.fov_deg, s.width, s.height)
+    return r
+
+
+@pytest.mark.parametrize("scene", ["guiding", "materials"])
+def test_guided_paths_match_oracle(scene):
+    if scene == "guiding":
+        s = scenes.guiding_scene(width=64, height=64, spp=1, max_depth=6, n_spheres=16)
+    else:
+        s = scenes.materials_scene(width=48, height=48, spp=1)
+    lo, hi = scene_box(s)
+    p = random_guide_params(5)
+    c = guided_ctx(s, params=p, bsdf_fraction=0.5, min_alpha=0.1, box=(lo, hi))
+    rays = camera_rays(s)
+    g = c.integrate_rays(rays, desc(s, guided=True)).astype(np.float64)
+    c.close()
+    o = oracle_scene(s).integrate_rays(rays, odesc(s), threads=CPU_THREADS,
+                                       guide=oracle.make_guide(p, lo, hi, 0.5, 0.1)).astype(np.float64)
+    close = np.all(np.abs(g - o) <= 2e-3 * np.abs(o) + 1e-5, axis=1)
+    lit = np.any(o > 0, axis=1)
+    print("%s: %d paths, %.4f agree, %d lit" % (scene, close.size, close.mean(), lit.sum()))
+    assert lit.sum() > 0.1 * lit.size
+    assert close.mean() > 0.97
+
+
+def test_guided_training_render_keeps_furnace_mean():
+    s = scenes.furnace_box(albedo=0.6, Le=1.0, max_depth=4, rr_depth=2, spp=64, width=32, height=32)
+    c = guided_ctx(s, bsdf_fraction=0.5, lr=5e-3, train_steps=2, guide_after=2, pass_paths=1024)
+    img = c.render(desc(s, guided=True))
+    st = c.render_stats()
+    c.close()
+    assert st["passes"] == 64 and st["train_steps"] == 128 and st["guided_passes"] == 62
+    assert st["train_records"] > 0
+    pix = img[..., 0].astype(np.float64).ravel()
+    err = pix.std() / np.sqrt(pix.size)
+    expect = scenes.furnace_expected(0.6, 1.0, 4)
+    print("guided furnace %.5f expected %.5f (+-%.5f)" % (pix.mean(), expect, err))
+    assert abs(pix.mean() - expect) < 5 * err
+
+
+def _slit_scene(res, spp):
+    return scenes.guiding_scene(width=res, height=res, spp=spp, max_depth=10)
+
+
+def test_trained_guide_beats_bsdf_sampling_at_equal_spp():
+    res, spp = 96, 64
+    s = _slit_scene(res, spp)
+    ref_ctx = native.Context(0)
+    ref_ctx.load_scene(s)
+    ref = ref_ctx.render(desc(s, spp=4096, seed=99)).astype(np.float64)
+    plain = ref_ctx.render(desc(s, spp=spp))
+    ref_ctx.close()
+    c = guided_ctx(s, bsdf_fraction=0.5, min_alpha=0.1, lr=2e-3, train_steps=4, guide_after=4,
+                   pass_paths=res * res)
+    guided = c.render(desc(s, spp=spp, guided=True))
+    st = c.render_stats()
+    c.close()
+    e_plain, e_guided = rel_mse(plain, ref), rel_mse(guided, ref)
+    print("slit room %dx%d @%d spp: relMSE plain %.4f guided %.4f (train steps %d, records %d)"
+          % (res, res, spp, e_plain, e_guided, st["train_steps"], st["train_records"]))
+    assert np.isfinite(guided).all()
+    assert e_guided < 0.85 * e_plain
+
+
+def test_slit_room_gpu_reference_matches_oracle():
+    s = _slit_scene(24, 1024)
+    c = native.Context(0)
+    c.load_scene(s)
+    g = c.render(desc(s)).astype(np.float64)
+    g2 = c.render(desc(s, seed=1234)).astype(np.float64)
+    c.close()
+    o = oracle_scene(s).render(odesc(s), threads=CPU_THREADS)
+    # two independent GPU renders bound the noise level of a single one
+    e_gg, e_go = rel_mse(g, g2), rel_mse(g, o)
+    print("slit room 24x24 @1024: relMSE gpu-gpu %.5f gpu-oracle %.5f" % (e_gg, e_go))
+    assert e_go < 1.5 * e_gg + 1e-4
+
+
+def test_guided_render_needs_init():
+    s = scenes.cornell_box(width=8, height=8, spp=1, sphere_lon=8, sphere_lat=4)
+    c = native.Context(0)
+    c.load_scene(s)
+    with pytest.raises(native.KpError):
+        c.render(desc(s, guided=True))
+    c.close()

@@ -64,3 +64,59 @@ def test_cornell_energy_is_finite_and_lit():
     # red wall on the left, green on the right
     left, right = img[:, 1:4].mean((0, 1)), img[:, -4:-1].mean((0, 1))
     assert left[0] > left[1] and right[1] > right[0]
+
+
+def random_guide_params(seed, out_scale=3.0):
+    """Glorot-uniform network parameters in the canonical order, output layer
+    scaled by out_scale so the mixture is far from uniform."""
+    rng = np.random.default_rng(seed)
+    dims = [(48, 64), (64, 64), (64, 64), (64, 64), (64, 32)]
+    parts = []
+    for li, (din, dout) in enumerate(dims):
+        lim = np.sqrt(6.0 / (din + dout)) * (out_scale if li == 4 else 1.0)
+        parts.append(rng.uniform(-lim, lim, din * dout))
+        parts.append(rng.uniform(-0.5, 0.5, dout) * (out_scale if li == 4 else 0.1))
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == 17696
+    return p
+
+
+@pytest.mark.parametrize("albedo,depth,rr", [(0.5, 4, 64), (0.7, 5, 2)])
+def test_guided_furnace_closed_form(albedo, depth, rr):
+    # One-sample MIS between the BSDF and an arbitrary (here random, sharply
+    # peaked) network density must stay unbiased, and next-event estimation
+    # must weight against the same combined density: any slip in either
+    # (a missing (1 - frac), the BSDF-only pdf in the power heuristic) moves
+    # the mean away from the closed form.
+    s = scenes.furnace_box(albedo=albedo, Le=1.0, max_depth=depth, rr_depth=rr, spp=128)
+    lo, hi = s.soup().reshape(-1, 3).min(0), s.soup().reshape(-1, 3).max(0)
+    g = oracle.make_guide(random_guide_params(7), lo, hi, bsdf_fraction=0.5)
+    img = oracle_scene(s).render(desc(s), guide=g)
+    expect = scenes.furnace_expected(albedo, 1.0, depth)
+    pix = img[..., 0].ravel()
+    err = pix.std() / np.sqrt(pix.size)
+    assert abs(pix.mean() - expect) < 5 * err + 1e-9, (pix.mean(), expect, err)
+
+
+def test_guided_network_changes_paths_not_the_mean():
+    # The guide must actually be sampled (images differ per path from BSDF-only
+    # sampling with the same 12-draw protocol) while the mean is unchanged.
+    s = scenes.cornell_box(width=16, height=16, spp=64, sphere_lon=16, sphere_lat=8)
+    lo, hi = s.soup().reshape(-1, 3).min(0), s.soup().reshape(-1, 3).max(0)
+    p = random_guide_params(11)
+    sc = oracle_scene(s)
+    a = sc.render(desc(s), guide=oracle.make_guide(p, lo, hi, sample=False))
+    b = sc.render(desc(s), guide=oracle.make_guide(p, lo, hi, sample=True))
+    plain = sc.render(desc(s, spp=256))
+    assert np.mean(np.abs(a - b)) > 1e-3
+
+    def mean_err(img):
+        # pixels are independent: the spread of pixel values bounds the
+        # standard error of the image mean from above
+        v = img.mean(2).ravel()
+        return v.mean(), v.std() / np.sqrt(v.size)
+
+    mp, ep = mean_err(plain)
+    for img in (a, b):
+        m, e = mean_err(img)
+        assert abs(m - mp) < 4 * np.hypot(e, ep), (m, mp, e, ep)
