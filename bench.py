@@ -118,6 +118,41 @@ def guiding_equal_error(args):
             "plain_throughput": plain_rate, "wall_seconds": round(time.time() - t0, 1)}
 
 
+def path_trace_leg(args):
+    """Plain wavefront path trace of the terrain workload (2048^2 fBm
+    heightfield, 256 spheres, three material families, 4 area lights) at
+    args.pt_res^2 x args.pt_spp, depth 8: samples/s and rays/s over the
+    whole kp_render_accumulate (all bounces, shading, shadow rays, film)."""
+    from kazen_proto_amd import native, scenes
+    t0 = time.time()
+    s = scenes.terrain_scene(grid=args.pt_grid, width=args.pt_res, height=args.pt_res, spp=args.pt_spp)
+    gen = time.time() - t0
+    c = native.Context(0)
+    c.load_scene(s)
+    st = c.stats()
+    print("[bench] terrain %d tris (generated in %.1fs), BVH build %.2fs" % (s.n_tris, gen, st["build_seconds"]),
+          file=sys.stderr, flush=True)
+    c.film_reset(s.width, s.height)
+    c.render_accumulate(native.make_render_desc(s, spp=1))      # warm-up (scene upload, path buffers)
+    c.film_reset(s.width, s.height)
+    c.render_accumulate(native.make_render_desc(s))
+    rs = c.render_stats()
+    img = c.film_read(s.width, s.height)
+    c.close()
+    rays = rs["extension_rays"] + rs["shadow_rays"]
+    out = {"workload": "terrain %dx%d heightfield + 256 spheres, %dx%d, %d spp, depth 8"
+                       % (args.pt_grid, args.pt_grid, s.width, s.height, s.spp),
+           "triangles": int(s.n_tris), "seconds": round(rs["seconds"], 4),
+           "samples_per_s": round(rs["paths"] / rs["seconds"], 1),
+           "mrays_per_s": round(rays / rs["seconds"] / 1e6, 2),
+           "extension_rays": int(rs["extension_rays"]), "shadow_rays": int(rs["shadow_rays"]),
+           "passes": int(rs["passes"]), "paths_per_pass": int(rs["paths_per_pass"]),
+           "bvh_build_seconds": round(st["build_seconds"], 3), "image_mean": round(float(img.mean()), 5),
+           "finite": bool(np.isfinite(img).all())}
+    print("[bench] path trace: %s" % out, file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +167,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy kernel of MIB MiB (reference kernel of "
                          "known traffic for HBM counter calibration)")
+    ap.add_argument("--no-pathtrace", action="store_true", help="skip the terrain path-trace leg")
+    ap.add_argument("--pt-grid", type=int, default=2048)
+    ap.add_argument("--pt-res", type=int, default=1024)
+    ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain equal-error leg")
     ap.add_argument("--guiding-res", type=int, default=512)
     ap.add_argument("--guiding-ref-spp", type=int, default=16384)
@@ -242,6 +281,9 @@ def main():
         mm = int(np.count_nonzero(np.any(got.view(np.uint32) != hits_cpu.view(np.uint32), axis=1)))
         parity = {"incoherent_mismatches": mm, "rays": int(n)}
 
+    path_trace = None
+    if not args.no_pathtrace:
+        path_trace = path_trace_leg(args)
     guiding = None
     if not args.no_guiding:
         guiding = guiding_equal_error(args)
@@ -267,6 +309,7 @@ def main():
         "primary_mrays_per_s": mrays_prim,
         "parity": parity,
         "bvh_build_seconds": round(st["build_seconds"], 3),
+        "path_trace": path_trace,
         "guiding": guiding,
     }
     print(json.dumps(line), flush=True)

@@ -385,3 +385,89 @@ def guiding_scene(width=512, height=512, spp=16, max_depth=10, rr_depth=3, seed=
         s.materials.append(material(CONDUCTOR, GGX, F0=F0, alpha=alpha))
         s.add_mesh(uv_sphere(sphere_lon, sphere_lat, center=tuple(c), radius=r), m)
     return s
+
+
+def fbm_heights(xs, zs, perm, octaves=6, lacunarity=2.0, gain=0.5, base_freq=0.25):
+    """fBm of Perlin noise on the grid xs x zs (sum over octaves o of
+    gain^o noise(lacunarity^o base_freq (x, y_o, z)), y_o = 0.37 o decorrelates
+    the octaves); returns [len(zs), len(xs)]."""
+    X, Z = np.meshgrid(np.asarray(xs, np.float64), np.asarray(zs, np.float64))
+    h = np.zeros(X.shape)
+    amp, freq = 1.0, base_freq
+    for o in range(octaves):
+        pts = np.stack([X.ravel() * freq, np.full(X.size, 0.37 * o), Z.ravel() * freq], 1)
+        h += amp * perlin3(pts, perm).reshape(X.shape)
+        amp *= gain
+        freq *= lacunarity
+    return h
+
+
+def heightfield_tris(xs, zs, h):
+    """Two triangles per grid cell, upward winding; [2 (nz-1)(nx-1), 9] float32."""
+    X, Z = np.meshgrid(np.asarray(xs, np.float32), np.asarray(zs, np.float32))
+    P = np.stack([X, np.asarray(h, np.float32), Z], -1)
+    v00, v10 = P[:-1, :-1], P[:-1, 1:]
+    v01, v11 = P[1:, :-1], P[1:, 1:]
+    a = np.concatenate([v00, v01, v10], -1)
+    b = np.concatenate([v10, v01, v11], -1)
+    return np.stack([a, b], 2).reshape(-1, 9)
+
+
+def random_material(rng):
+    """Uniform over Lambert (albedo U[0.2,0.8]), GGX conductor (alpha
+    U[0.05,0.8], F0 U[0.5,1]) and Beckmann dielectric (eta 1.5, alpha
+    U[0.05,0.5])."""
+    kind = int(rng.integers(3))
+    if kind == 0:
+        return material(LAMBERT, albedo=rng.uniform(0.2, 0.8, 3))
+    if kind == 1:
+        return material(CONDUCTOR, GGX, F0=rng.uniform(0.5, 1.0, 3), alpha=rng.uniform(0.05, 0.8))
+    return material(DIELECTRIC, BECKMANN, alpha=rng.uniform(0.05, 0.5), eta=1.5)
+
+
+def terrain_scene(grid=2048, n_spheres=256, sphere_lon=64, sphere_lat=32, tiles=8, width=1024, height=1024,
+                  spp=64, max_depth=8, rr_depth=3, seed=2, extent=10.0):
+    """Full plain path-trace workload: a grid x grid heightfield over
+    [-extent, extent]^2 (2 (grid-1)^2 triangles, 8,380,418 at 2048), heights
+    1.5 fBm (6 octaves, lacunarity 2, gain 0.5), split into tiles x tiles
+    meshes; n_spheres copies of one UV sphere (sphere_lon x sphere_lat, 4,032
+    triangles at 64 x 32) at uniform random positions above the ground,
+    radius U[0.15, 0.45], wound outward, flattened into world space; every mesh draws its
+    material with random_material; 4 area-light quads (2 x 2, radiance 30)
+    at height 7 facing down.  Camera (0, 6, 15) looking at (0, 0, 0), 50
+    degree field of view."""
+    rng = np.random.default_rng(seed)
+    perm = _perlin_perm(rng)
+    s = Scene(name="terrain", width=width, height=height, spp=spp, max_depth=max_depth,
+              rr_depth=rr_depth, seed=seed, eye=(0.0, 6.0, 15.0), lookat=(0.0, 0.0, 0.0),
+              up=(0.0, 1.0, 0.0), fov_deg=50.0)
+    xs = np.linspace(-extent, extent, grid)
+    h = 1.5 * fbm_heights(xs, xs, perm)
+    step = (grid - 1 + tiles - 1) // tiles
+    for tz in range(tiles):
+        for tx in range(tiles):
+            z0, x0 = tz * step, tx * step
+            z1, x1 = min(z0 + step, grid - 1), min(x0 + step, grid - 1)
+            if z1 <= z0 or x1 <= x0:
+                continue
+            m = len(s.materials)
+            s.materials.append(random_material(rng))
+            s.add_mesh(heightfield_tris(xs[x0:x1 + 1], xs[z0:z1 + 1], h[z0:z1 + 1, x0:x1 + 1]), m)
+    # uv_sphere winds inward; swap two vertices so winding normals point out
+    # (the dielectric's outside is the side its winding normal faces)
+    unit = uv_sphere(sphere_lon, sphere_lat).reshape(-1, 3, 3)[:, [0, 2, 1]]
+    for _ in range(n_spheres):
+        r = rng.uniform(0.15, 0.45)
+        cx, cz = rng.uniform(-0.9 * extent, 0.9 * extent, 2)
+        ix = int(np.clip(np.searchsorted(xs, cx), 0, grid - 1))
+        iz = int(np.clip(np.searchsorted(xs, cz), 0, grid - 1))
+        cy = h[iz, ix] + r * rng.uniform(1.0, 2.5)
+        m = len(s.materials)
+        s.materials.append(random_material(rng))
+        s.add_mesh((unit * r + np.array([cx, cy, cz])).reshape(-1, 9), m)
+    light = len(s.materials)
+    s.materials.append(material(LAMBERT, albedo=(0, 0, 0), Le=(30, 30, 30)))
+    for lx, lz in ((-5, -5), (5, -5), (-5, 5), (5, 5)):
+        y = 7.0
+        s.add_mesh(quad((lx - 1, y, lz - 1), (lx + 1, y, lz - 1), (lx + 1, y, lz + 1), (lx - 1, y, lz + 1)), light)
+    return s

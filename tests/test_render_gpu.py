@@ -120,6 +120,30 @@ def test_all_closures_match_oracle_statistics():
     assert e < 1.6 * e0
 
 
+def _small_terrain():
+    return scenes.terrain_scene(grid=128, n_spheres=16, sphere_lon=16, sphere_lat=8, width=32, height=32,
+                                spp=1, max_depth=8)
+
+
+def test_terrain_per_path_agreement():
+    frac, g, o = _per_path(_small_terrain())
+    print("terrain: %.4f of pixels agree path by path" % frac)
+    assert frac >= 0.95
+
+
+def test_terrain_matches_oracle_statistics():
+    s = _small_terrain()
+    osc = oracle_scene(s)
+    ref = osc.render(odesc(s, spp=4096, seed=700), threads=CPU_THREADS)
+    indep = osc.render(odesc(s, spp=256, seed=701), threads=CPU_THREADS)
+    c = gpu_ctx(s)
+    img = c.render(gdesc(s, spp=256, seed=702))
+    c.close()
+    e, e0 = rel_mse(img, ref), rel_mse(indep, ref)
+    print("terrain 256 spp relMSE gpu %.3e oracle %.3e" % (e, e0))
+    assert e < 1.6 * e0
+
+
 def test_film_accumulate_resumes_bit_for_bit_samples():
     s = scenes.cornell_box(width=32, height=32, spp=8)
     c = gpu_ctx(s)
