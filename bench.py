@@ -153,6 +153,57 @@ def path_trace_leg(args):
     return out
 
 
+def batch_leg(args):
+    """Batched uneven workloads: 32 terrain-style scenes (seeds 100-131),
+    triangle counts log-uniform in [1e4, 2e6], resolutions from {128^2,
+    256^2, 512^2, 1024^2}, spp from {4, 16, 64}, depth 8, rendered as one
+    batch (one traversal launch per bounce over all scenes, replayed pass
+    graph) after autotuning, and for comparison one scene after another."""
+    from kazen_proto_amd import native, scenes
+    t0 = time.time()
+    rng = np.random.default_rng(100)
+    ss = []
+    for i in range(args.batch_scenes):
+        target = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        grid = max(16, int(round(np.sqrt(max(target - 16 * 240, 1000) / 2.0))) + 1)
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.choice([4, 16, 64]))
+        ss.append(scenes.terrain_scene(grid=grid, n_spheres=16, sphere_lon=16, sphere_lat=8, width=res,
+                                       height=res, spp=spp, max_depth=8, seed=100 + i))
+    gen = time.time() - t0
+    owner = native.Context(0)
+    ctxs = []
+    for s in ss:
+        c = native.Context(0)
+        c.load_scene(s)
+        ctxs.append(c)
+    descs = [native.make_render_desc(s) for s in ss]
+    # one scene after another (each render is its own sequence of launches)
+    seq_sec = 0.0
+    for c, d in zip(ctxs, descs):
+        c.render(d)
+        seq_sec += c.render_stats()["seconds"]
+    b = native.Batch(owner, ctxs)
+    for c in ctxs:
+        c.close()
+    tuned = b.autotune(descs, passes=2).as_dict()
+    b.render(descs)                 # warm-up with the tuned settings
+    b.render(descs)
+    st = b.stats()
+    b.close()
+    owner.close()
+    rays = st["extension_rays"] + st["shadow_rays"]
+    out = {"workload": "%d terrain scenes, seeds 100-%d, 1e4-2e6 tris, 128^2-1024^2, 4-64 spp, depth 8"
+                       % (len(ss), 99 + len(ss)),
+           "triangles": int(sum(s.n_tris for s in ss)), "samples": int(st["paths"]),
+           "seconds": round(st["seconds"], 4), "samples_per_s": round(st["paths"] / st["seconds"], 1),
+           "mrays_per_s": round(rays / st["seconds"] / 1e6, 2), "passes": int(st["passes"]),
+           "one_by_one_seconds": round(seq_sec, 4), "speedup_vs_one_by_one": round(seq_sec / st["seconds"], 3),
+           "tuning": tuned, "setup_seconds": round(time.time() - t0, 1), "generate_seconds": round(gen, 1)}
+    print("[bench] batch: %s" % out, file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +222,8 @@ def main():
     ap.add_argument("--pt-grid", type=int, default=2048)
     ap.add_argument("--pt-res", type=int, default=1024)
     ap.add_argument("--pt-spp", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain equal-error leg")
     ap.add_argument("--guiding-res", type=int, default=512)
     ap.add_argument("--guiding-ref-spp", type=int, default=16384)
@@ -284,6 +337,9 @@ def main():
     path_trace = None
     if not args.no_pathtrace:
         path_trace = path_trace_leg(args)
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(args)
     guiding = None
     if not args.no_guiding:
         guiding = guiding_equal_error(args)
@@ -310,6 +366,7 @@ def main():
         "parity": parity,
         "bvh_build_seconds": round(st["build_seconds"], 3),
         "path_trace": path_trace,
+        "batch": batch,
         "guiding": guiding,
     }
     print(json.dumps(line), flush=True)

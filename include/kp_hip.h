@@ -274,6 +274,39 @@ typedef struct {
 /* Needs kp_guiding_init first; a later kp_guiding_init keeps the settings. */
 int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);
 
+/* Batched scenes -------------------------------------------------------------
+ * No counterpart in the reference (its Renderer::render, src/core/renderer.cpp:
+ * 18-90, renders one scene per call). kp_batch_create copies the built scenes
+ * of n contexts (same device as owner; kp_bvh_build and materials done) into
+ * one merged workload owned by owner; the scene contexts may change or go
+ * away afterwards. kp_batch_render traces descs[i].spp samples of every scene
+ * i into fresh per-scene films, in passes of pass_paths paths drawn from all
+ * scenes at once: each bounce is one persistent traversal launch over every
+ * scene's rays. With use_graph one pass (about six launches per bounce) is
+ * captured into a HIP graph once and replayed. Scene i's film equals, bit for
+ * bit, kp_render of that scene alone with descs[i] (not guided).
+ * kp_batch_autotune times candidate settings on `passes` passes of the
+ * batch's own work, one setting at a time, and keeps the fastest. The BVH
+ * node width is 8 in every configuration. */
+typedef struct kp_batch kp_batch;
+typedef struct {
+    uint32_t pass_paths;           /* paths in flight per pass (queue size), 256 .. 2^24 */
+    uint32_t shade_blocks_per_cu;  /* 256-thread shade / connect blocks per CU */
+    uint32_t trace_blocks_per_cu;  /* persistent traversal blocks per CU (occupancy) */
+    uint32_t refill_below;         /* traversal wave refill threshold, lanes 1..64 */
+    uint32_t use_graph;            /* replay a captured pass graph */
+    uint32_t reserved[3];
+} kp_batch_tuning;
+int kp_batch_create(kp_ctx* owner, kp_ctx* const* scenes, uint32_t n, kp_batch** out);
+int kp_batch_destroy(kp_batch* batch);
+int kp_batch_set_tuning(kp_batch* batch, const kp_batch_tuning* tuning);
+int kp_batch_get_tuning(kp_batch* batch, kp_batch_tuning* tuning);
+int kp_batch_render(kp_batch* batch, const kp_render_desc* descs);
+int kp_batch_autotune(kp_batch* batch, const kp_render_desc* descs, uint32_t passes, kp_batch_tuning* out);
+/* Mean radiance of scene `scene` of the last kp_batch_render. */
+int kp_batch_film_read(kp_batch* batch, uint32_t scene, float* rgb, int ptr_kind);
+int kp_batch_get_stats(kp_batch* batch, kp_render_stats* out);
+
 /* Device memory and timing helpers for callers without their own HIP
  * runtime handle (the reference has no counterpart: its callers never see
  * device memory). Buffers belong to the caller and must be freed with

@@ -24,6 +24,7 @@ struct TraceArgs {
     uint32_t* err;          // bit0 stack overflow, bit1 step limit
     uint32_t max_steps;     // per-ray step limit (> any legitimate traversal)
     unsigned long long* counters;   // [nodes visited, triangles tested]
+    const uint32_t* roots;  // if set, root node of each ray (batched scenes); else node 0
 };
 
 hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStream_t s);

@@ -58,6 +58,23 @@ struct PathState {
     uint32_t* socc;       // occlusion result per shadow ray
     uint32_t* cnt;        // [0],[1] ray queue sizes, [2] shadow queue size
     unsigned long long* totals;   // [0] extension rays, [1] shadow rays
+    // batched scenes only
+    uint32_t* slot_scene; // per slot: scene of the path
+    uint32_t* slot_pixel; // per slot: pixel of the path in its scene
+    uint32_t* qroot[2];   // per ray-queue entry: BVH root of the ray's scene
+    uint32_t* sroot;      // per shadow-queue entry: BVH root
+};
+
+// One scene of a batch (the camera, film and sampling settings of its
+// kp_render_desc, its BVH root and light range in the merged arrays).  Work
+// item w of the batch is sample (w - work_first) % spp of pixel
+// (w - work_first) / spp of the scene with the last work_first <= w.
+struct SceneRec {
+    float3 eye, fwd, right, upv;
+    float tan_half, aspect;
+    uint32_t width, height, spp, spp_offset, seed, max_depth, rr_depth;
+    uint32_t root, light_first, n_lights, film_first, pad;
+    unsigned long long work_first;
 };
 
 struct RenderArgs {
@@ -81,6 +98,11 @@ struct RenderArgs {
     float* rec_in6;
     float* rec_dirs;
     float4* rec_aux;
+    // batched scenes (scenes != nullptr): n_paths is the pass capacity and
+    // pass[0] / pass[1] hold the pass's first work item / the total on the device
+    const SceneRec* scenes;
+    uint32_t n_scenes;
+    unsigned long long* pass;
 };
 
 // ------------------------------------------------------------------ RNG --
@@ -186,6 +208,125 @@ __global__ __launch_bounds__(256) void k_generate_rays(RenderArgs a, const float
     a.ps.rng[slot] = rng.state;
 }
 
+// Paths of the pass that starts at work item pass[0] (batched scenes).
+__device__ __forceinline__ uint32_t batch_pass_paths(const RenderArgs& a) {
+    const unsigned long long w0 = a.pass[0], total = a.pass[1];
+    const unsigned long long left = total > w0 ? total - w0 : 0ull;
+    return left < a.n_paths ? (uint32_t)left : a.n_paths;
+}
+
+// Starts the paths of one pass over a batch of scenes: the same camera ray,
+// PCG32 stream and sample index as k_generate gives the pixel's sample in a
+// render of that scene alone.
+__global__ __launch_bounds__(256) void k_generate_batch(RenderArgs a) {
+#pragma clang fp contract(off)
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = batch_pass_paths(a);
+    if (slot == 0) {
+        a.ps.cnt[0] = n;
+        a.ps.cnt[1] = 0;
+        a.ps.cnt[2] = 0;
+    }
+    if (slot >= n) return;
+    const unsigned long long w = a.pass[0] + slot;
+    uint32_t lo = 0, hi = a.n_scenes;   // last scene with work_first <= w
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.scenes[mid].work_first <= w) lo = mid;
+        else hi = mid;
+    }
+    const SceneRec sc = a.scenes[lo];
+    const unsigned long long l = w - sc.work_first;
+    const uint32_t pixel = (uint32_t)(l / sc.spp);
+    const uint32_t sample = sc.spp_offset + (uint32_t)(l - (unsigned long long)pixel * sc.spp);
+    const uint32_t px = pixel % sc.width, py = pixel / sc.width;
+    const uint64_t initstate = ((uint64_t)sample << 32) ^ (uint64_t)sc.seed ^ 0x9E3779B97F4A7C15ull;
+    Pcg rng = pcg_seed(initstate, pixel);
+    const float ux = rng.nextf(), uy = rng.nextf();
+    const float sx = ((float)px + ux) / (float)sc.width, sy = ((float)py + uy) / (float)sc.height;
+    const float nx = (2.0f * sx - 1.0f) * sc.tan_half * sc.aspect;
+    const float ny = (1.0f - 2.0f * sy) * sc.tan_half;
+    const f3 fwd = mk(sc.fwd.x, sc.fwd.y, sc.fwd.z), right = mk(sc.right.x, sc.right.y, sc.right.z);
+    const f3 upv = mk(sc.upv.x, sc.upv.y, sc.upv.z);
+    const f3 d = normalize(fwd + right * nx + upv * ny);
+    a.ps.rays[0][2 * (size_t)slot] = make_float4(sc.eye.x, sc.eye.y, sc.eye.z, 0.0f);
+    a.ps.rays[0][2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, __int_as_float(0x7f800000));
+    a.ps.qmap[0][slot] = slot;
+    a.ps.qroot[0][slot] = sc.root;
+    a.ps.slot_scene[slot] = lo;
+    a.ps.slot_pixel[slot] = pixel;
+    a.ps.s0[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.ps.s1[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a.ps.rng[slot] = rng.state;
+}
+
+// Folds the pass's path radiances into the scenes' films: the first slot of
+// every pixel adds that pixel's consecutive samples in sample order (Kahan),
+// continuing the sum of earlier passes, so the film equals the one-scene
+// render's bit for bit whatever the pass size.
+__global__ __launch_bounds__(256) void k_film_batch(RenderArgs a, float4* sum, float4* comp) {
+#pragma clang fp contract(off)
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = batch_pass_paths(a);
+    if (slot >= n) return;
+    const uint32_t sid = a.ps.slot_scene[slot], pix = a.ps.slot_pixel[slot];
+    if (slot > 0 && a.ps.slot_scene[slot - 1] == sid && a.ps.slot_pixel[slot - 1] == pix) return;
+    const uint32_t f = a.scenes[sid].film_first + pix;
+    float4 S = sum[f], C = comp[f];
+    for (uint32_t j = slot; j < n && a.ps.slot_scene[j] == sid && a.ps.slot_pixel[j] == pix; ++j) {
+        const float4 x = a.ps.s1[j];
+        float y, t;
+        y = x.x - C.x; t = S.x + y; C.x = (t - S.x) - y; S.x = t;
+        y = x.y - C.y; t = S.y + y; C.y = (t - S.y) - y; S.y = t;
+        y = x.z - C.z; t = S.z + y; C.z = (t - S.z) - y; S.z = t;
+    }
+    sum[f] = S;
+    comp[f] = C;
+}
+
+// Moves the batch to its next pass (part of the captured pass graph).
+__global__ void k_pass_advance(unsigned long long* pass, uint32_t paths) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) pass[0] += paths;
+}
+
+// Adds offsets to the node and triangle records of one scene copied into the
+// merged arrays: inner-child and leaf bases, and the primitive id.
+__global__ void k_rebase_nodes(uint4* nodes, uint32_t n, uint32_t node_off, uint32_t tri_off) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint4 w = nodes[8 * (size_t)i + 1];   // dwords 4-7: child_base, prim_base, meta
+    w.x += node_off;
+    w.y += tri_off;
+    nodes[8 * (size_t)i + 1] = w;
+}
+__global__ void k_rebase_tris(float4* tris, uint32_t n, uint32_t prim_off) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 v = tris[3 * (size_t)i];
+    v.w = __uint_as_float(__float_as_uint(v.w) + prim_off);
+    tris[3 * (size_t)i] = v;
+}
+// Shading records: material id += mat_off, light id (if >= 0) += light_off.
+__global__ void k_rebase_stris(float4* st, uint32_t n, uint32_t mat_off, uint32_t light_off) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float4 a = st[3 * (size_t)i], b = st[3 * (size_t)i + 1];
+    a.w = __uint_as_float(__float_as_uint(a.w) + mat_off);
+    const int lid = __float_as_int(b.w);
+    if (lid >= 0) b.w = __int_as_float(lid + (int)light_off);
+    st[3 * (size_t)i] = a;
+    st[3 * (size_t)i + 1] = b;
+}
+__global__ void k_rebase_lights(LightRec* l, uint32_t n, uint32_t lt_off, uint32_t cdf_off, uint32_t* lt,
+                                uint32_t n_lt, uint32_t prim_off) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        l[i].first += lt_off;
+        l[i].cdf_first += cdf_off;
+    }
+    if (i < n_lt) lt[i] += prim_off;
+}
+
 __global__ void k_write_radiance(RenderArgs a, float* out) {
     const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
     if (slot >= a.n_paths) return;
@@ -235,7 +376,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
     const uint32_t* __restrict__ qin = a.ps.qmap[cur];
     float4* __restrict__ rout = a.ps.rays[cur ^ 1];
     uint32_t* __restrict__ qout = a.ps.qmap[cur ^ 1];
-    const float inv_nl = a.n_lights ? 1.0f / (float)a.n_lights : 0.0f;
+    const float inv_nl_all = a.n_lights ? 1.0f / (float)a.n_lights : 0.0f;
     constexpr int kDraws = kGuided ? 12 : 8;
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t q = it * total + gid;
@@ -259,6 +400,9 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
         f3 bs_wi = beta, bs_f = beta;
         float bs_pdf = 0.0f;
         bool use_guide = false;
+        uint32_t max_depth = a.max_depth, rr_depth = a.rr_depth, n_lights = a.n_lights, light_first = 0,
+                 root = 0;
+        float inv_nl = inv_nl_all;
         if (active) {
             slot = qin[q];
             const float4 hit = a.ps.hits[q];
@@ -269,7 +413,17 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
             const float prev_pdf = st0.w;
             L = mk(st1.x, st1.y, st1.z);
             const uint32_t prim = __float_as_uint(hit.y);
-            const uint32_t pixel = a.seq_base + slot / a.k;
+            uint32_t pixel = a.seq_base + slot / a.k;
+            if (a.scenes) {
+                const SceneRec& sc = a.scenes[a.ps.slot_scene[slot]];
+                pixel = a.ps.slot_pixel[slot];
+                max_depth = sc.max_depth;
+                rr_depth = sc.rr_depth;
+                n_lights = sc.n_lights;
+                light_first = sc.light_first;
+                root = sc.root;
+                inv_nl = n_lights ? 1.0f / (float)n_lights : 0.0f;
+            }
             rng.state = a.ps.rng[slot];
             rng.inc = ((uint64_t)pixel << 1u) | 1u;
             if constexpr (kGuided) {
@@ -303,7 +457,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                     }
                     L = L + mul(beta, Le) * w;
                 }
-                if (depth <= a.max_depth) {
+                if (depth <= max_depth) {
 #pragma unroll
                     for (int i = 0; i < kDraws; ++i) u[i] = rng.nextf();
                     const f3 wo_w = -dir;
@@ -316,10 +470,10 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                         recorded = recorded && eligible;
                     }
                     // next-event estimation (GeometryLight::sample, light.cpp:52-77)
-                    if (a.n_lights > 0) {
-                        uint32_t li = (uint32_t)(u[0] * (float)a.n_lights);
-                        li = li < a.n_lights - 1 ? li : a.n_lights - 1;
-                        const LightRec lr = a.lights[li];
+                    if (n_lights > 0) {
+                        uint32_t li = (uint32_t)(u[0] * (float)n_lights);
+                        li = li < n_lights - 1 ? li : n_lights - 1;
+                        const LightRec lr = a.lights[light_first + li];
                         const uint32_t lt = a.light_tris[lr.first + cdf_sample(a.light_cdf + lr.cdf_first, lr.count, u[1])];
                         const f3 la = ld3(a.stris[3 * (size_t)lt]), lb = ld3(a.stris[3 * (size_t)lt + 1]),
                                  lc = ld3(a.stris[3 * (size_t)lt + 2]);
@@ -354,7 +508,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                     beta_pre = beta;
                     alive = true;
                     // Russian roulette (integrator.cpp:491-497)
-                    if (depth >= a.rr_depth) {
+                    if (depth >= rr_depth) {
                         const float eta_rr = 0.95f;
                         const float qs = fminf(maxc(beta) * eta_rr * eta_rr, 0.99f);
                         if (!(u[4] < qs)) alive = false;
@@ -459,12 +613,14 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
             rout[2 * (size_t)qo] = nr0;
             rout[2 * (size_t)qo + 1] = nr1;
             qout[qo] = slot;
+            if (a.scenes) a.ps.qroot[cur ^ 1][qo] = root;
         }
         const uint32_t so = wave_append(shad, &a.ps.cnt[2]);
         if (shad) {
             a.ps.srays[2 * (size_t)so] = sr0;
             a.ps.srays[2 * (size_t)so + 1] = sr1;
             a.ps.scontrib[so] = sc;
+            if (a.scenes) a.ps.sroot[so] = root;
         }
     }
 }
@@ -559,7 +715,7 @@ struct RenderState {
     kp::LightRec* d_lights = nullptr;
     uint32_t* d_light_tris = nullptr;
     float* d_light_cdf = nullptr;
-    uint32_t n_lights = 0;
+    uint32_t n_lights = 0, n_light_tris = 0, n_cdf = 0;
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
     // paths
     uint32_t cap_paths = 0;
@@ -688,6 +844,8 @@ static int prepare_scene(kp_ctx* c) {
     KP_HIP_TRY(c, hipMalloc((void**)&r->d_mats, r->mats.size() * sizeof(Material)));
     KP_HIP_TRY(c, hipMemcpy(r->d_mats, r->mats.data(), r->mats.size() * sizeof(Material), hipMemcpyHostToDevice));
     r->n_lights = (uint32_t)lights.size();
+    r->n_light_tris = (uint32_t)light_tris.size();
+    r->n_cdf = (uint32_t)cdf.size();
     if (!lights.empty()) {
         KP_HIP_TRY(c, hipMalloc((void**)&r->d_lights, lights.size() * sizeof(LightRec)));
         KP_HIP_TRY(c, hipMemcpy(r->d_lights, lights.data(), lights.size() * sizeof(LightRec), hipMemcpyHostToDevice));
@@ -700,15 +858,16 @@ static int prepare_scene(kp_ctx* c) {
     return KP_OK;
 }
 
-static int ensure_paths(kp_ctx* c, uint32_t n) {
-    RenderState* r = rstate(c);
-    if (r->cap_paths >= n && r->path_mem) return KP_OK;
+// Grows a path-state allocation to at least n slots.
+static int alloc_paths(kp_ctx* c, PathState& ps, void*& mem, uint32_t& cap, uint32_t n) {
+    if (cap >= n && mem) return KP_OK;
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (r->path_mem) (void)hipFree(r->path_mem);
-    r->path_mem = nullptr;
-    r->cap_paths = 0;
+    if (mem) (void)hipFree(mem);
+    mem = nullptr;
+    cap = 0;
     // bytes per slot: 2x32 rays + 2x4 qmap + 16 hit + 16 s0 + 16 s1 + 8 rng
     //                 + 32 shadow ray + 16 contrib + 4 occl
+    //                 + 5x4 batch (scene, pixel, roots of the three queues)
     const size_t N = n;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
@@ -716,9 +875,10 @@ static int ensure_paths(kp_ctx* c, uint32_t n) {
     const size_t o_h = take(N * 16), o_s0 = take(N * 16), o_s1 = take(N * 16), o_rng = take(N * 8);
     const size_t o_sr = take(N * 32), o_sc = take(N * 16), o_so = take(N * 4), o_cnt = take(64),
                  o_tot = take(64);
-    KP_HIP_TRY(c, hipMalloc(&r->path_mem, off));
-    char* b = (char*)r->path_mem;
-    PathState& ps = r->ps;
+    const size_t o_ss = take(N * 4), o_sp = take(N * 4), o_qr0 = take(N * 4), o_qr1 = take(N * 4),
+                 o_sr2 = take(N * 4);
+    KP_HIP_TRY(c, hipMalloc(&mem, off));
+    char* b = (char*)mem;
     ps.rays[0] = (float4*)(b + o_r0);
     ps.rays[1] = (float4*)(b + o_r1);
     ps.qmap[0] = (uint32_t*)(b + o_q0);
@@ -732,10 +892,20 @@ static int ensure_paths(kp_ctx* c, uint32_t n) {
     ps.socc = (uint32_t*)(b + o_so);
     ps.cnt = (uint32_t*)(b + o_cnt);
     ps.totals = (unsigned long long*)(b + o_tot);
+    ps.slot_scene = (uint32_t*)(b + o_ss);
+    ps.slot_pixel = (uint32_t*)(b + o_sp);
+    ps.qroot[0] = (uint32_t*)(b + o_qr0);
+    ps.qroot[1] = (uint32_t*)(b + o_qr1);
+    ps.sroot = (uint32_t*)(b + o_sr2);
     KP_HIP_TRY(c, hipMemset(ps.cnt, 0, 64));
     KP_HIP_TRY(c, hipMemset(ps.totals, 0, 64));
-    r->cap_paths = n;
+    cap = n;
     return KP_OK;
+}
+
+static int ensure_paths(kp_ctx* c, uint32_t n) {
+    RenderState* r = rstate(c);
+    return alloc_paths(c, r->ps, r->path_mem, r->cap_paths, n);
 }
 
 static int film_alloc(kp_ctx* c, uint32_t w, uint32_t h) {
@@ -792,42 +962,60 @@ static uint32_t samples_per_pass(uint32_t npix, uint32_t spp, uint64_t target) {
     return best;
 }
 
+// Traversal and launch-shape settings of the integrator's launches.
+struct TraceCfg {
+    TraceArgs base;            // BVH, work counter, error word, spill, tuning
+    uint32_t grid_closest, grid_occluded, shade_blocks;
+    bool roots;                // per-ray roots from the path state (batched scenes)
+};
+
+static TraceCfg ctx_trace_cfg(kp_ctx* c) {
+    TraceCfg t;
+    std::memset(&t, 0, sizeof(t));
+    t.base.nodes = c->d_nodes;
+    t.base.tris = c->d_tris;
+    t.base.counter = c->d_words;
+    t.base.err = c->d_words + 1;
+    t.base.spill = c->d_spill;
+    t.base.spill_cap = c->spill_cap;
+    t.base.refill_below = c->refill_below;
+    t.base.max_steps = c->max_steps;
+    t.base.counters = c->d_counters;
+    t.grid_closest = c->grid_closest;
+    t.grid_occluded = c->grid_occluded;
+    t.shade_blocks = (uint32_t)c->cus * 8u;
+    t.roots = false;
+    return t;
+}
+
 // Extend / shade / connect for max_depth + 1 bounces of the paths that the
-// generate kernel has queued (queue sizes live on the device).
-static int run_bounces(kp_ctx* c, const RenderArgs& a, bool guided) {
-    const uint32_t shade_blocks = (uint32_t)c->cus * 8u;
+// generate kernel has queued (queue sizes live on the device).  Only kernel
+// launches: the sequence can be captured into a graph.
+static int run_bounces(kp_ctx* c, const RenderArgs& a, bool guided, const TraceCfg& tc, uint32_t max_depth) {
     int cur = 0;
-    for (uint32_t depth = 1; depth <= a.max_depth + 1; ++depth) {
-        TraceArgs ta;
-        std::memset(&ta, 0, sizeof(ta));
-        ta.nodes = c->d_nodes;
-        ta.tris = c->d_tris;
-        ta.counter = c->d_words;
-        ta.err = c->d_words + 1;
-        ta.spill = c->d_spill;
-        ta.spill_cap = c->spill_cap;
-        ta.refill_below = c->refill_below;
-        ta.max_steps = c->max_steps;
-        ta.counters = c->d_counters;
+    for (uint32_t depth = 1; depth <= max_depth + 1; ++depth) {
+        TraceArgs ta = tc.base;
         ta.rays = a.ps.rays[cur];
         ta.hits = a.ps.hits;
         ta.n = a.n_paths;
         ta.n_dev = &a.ps.cnt[cur];
-        KP_HIP_TRY(c, launch_closest(ta, c->grid_closest, false, c->stream));
+        ta.roots = tc.roots ? a.ps.qroot[cur] : nullptr;
+        KP_HIP_TRY(c, launch_closest(ta, tc.grid_closest, false, c->stream));
         hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, c->stream, a, cur);
         if (guided)
-            hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
+            hipLaunchKernelGGL(k_shade<true>, dim3(tc.shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
         else
-            hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
+            hipLaunchKernelGGL(k_shade<false>, dim3(tc.shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
         KP_HIP_TRY(c, hipGetLastError());
-        if (depth <= a.max_depth) {
+        if (depth <= max_depth) {
             ta.rays = a.ps.srays;
             ta.hits = nullptr;
             ta.occl = a.ps.socc;
             ta.n_dev = &a.ps.cnt[2];
-            KP_HIP_TRY(c, launch_occluded(ta, c->grid_occluded, c->stream));
+            ta.roots = tc.roots ? a.ps.sroot : nullptr;
+            KP_HIP_TRY(c, launch_occluded(ta, tc.grid_occluded, c->stream));
             hipLaunchKernelGGL(k_count_shadow, dim3(1), dim3(64), 0, c->stream, a);
-            hipLaunchKernelGGL(k_connect, dim3(shade_blocks), dim3(256), 0, c->stream, a);
+            hipLaunchKernelGGL(k_connect, dim3(tc.shade_blocks), dim3(256), 0, c->stream, a);
             KP_HIP_TRY(c, hipGetLastError());
         }
         cur ^= 1;
@@ -911,7 +1099,7 @@ static int integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_re
         }
         hipLaunchKernelGGL(k_generate_rays, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a, src);
         KP_HIP_TRY(c, hipGetLastError());
-        if ((rc = run_bounces(c, a, guided))) return rc;
+        if ((rc = run_bounces(c, a, guided, ctx_trace_cfg(c), a.max_depth))) return rc;
         hipLaunchKernelGGL(k_write_radiance, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a, dst);
         KP_HIP_TRY(c, hipGetLastError());
         if (kind == KP_PTR_HOST)
@@ -1001,7 +1189,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
         }
         hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, a);
         KP_HIP_TRY(c, hipGetLastError());
-        if ((rc = run_bounces(c, a, guided))) return rc;
+        if ((rc = run_bounces(c, a, guided, ctx_trace_cfg(c), a.max_depth))) return rc;
         hipLaunchKernelGGL(k_film, dim3((npix + 255) / 256), dim3(256), 0, c->stream, a, r->film_sum,
                            r->film_comp);
         KP_HIP_TRY(c, hipGetLastError());
@@ -1031,6 +1219,400 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     r->stats.guided_passes = guided_passes;
     r->stats.train_records = 0;
     if (train) KP_HIP_TRY(c, hipMemcpy(&r->stats.train_records, g->rec_valid, 4, hipMemcpyDeviceToHost));
+    return KP_OK;
+}
+
+// ============================================================== batches ==
+// Many scenes rendered as one workload: their BVHs, shading records,
+// materials and lights are copied into merged arrays (indices rebased), every
+// ray carries its scene's BVH root, and each pass runs the paths of a window
+// of the batch's work items whatever scene they belong to.
+struct BatchState {
+    kp_ctx* c = nullptr;
+    uint4* nodes = nullptr;
+    float4* tris = nullptr;
+    float4* stris = nullptr;
+    Material* mats = nullptr;
+    LightRec* lights = nullptr;
+    uint32_t* light_tris = nullptr;
+    float* light_cdf = nullptr;
+    uint32_t n_nodes = 0, n_tris = 0, bvh_depth = 0;
+    std::vector<uint32_t> root, light_first, n_lights;
+    uint2* spill = nullptr;
+    uint32_t spill_cap = 0, max_steps = 0;
+    // films of the last render
+    std::vector<uint32_t> fw, fh, film_first, film_spp;
+    float4* film_sum = nullptr;
+    float4* film_comp = nullptr;
+    float* film_out = nullptr;
+    uint32_t film_cap = 0;
+    // paths and the pass table
+    PathState ps{};
+    void* path_mem = nullptr;
+    uint32_t cap_paths = 0;
+    SceneRec* d_scenes = nullptr;
+    uint32_t scenes_cap = 0;
+    unsigned long long* d_pass = nullptr;
+    std::vector<SceneRec> h_scenes;
+    kp_batch_tuning tune{};
+    // captured pass graph and what it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    kp_render_stats stats{};
+};
+
+static void batch_drop_graph(BatchState* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_key.clear();
+}
+
+static void batch_free(BatchState* b) {
+    batch_drop_graph(b);
+    for (void* p : {(void*)b->nodes, (void*)b->tris, (void*)b->stris, (void*)b->mats, (void*)b->lights,
+                    (void*)b->light_tris, (void*)b->light_cdf, (void*)b->spill, (void*)b->film_sum,
+                    (void*)b->film_comp, (void*)b->film_out, b->path_mem, (void*)b->d_scenes, (void*)b->d_pass})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState** out) {
+    uint64_t nn = 0, nt = 0, nm = 0, nl = 0, nlt = 0, ncdf = 0;
+    uint32_t depth = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        kp_ctx* s = scenes[i];
+        if (!s) return fail(c, KP_ERR_INVALID, "kp_batch_create: NULL scene context");
+        if (s->device != c->device) return fail(c, KP_ERR_INVALID, "kp_batch_create: scenes on another device");
+        if (!s->built) return fail(c, KP_ERR_STATE, "kp_batch_create: scene without kp_bvh_build");
+        KP_HIP_TRY(c, hipSetDevice(s->device));
+        int rc = prepare_scene(s);
+        if (rc) return fail(c, rc, std::string("kp_batch_create: scene ") + std::to_string(i) + ": " + s->err);
+        KP_HIP_TRY(c, hipStreamSynchronize(s->stream));
+        const RenderState* r = s->render;
+        nn += s->n_nodes;
+        nt += s->n_tris_built;
+        nm += r->mats.size();
+        nl += r->n_lights;
+        nlt += r->n_light_tris;
+        ncdf += r->n_cdf;
+        depth = std::max(depth, s->max_depth);
+    }
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    if (nt >= (1ull << 28) || nn >= (1ull << 31))
+        return fail(c, KP_ERR_INVALID, "kp_batch_create: more than 2^28 triangles in the batch");
+    BatchState* b = new BatchState();
+    b->c = c;
+    auto bail = [&](hipError_t e, const char* what) {
+        batch_free(b);
+        return hip_fail(c, e, what);
+    };
+    hipError_t e = hipSuccess;
+    auto alloc = [&](void** p, uint64_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes ? bytes : 16);
+    };
+    alloc((void**)&b->nodes, nn * KP_NODE_BYTES);
+    alloc((void**)&b->tris, nt * KP_TRI_BYTES);
+    alloc((void**)&b->stris, nt * 3 * sizeof(float4));
+    alloc((void**)&b->mats, nm * sizeof(Material));
+    alloc((void**)&b->lights, nl * sizeof(LightRec));
+    alloc((void**)&b->light_tris, nlt * 4);
+    alloc((void**)&b->light_cdf, ncdf * 4);
+    alloc((void**)&b->d_pass, 64);
+    if (e != hipSuccess) return bail(e, "kp_batch_create: allocation");
+    uint32_t on = 0, ot = 0, om = 0, ol = 0, olt = 0, ocdf = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        kp_ctx* s = scenes[i];
+        const RenderState* r = s->render;
+        const uint32_t sn = s->n_nodes, st = s->n_tris_built, sm = (uint32_t)r->mats.size();
+        e = hipMemcpy(b->nodes + 8 * (size_t)on, s->d_nodes, (size_t)sn * KP_NODE_BYTES, hipMemcpyDeviceToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->tris + 3 * (size_t)ot, s->d_tris, (size_t)st * KP_TRI_BYTES, hipMemcpyDeviceToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->stris + 3 * (size_t)ot, r->stris, (size_t)st * 3 * sizeof(float4),
+                          hipMemcpyDeviceToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->mats + om, r->d_mats, (size_t)sm * sizeof(Material), hipMemcpyDeviceToDevice);
+        if (e == hipSuccess && r->n_lights) {
+            e = hipMemcpy(b->lights + ol, r->d_lights, (size_t)r->n_lights * sizeof(LightRec),
+                          hipMemcpyDeviceToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(b->light_tris + olt, r->d_light_tris, (size_t)r->n_light_tris * 4,
+                              hipMemcpyDeviceToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(b->light_cdf + ocdf, r->d_light_cdf, (size_t)r->n_cdf * 4, hipMemcpyDeviceToDevice);
+        }
+        if (e != hipSuccess) return bail(e, "kp_batch_create: copy");
+        hipLaunchKernelGGL(k_rebase_nodes, dim3((sn + 255) / 256), dim3(256), 0, c->stream, b->nodes + 8 * (size_t)on,
+                           sn, on, ot);
+        hipLaunchKernelGGL(k_rebase_tris, dim3((st + 255) / 256), dim3(256), 0, c->stream, b->tris + 3 * (size_t)ot,
+                           st, ot);
+        hipLaunchKernelGGL(k_rebase_stris, dim3((st + 255) / 256), dim3(256), 0, c->stream,
+                           b->stris + 3 * (size_t)ot, st, om, ol);
+        const uint32_t nmax = std::max(r->n_lights, r->n_light_tris);
+        if (nmax)
+            hipLaunchKernelGGL(k_rebase_lights, dim3((nmax + 255) / 256), dim3(256), 0, c->stream, b->lights + ol,
+                               r->n_lights, olt, ocdf, b->light_tris + olt, r->n_light_tris, ot);
+        if ((e = hipGetLastError()) != hipSuccess) return bail(e, "kp_batch_create: rebase");
+        b->root.push_back(on);
+        b->light_first.push_back(ol);
+        b->n_lights.push_back(r->n_lights);
+        on += sn;
+        ot += st;
+        om += sm;
+        ol += r->n_lights;
+        olt += r->n_light_tris;
+        ocdf += r->n_cdf;
+    }
+    b->n_nodes = on;
+    b->n_tris = ot;
+    b->bvh_depth = depth;
+    b->spill_cap = 7u * depth + 8u;
+    b->max_steps = 2u * on + ot + 64u;
+    const uint32_t grid = std::max(c->grid_closest, c->grid_occluded);
+    e = hipMalloc((void**)&b->spill, (size_t)grid * KP_TRACE_BLOCK * b->spill_cap * sizeof(uint2));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return bail(e, "kp_batch_create: spill / rebase");
+    b->tune.pass_paths = 1u << 21;
+    b->tune.shade_blocks_per_cu = 8;
+    b->tune.trace_blocks_per_cu = c->grid_closest / (uint32_t)c->cus;
+    b->tune.refill_below = c->refill_below;
+    b->tune.use_graph = 1;
+    *out = b;
+    return KP_OK;
+}
+
+// The launches of one pass: start paths, bounce, fold into the films, advance.
+static int batch_enqueue_pass(BatchState* b, const RenderArgs& a, const TraceCfg& tc, uint32_t max_depth) {
+    kp_ctx* c = b->c;
+    const uint32_t blocks = (a.n_paths + 255) / 256;
+    hipLaunchKernelGGL(k_generate_batch, dim3(blocks), dim3(256), 0, c->stream, a);
+    KP_HIP_TRY(c, hipGetLastError());
+    int rc = run_bounces(c, a, false, tc, max_depth);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_film_batch, dim3(blocks), dim3(256), 0, c->stream, a, b->film_sum, b->film_comp);
+    hipLaunchKernelGGL(k_pass_advance, dim3(1), dim3(64), 0, c->stream, b->d_pass, a.n_paths);
+    KP_HIP_TRY(c, hipGetLastError());
+    return KP_OK;
+}
+
+// Renders desc[i].spp samples of every scene into freshly zeroed films; with
+// max_passes > 0 stops after that many passes (autotuning).  Returns the
+// number of passes run in *passes_run.
+static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max_passes, uint32_t* passes_run) {
+    kp_ctx* c = b->c;
+    const uint32_t n = (uint32_t)b->root.size();
+    uint32_t max_depth = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const kp_render_desc& d = descs[i];
+        if (d.width == 0 || d.height == 0 || (uint64_t)d.width * d.height > (1u << 26) || d.spp == 0)
+            return fail(c, KP_ERR_INVALID, "kp_batch_render: bad film size or spp");
+        if (d.max_depth == 0 || d.max_depth > 64) return fail(c, KP_ERR_INVALID, "kp_batch_render: max_depth must be 1..64");
+        if (d.flags & KP_RENDER_GUIDED) return fail(c, KP_ERR_INVALID, "kp_batch_render: guided batches are not supported");
+        max_depth = std::max(max_depth, d.max_depth);
+    }
+    // films
+    b->fw.assign(n, 0);
+    b->fh.assign(n, 0);
+    b->film_first.assign(n, 0);
+    b->film_spp.assign(n, 0);
+    uint64_t npix_all = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        b->fw[i] = descs[i].width;
+        b->fh[i] = descs[i].height;
+        b->film_spp[i] = descs[i].spp;
+        b->film_first[i] = (uint32_t)npix_all;
+        npix_all += (uint64_t)descs[i].width * descs[i].height;
+    }
+    if (npix_all > (1ull << 30)) return fail(c, KP_ERR_INVALID, "kp_batch_render: films too large");
+    if (npix_all > b->film_cap) {
+        KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (void* p : {(void*)b->film_sum, (void*)b->film_comp, (void*)b->film_out})
+            if (p) (void)hipFree(p);
+        b->film_sum = b->film_comp = nullptr;
+        b->film_out = nullptr;
+        b->film_cap = 0;
+        KP_HIP_TRY(c, hipMalloc((void**)&b->film_sum, npix_all * sizeof(float4)));
+        KP_HIP_TRY(c, hipMalloc((void**)&b->film_comp, npix_all * sizeof(float4)));
+        KP_HIP_TRY(c, hipMalloc((void**)&b->film_out, npix_all * 3 * sizeof(float)));
+        b->film_cap = (uint32_t)npix_all;
+        batch_drop_graph(b);
+    }
+    const uint32_t zb = (uint32_t)((npix_all + 255) / 256);
+    hipLaunchKernelGGL(k_zero_f4, dim3(zb), dim3(256), 0, c->stream, b->film_sum, (uint32_t)npix_all);
+    hipLaunchKernelGGL(k_zero_f4, dim3(zb), dim3(256), 0, c->stream, b->film_comp, (uint32_t)npix_all);
+    KP_HIP_TRY(c, hipGetLastError());
+    // scene table and work items
+    b->h_scenes.assign(n, SceneRec{});
+    unsigned long long work = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const kp_render_desc& d = descs[i];
+        RenderArgs cam{};
+        camera_basis(d, cam);
+        SceneRec& r = b->h_scenes[i];
+        r.eye = cam.eye;
+        r.fwd = cam.fwd;
+        r.right = cam.right;
+        r.upv = cam.upv;
+        r.tan_half = cam.tan_half;
+        r.aspect = cam.aspect;
+        r.width = d.width;
+        r.height = d.height;
+        r.spp = d.spp;
+        r.spp_offset = d.spp_offset;
+        r.seed = d.seed;
+        r.max_depth = d.max_depth;
+        r.rr_depth = d.rr_depth;
+        r.root = b->root[i];
+        r.light_first = b->light_first[i];
+        r.n_lights = b->n_lights[i];
+        r.film_first = b->film_first[i];
+        r.work_first = work;
+        work += (unsigned long long)d.width * d.height * d.spp;
+    }
+    if (n > b->scenes_cap) {
+        KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (b->d_scenes) (void)hipFree(b->d_scenes);
+        b->d_scenes = nullptr;
+        KP_HIP_TRY(c, hipMalloc((void**)&b->d_scenes, n * sizeof(SceneRec)));
+        b->scenes_cap = n;
+        batch_drop_graph(b);
+    }
+    // the table and pass words are read by kernels already queued from an
+    // earlier render: wait for them before overwriting
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    KP_HIP_TRY(c, hipMemcpy(b->d_scenes, b->h_scenes.data(), n * sizeof(SceneRec), hipMemcpyHostToDevice));
+    const unsigned long long pass_words[2] = {0ull, work};
+    KP_HIP_TRY(c, hipMemcpy(b->d_pass, pass_words, sizeof(pass_words), hipMemcpyHostToDevice));
+    const uint32_t chunk = b->tune.pass_paths;
+    const uint32_t old_cap = b->cap_paths;
+    void* old_mem = b->path_mem;
+    int rc = alloc_paths(c, b->ps, b->path_mem, b->cap_paths, chunk);
+    if (rc) return rc;
+    if (b->path_mem != old_mem || b->cap_paths != old_cap) batch_drop_graph(b);
+    unsigned long long zero2[2] = {0, 0};
+    KP_HIP_TRY(c, hipMemcpy(b->ps.totals, zero2, 16, hipMemcpyHostToDevice));
+    RenderArgs a{};
+    a.stris = b->stris;
+    a.mats = b->mats;
+    a.lights = b->lights;
+    a.light_tris = b->light_tris;
+    a.light_cdf = b->light_cdf;
+    a.n_lights = 0;
+    a.k = 1;
+    a.n_paths = chunk;
+    a.max_depth = max_depth;
+    a.rr_depth = 0;
+    a.ps = b->ps;
+    a.scenes = b->d_scenes;
+    a.n_scenes = n;
+    a.pass = b->d_pass;
+    TraceCfg tc;
+    std::memset(&tc, 0, sizeof(tc));
+    tc.base.nodes = b->nodes;
+    tc.base.tris = b->tris;
+    tc.base.counter = c->d_words;
+    tc.base.err = c->d_words + 1;
+    tc.base.spill = b->spill;
+    tc.base.spill_cap = b->spill_cap;
+    tc.base.refill_below = b->tune.refill_below;
+    tc.base.max_steps = b->max_steps;
+    tc.base.counters = c->d_counters;
+    const uint32_t tb_max = std::max(c->grid_closest, c->grid_occluded) / (uint32_t)c->cus;
+    const uint32_t tb = std::min(std::max(b->tune.trace_blocks_per_cu, 1u), tb_max);
+    tc.grid_closest = std::min((uint32_t)c->cus * tb, c->grid_closest);
+    tc.grid_occluded = std::min((uint32_t)c->cus * tb, c->grid_occluded);
+    tc.shade_blocks = (uint32_t)c->cus * std::max(b->tune.shade_blocks_per_cu, 1u);
+    tc.roots = true;
+    uint64_t passes = (work + chunk - 1) / chunk;
+    if (max_passes && passes > max_passes) passes = max_passes;
+    auto t0 = std::chrono::steady_clock::now();
+    if (b->tune.use_graph) {
+        const std::vector<uint64_t> key = {chunk, n, max_depth, tc.grid_closest, tc.grid_occluded, tc.shade_blocks,
+                                           tc.base.refill_below, (uint64_t)(uintptr_t)b->path_mem,
+                                           (uint64_t)(uintptr_t)b->film_sum, (uint64_t)(uintptr_t)b->d_scenes};
+        if (!b->exec || b->graph_key != key) {
+            batch_drop_graph(b);
+            KP_HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            rc = batch_enqueue_pass(b, a, tc, max_depth);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess) return hip_fail(c, ec, "kp_batch_render: pass graph capture");
+            b->graph = g;
+            const hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                batch_drop_graph(b);
+                return hip_fail(c, ei, "kp_batch_render: pass graph instantiation");
+            }
+            b->graph_key = key;
+        }
+        for (uint64_t p = 0; p < passes; ++p) KP_HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
+    } else {
+        for (uint64_t p = 0; p < passes; ++p)
+            if ((rc = batch_enqueue_pass(b, a, tc, max_depth))) return rc;
+    }
+    if ((rc = kp_synchronize(c))) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    unsigned long long tot[2];
+    KP_HIP_TRY(c, hipMemcpy(tot, b->ps.totals, 16, hipMemcpyDeviceToHost));
+    b->stats = kp_render_stats{};
+    b->stats.paths = std::min<unsigned long long>(work, (unsigned long long)passes * chunk);
+    b->stats.extension_rays = tot[0];
+    b->stats.shadow_rays = tot[1];
+    b->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    b->stats.passes = (uint32_t)passes;
+    b->stats.paths_per_pass = chunk;
+    if (passes_run) *passes_run = (uint32_t)passes;
+    return KP_OK;
+}
+
+// Coordinate descent over the launch settings, one setting at a time, each
+// candidate timed over `passes` passes of the batch's own work (seconds per
+// path, the first pass of each candidate is a warm-up).
+static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t passes, kp_batch_tuning* out) {
+    kp_ctx* c = b->c;
+    const uint32_t tb_max = std::max(c->grid_closest, c->grid_occluded) / (uint32_t)c->cus;
+    auto cost = [&](const kp_batch_tuning& t, double* sec_per_path) -> int {
+        b->tune = t;
+        uint32_t run = 0;
+        int rc = batch_render(b, descs, 1, &run);   // warm-up: allocation, graph capture
+        if (rc) return rc;
+        if ((rc = batch_render(b, descs, passes, &run))) return rc;
+        *sec_per_path = b->stats.seconds / (double)std::max<uint64_t>(1, b->stats.paths);
+        return KP_OK;
+    };
+    kp_batch_tuning best = b->tune;
+    double best_cost = 0.0;
+    int rc = cost(best, &best_cost);
+    if (rc) return rc;
+    auto try_values = [&](uint32_t kp_batch_tuning::*field, std::initializer_list<uint32_t> values) -> int {
+        for (uint32_t v : values) {
+            if (v == best.*field) continue;
+            kp_batch_tuning t = best;
+            t.*field = v;
+            double cst = 0.0;
+            int r2 = cost(t, &cst);
+            if (r2) return r2;
+            if (cst < best_cost) {
+                best_cost = cst;
+                best = t;
+            }
+        }
+        return KP_OK;
+    };
+    if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 19, 1u << 20, 1u << 21, 1u << 22}))) return rc;
+    if ((rc = try_values(&kp_batch_tuning::trace_blocks_per_cu, {tb_max, std::max(1u, tb_max / 2), std::max(1u, tb_max - 1)})))
+        return rc;
+    if ((rc = try_values(&kp_batch_tuning::refill_below, {16u, 32u, 48u, 56u}))) return rc;
+    if ((rc = try_values(&kp_batch_tuning::shade_blocks_per_cu, {4u, 8u, 16u}))) return rc;
+    b->tune = best;
+    if (out) *out = best;
     return KP_OK;
 }
 
@@ -1116,6 +1698,82 @@ int kp_integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_render
 int kp_render_get_stats(kp_ctx* c, kp_render_stats* s) {
     if (!c || !s) return KP_ERR_INVALID;
     *s = kp::rstate(c)->stats;
+    return KP_OK;
+}
+
+
+int kp_batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, kp_batch** out) {
+    if (!c || !out || (n && !scenes)) return KP_ERR_INVALID;
+    *out = nullptr;
+    if (n == 0 || n > 4096) return kp::fail(c, KP_ERR_INVALID, "kp_batch_create: 1..4096 scenes");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    kp::BatchState* b = nullptr;
+    int rc = kp::batch_create(c, scenes, n, &b);
+    if (rc) return rc;
+    *out = reinterpret_cast<kp_batch*>(b);
+    return KP_OK;
+}
+
+int kp_batch_destroy(kp_batch* h) {
+    if (!h) return KP_ERR_INVALID;
+    kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
+    (void)hipSetDevice(b->c->device);
+    (void)hipStreamSynchronize(b->c->stream);
+    kp::batch_free(b);
+    return KP_OK;
+}
+
+int kp_batch_set_tuning(kp_batch* h, const kp_batch_tuning* t) {
+    if (!h || !t) return KP_ERR_INVALID;
+    kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
+    if (t->pass_paths < 256 || t->pass_paths > (1u << 24) || t->shade_blocks_per_cu == 0 ||
+        t->shade_blocks_per_cu > 64 || t->trace_blocks_per_cu == 0 || t->refill_below == 0 || t->refill_below > 64)
+        return kp::fail(b->c, KP_ERR_INVALID, "kp_batch_set_tuning: value out of range");
+    b->tune = *t;
+    return KP_OK;
+}
+
+int kp_batch_get_tuning(kp_batch* h, kp_batch_tuning* t) {
+    if (!h || !t) return KP_ERR_INVALID;
+    *t = reinterpret_cast<kp::BatchState*>(h)->tune;
+    return KP_OK;
+}
+
+int kp_batch_render(kp_batch* h, const kp_render_desc* descs) {
+    if (!h || !descs) return KP_ERR_INVALID;
+    kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
+    KP_HIP_TRY(b->c, hipSetDevice(b->c->device));
+    return kp::batch_render(b, descs, 0, nullptr);
+}
+
+int kp_batch_autotune(kp_batch* h, const kp_render_desc* descs, uint32_t passes, kp_batch_tuning* out) {
+    if (!h || !descs) return KP_ERR_INVALID;
+    kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
+    KP_HIP_TRY(b->c, hipSetDevice(b->c->device));
+    return kp::batch_autotune(b, descs, passes ? passes : 2, out);
+}
+
+int kp_batch_film_read(kp_batch* h, uint32_t scene, float* out, int kind) {
+    if (!h || !out) return KP_ERR_INVALID;
+    kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
+    kp_ctx* c = b->c;
+    if (scene >= b->fw.size()) return kp::fail(c, KP_ERR_INVALID, "kp_batch_film_read: no such scene in the last render");
+    if (kind != KP_PTR_HOST && kind != KP_PTR_DEVICE) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t npix = b->fw[scene] * b->fh[scene], f0 = b->film_first[scene];
+    const float inv = 1.0f / (float)b->film_spp[scene];
+    float* dst = kind == KP_PTR_DEVICE ? out : b->film_out + 3 * (size_t)f0;
+    hipLaunchKernelGGL(kp::k_film_resolve, dim3((npix + 255) / 256), dim3(256), 0, c->stream, b->film_sum + f0,
+                       b->film_comp + f0, npix, inv, dst);
+    KP_HIP_TRY(c, hipGetLastError());
+    if (kind == KP_PTR_HOST)
+        KP_HIP_TRY(c, hipMemcpyAsync(out, dst, (size_t)npix * 12, hipMemcpyDeviceToHost, c->stream));
+    return kp_synchronize(c);
+}
+
+int kp_batch_get_stats(kp_batch* h, kp_render_stats* s) {
+    if (!h || !s) return KP_ERR_INVALID;
+    *s = reinterpret_cast<kp::BatchState*>(h)->stats;
     return KP_OK;
 }
 

@@ -222,13 +222,16 @@ struct RayFeeder {
     bool drained;
     uint32_t* counter;
     const float4* rays;
+    const uint32_t* roots;
     bool pending;       // this lane holds a prefetched ray
-    uint32_t pid;
+    uint32_t pid, proot;
     float4 p0, p1;
 
-    __device__ __forceinline__ void init(uint32_t* ctr, const float4* rs, uint32_t count) {
+    __device__ __forceinline__ void init(uint32_t* ctr, const float4* rs, const uint32_t* rts, uint32_t count) {
         counter = ctr;
         rays = rs;
+        roots = rts;
+        proot = 0u;
         n = count;
         next = end = 0;
         drained = false;
@@ -258,6 +261,7 @@ struct RayFeeder {
             pid = next + prefix;
             p0 = rays[2 * (size_t)pid];
             p1 = rays[2 * (size_t)pid + 1];
+            if (roots) proot = roots[pid];
             pending = true;
         }
         next += want < avail ? want : avail;
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
     uint64_t cnt_nodes = 0, cnt_tris = 0;
     (void)lane;
     RayFeeder feed;
-    feed.init(args.counter, args.rays, n_rays);
+    feed.init(args.counter, args.rays, args.roots, n_rays);
 
     for (;;) {
         if (ray_id == KP_INVALID && feed.pending) {
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
             best_t = r.tmax;
             best_p = KP_INVALID;
             bu = bv = 0.0f;
-            cur = ray_valid(feed.p0, feed.p1) ? 0u : KP_INVALID;   // root
+            cur = ray_valid(feed.p0, feed.p1) ? feed.proot : KP_INVALID;   // root
             st.sp = 0;
             steps = 0;
             feed.pending = false;
@@ -392,12 +396,12 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
     uint32_t cur = KP_INVALID, steps = 0;
     (void)lane;
     RayFeeder feed;
-    feed.init(args.counter, args.rays, n_rays);
+    feed.init(args.counter, args.rays, args.roots, n_rays);
     for (;;) {
         if (ray_id == KP_INVALID && feed.pending) {
             ray_id = feed.pid;
             setup_ray(feed.p0, feed.p1, r);
-            cur = ray_valid(feed.p0, feed.p1) ? 0u : KP_INVALID;
+            cur = ray_valid(feed.p0, feed.p1) ? feed.proot : KP_INVALID;
             st.sp = 0;
             steps = 0;
             feed.pending = false;

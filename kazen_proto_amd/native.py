@@ -68,6 +68,14 @@ class RenderStats(ctypes.Structure):
 RENDER_GUIDED = 1
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("pass_paths", _u32), ("shade_blocks_per_cu", _u32), ("trace_blocks_per_cu", _u32),
+                ("refill_below", _u32), ("use_graph", _u32), ("reserved", _u32 * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class GuidingDesc(ctypes.Structure):
     _fields_ = [("bsdf_fraction", ctypes.c_float), ("min_alpha", ctypes.c_float),
                 ("lr", ctypes.c_float), ("train_steps", _u32), ("guide_after", _u32),
@@ -142,6 +150,14 @@ SIGNATURES = {
     "kp_guiding_sample": (_i32, [_p, _p, _p, _u32, _p, _p]),
     "kp_guiding_train_step": (_i32, [_p, _p, _p, _p, _u32, ctypes.c_float, _i32, _p]),
     "kp_guiding_configure": (_i32, [_p, _p]),
+    "kp_batch_create": (_i32, [_p, _p, _u32, ctypes.POINTER(_p)]),
+    "kp_batch_destroy": (_i32, [_p]),
+    "kp_batch_set_tuning": (_i32, [_p, _p]),
+    "kp_batch_get_tuning": (_i32, [_p, _p]),
+    "kp_batch_render": (_i32, [_p, _p]),
+    "kp_batch_autotune": (_i32, [_p, _p, _u32, _p]),
+    "kp_batch_film_read": (_i32, [_p, _u32, _p, _i32]),
+    "kp_batch_get_stats": (_i32, [_p, _p]),
     "kp_device_alloc": (_i32, [_p, _u64, ctypes.POINTER(_p)]),
     "kp_device_free": (_i32, [_p, _p]),
     "kp_copy_to_device": (_i32, [_p, _p, _p, _u64]),
@@ -464,3 +480,63 @@ def hit_fields(hits):
     """Split an [n,4] float32 hit array into (t, prim uint32, u, v) numpy views."""
     h = np.asarray(hits)
     return h[:, 0], h[:, 1].view(np.uint32), h[:, 2], h[:, 3]
+
+
+class Batch:
+    """Many scenes rendered as one workload (kp_batch_*).  `owner` is a
+    Context; `scenes` are Contexts with loaded scenes (same device).  The
+    batch copies them, so they may be closed afterwards."""
+
+    def __init__(self, owner, scenes):
+        self._owner = owner
+        arr = (_p * len(scenes))(*[sc._h for sc in scenes])
+        h = _p()
+        owner._check(lib().kp_batch_create(owner._h, arr, len(scenes), ctypes.byref(h)))
+        self._h = h
+        self.n = len(scenes)
+
+    def close(self):
+        if self._h:
+            lib().kp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _descs(descs):
+        return (RenderDesc * len(descs))(*descs)
+
+    def tuning(self):
+        t = BatchTuning()
+        self._owner._check(lib().kp_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, t=None, **kw):
+        if t is None:
+            t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._owner._check(lib().kp_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def render(self, descs):
+        assert len(descs) == self.n
+        self._owner._check(lib().kp_batch_render(self._h, self._descs(descs)))
+
+    def autotune(self, descs, passes=2):
+        t = BatchTuning()
+        self._owner._check(lib().kp_batch_autotune(self._h, self._descs(descs), passes, ctypes.byref(t)))
+        return t
+
+    def film_read(self, scene, width, height):
+        out = np.zeros((height, width, 3), np.float32)
+        self._owner._check(lib().kp_batch_film_read(self._h, scene, _np_ptr(out), KP_PTR_HOST))
+        return out
+
+    def stats(self):
+        s = RenderStats()
+        self._owner._check(lib().kp_batch_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in RenderStats._fields_ if k != "reserved"}

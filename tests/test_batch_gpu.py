@@ -1,0 +1,115 @@
+"""Batched scenes in one workload (kp_batch_*).
+
+Every path of a batch draws the same PCG32 stream, camera ray and BVH hits as
+in a render of its scene alone, and the films fold samples in the same order
+with the same Kahan compensation, so each scene's film must equal the
+single-scene render bit for bit: eagerly, through the replayed pass graph,
+for any pass size (passes that split a pixel's samples included), with scenes
+of different depths, and after autotuning."""
+import numpy as np
+import pytest
+
+from kazen_proto_amd import native, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    return [
+        scenes.cornell_box(width=40, height=32, spp=8, sphere_lon=16, sphere_lat=8),
+        scenes.materials_scene(width=24, height=24, spp=4),
+        scenes.terrain_scene(grid=64, n_spheres=6, sphere_lon=12, sphere_lat=6, width=32, height=24, spp=6,
+                             max_depth=5, seed=21),
+        scenes.furnace_box(albedo=0.5, width=16, height=16, spp=3, max_depth=3),
+        scenes.guiding_scene(width=24, height=24, spp=5, max_depth=10, n_spheres=8),
+    ]
+
+
+def _alone(s):
+    c = native.Context(0)
+    c.load_scene(s)
+    img = c.render(native.make_render_desc(s))
+    c.close()
+    return img
+
+
+def _batch(owner, ss):
+    ctxs = []
+    for s in ss:
+        c = native.Context(0)
+        c.load_scene(s)
+        ctxs.append(c)
+    b = native.Batch(owner, ctxs)
+    for c in ctxs:
+        c.close()   # the batch holds its own copies
+    return b
+
+
+def _check(b, ss, alone, tag):
+    for i, s in enumerate(ss):
+        img = b.film_read(i, s.width, s.height)
+        diff = int(np.count_nonzero(img.view(np.uint32) != alone[i].view(np.uint32)))
+        assert diff == 0, "%s: scene %d (%s) differs in %d values" % (tag, i, s.name, diff)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    ss = _scenes()
+    alone = [_alone(s) for s in ss]
+    owner = native.Context(0)
+    b = _batch(owner, ss)
+    descs = [native.make_render_desc(s) for s in ss]
+    yield ss, alone, owner, b, descs
+    b.close()
+    owner.close()
+
+
+@pytest.mark.parametrize("use_graph,pass_paths", [(0, 1 << 21), (1, 1 << 21), (0, 777), (1, 3000)])
+def test_batch_films_equal_single_scene_renders(setup, use_graph, pass_paths):
+    ss, alone, owner, b, descs = setup
+    b.set_tuning(use_graph=use_graph, pass_paths=pass_paths)
+    b.render(descs)
+    _check(b, ss, alone, "graph %d pass %d" % (use_graph, pass_paths))
+    st = b.stats()
+    work = sum(s.width * s.height * s.spp for s in ss)
+    assert st["paths"] == work
+    assert st["passes"] == (work + pass_paths - 1) // pass_paths
+    assert st["extension_rays"] >= work and st["shadow_rays"] > 0
+
+
+def test_graph_replays_match_with_eager_work_between(setup):
+    ss, alone, owner, b, descs = setup
+    other = scenes.cornell_box(width=16, height=16, spp=2, sphere_lon=8, sphere_lat=4)
+    owner.load_scene(other)
+    other_ref = owner.render(native.make_render_desc(other))
+    b.set_tuning(use_graph=1, pass_paths=1024)
+    for rep in range(3):
+        b.render(descs)
+        _check(b, ss, alone, "replay %d" % rep)
+        # eager work on the same stream between replays
+        again = owner.render(native.make_render_desc(other))
+        assert np.array_equal(again.view(np.uint32), other_ref.view(np.uint32))
+
+
+def test_autotune_keeps_results(setup):
+    ss, alone, owner, b, descs = setup
+    t = b.autotune(descs, passes=2)
+    d = t.as_dict()
+    print("autotuned:", d)
+    assert d["pass_paths"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22)
+    assert d["shade_blocks_per_cu"] in (4, 8, 16)
+    assert d["refill_below"] in (16, 32, 48, 56)
+    b.render(descs)
+    _check(b, ss, alone, "autotuned")
+
+
+def test_batch_rejects_guided_and_bad_input(setup):
+    ss, alone, owner, b, descs = setup
+    bad = [native.make_render_desc(s) for s in ss]
+    bad[0].flags = native.RENDER_GUIDED
+    with pytest.raises(native.KpError):
+        b.render(bad)
+    with pytest.raises(native.KpError):
+        b.set_tuning(pass_paths=16)
+    with pytest.raises(native.KpError):
+        b.film_read(len(ss), 4, 4)
