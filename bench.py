@@ -23,6 +23,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "closest_incoherent_microbench.md")
+
+
+def profiled_traffic():
+    """HBM bytes per incoherent launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a copy kernel), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 BASELINE_METRIC = "incoherent Mrays/s (1 GPU); equal-relMSE time, guided vs plain GPU tracer"
 
 
@@ -309,7 +322,8 @@ def main():
     b_ray = 32 + 16 + 128 * nodes_per_ray + 48 * tris_per_ray
     achieved = b_ray * n / (ms_inc * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
+                "traffic_source": "profiles/closest_incoherent_microbench.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
                 "bytes_per_ray": round(b_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
                 "tris_per_ray": round(tris_per_ray, 3)}
 
