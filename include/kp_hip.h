@@ -256,19 +256,20 @@ int kp_guiding_train_step(kp_ctx* ctx, const float* in6, const float* dirs, cons
  * direction, weight = luminance of the radiance gathered beyond the vertex
  * divided by the path throughput before it) and takes train_steps Adam steps
  * of the loss of kp_guiding_train_step on them, weights normalised to sum 1.
- * Sampling from the network starts once guide_after passes have trained it.
+ * Sampling from the network starts once guide_after passes have trained it;
+ * after train_passes training passes (if nonzero) the network stays fixed.
  * Guided renders run at most pass_paths paths per pass. The position is
  * normalised with the box [bmin, bmax] (bmin == bmax: the scene's bounds).
  * kp_integrate_rays with the flag samples from the network without training. */
 typedef struct {
     float bsdf_fraction;   /* (0, 1]; 0.5 */
-    float min_alpha;       /* 0.1 */
+    float min_alpha;       /* 0.5 */
     float lr;              /* Adam learning rate; 1e-3 */
     uint32_t train_steps;  /* per pass; 0 = frozen network */
     uint32_t guide_after;  /* training passes before sampling from the network */
     uint32_t max_records;  /* 1 .. 262144 */
     uint32_t pass_paths;   /* >= 1; 262144 */
-    uint32_t reserved;
+    uint32_t train_passes; /* passes that train (then the network is frozen); 0 = every pass */
     float bmin[3], bmax[3];
 } kp_guiding_desc;
 /* Needs kp_guiding_init first; a later kp_guiding_init keeps the settings. */

@@ -159,6 +159,18 @@ struct Builder {
 
 }  // namespace
 
+constexpr uint32_t kNoKid = 0xFFFFFFFFu;   // empty slot of an 8-wide node
+constexpr uint32_t kQMax = 2047;            // plane grid: integers 0..2047, exact in binary16
+
+// binary16 bit pattern of an integer 0..2047 (exact).
+static uint16_t f16_int(uint32_t q) {
+    if (q == 0) return 0;
+    int e = 0;
+    while ((q >> (e + 1)) != 0) ++e;
+    const uint32_t mant = (q << (10 - e)) & 0x3FFu;
+    return (uint16_t)(((uint32_t)(e + 15) << 10) | mant);
+}
+
 bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh8Host& out,
                 std::string& err) {
     out = Bvh8Host{};
@@ -233,10 +245,44 @@ bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh
                 kids.push_back(c.b);
             }
         }
+        // Octant slot order: slot s (bit k set = high side of axis k) takes the
+        // child whose centre lies most in that direction from the node centre
+        // (greedy over all child/slot pairs), so a ray visiting slots in the
+        // order s ^ octant(ray direction) meets children roughly near to far.
+        {
+            Box pb;
+            for (uint32_t k : kids) pb.grow(b.nodes[k].box);
+            struct Cand { double cost; int kid, slot; };
+            std::vector<Cand> cand;
+            for (int i = 0; i < (int)kids.size(); ++i) {
+                const Box& cb = b.nodes[kids[i]].box;
+                for (int s = 0; s < 8; ++s) {
+                    double cost = 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        const double off = 0.5 * ((double)cb.lo[k] + cb.hi[k]) - 0.5 * ((double)pb.lo[k] + pb.hi[k]);
+                        cost -= ((s >> k) & 1 ? 1.0 : -1.0) * off;
+                    }
+                    cand.push_back({cost, i, s});
+                }
+            }
+            std::stable_sort(cand.begin(), cand.end(), [](const Cand& x, const Cand& y) { return x.cost < y.cost; });
+            int slot_kid[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+            bool kid_done[8] = {false, false, false, false, false, false, false, false};
+            for (const Cand& cd : cand) {
+                if (kid_done[cd.kid] || slot_kid[cd.slot] >= 0) continue;
+                slot_kid[cd.slot] = cd.kid;
+                kid_done[cd.kid] = true;
+            }
+            std::vector<uint32_t> by_slot(8, kNoKid);
+            for (int s = 0; s < 8; ++s)
+                if (slot_kid[s] >= 0) by_slot[s] = kids[slot_kid[s]];
+            kids = by_slot;
+        }
         KpNode8 nd;
         std::memset(&nd, 0, sizeof(nd));
         Box nb;
-        for (uint32_t k : kids) nb.grow(b.nodes[k].box);
+        for (uint32_t k : kids)
+            if (k != kNoKid) nb.grow(b.nodes[k].box);
         nd.px = nb.lo[0];
         nd.py = nb.lo[1];
         nd.pz = nb.lo[2];
@@ -244,9 +290,9 @@ bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh
         for (int k = 0; k < 3; ++k) {
             double ext = (double)nb.hi[k] - (double)nb.lo[k];
             int ek = -126;
-            if (ext > 0.0) ek = (int)std::ceil(std::log2(ext / 65534.0));
+            if (ext > 0.0) ek = (int)std::ceil(std::log2(ext / (double)kQMax));
             ek = std::max(-126, std::min(127, ek));
-            while (std::ldexp(65534.0, ek) < ext && ek < 127) ++ek;
+            while (std::ldexp((double)kQMax, ek) < ext && ek < 127) ++ek;
             e[k] = ek;
         }
         nd.ex = (int8_t)e[0];
@@ -259,8 +305,8 @@ bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh
         nd.child_base = child_base;
         nd.prim_base = prim_base;
         for (int s = 0; s < 8; ++s) {
-            if (s >= (int)kids.size()) {
-                for (int k = 0; k < 3; ++k) { nd.q[k][s] = 0xFFFF; nd.q[3 + k][s] = 0; }
+            if (kids[s] == kNoKid) {
+                for (int k = 0; k < 3; ++k) { nd.q[k][s] = f16_int(kQMax); nd.q[3 + k][s] = f16_int(0); }
                 nd.meta[s] = 0;
                 continue;
             }
@@ -269,10 +315,10 @@ bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh
                 const double sc = std::ldexp(1.0, e[k]);
                 double ql = std::floor(((double)c.box.lo[k] - p[k]) / sc);
                 double qh = std::ceil(((double)c.box.hi[k] - p[k]) / sc);
-                ql = std::max(0.0, std::min(65535.0, ql));
-                qh = std::max(0.0, std::min(65535.0, qh));
-                nd.q[k][s] = (uint16_t)ql;
-                nd.q[3 + k][s] = (uint16_t)qh;
+                ql = std::max(0.0, std::min((double)kQMax, ql));
+                qh = std::max(0.0, std::min((double)kQMax, qh));
+                nd.q[k][s] = f16_int((uint32_t)ql);
+                nd.q[3 + k][s] = f16_int((uint32_t)qh);
             }
             if (c.leaf) {
                 const uint32_t off = (uint32_t)tri_order.size() - prim_base;
@@ -293,6 +339,10 @@ bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh
             max_depth = std::max(max_depth, depth8.back());
         }
         nodes8[pend.n8] = nd;
+    }
+    if (nodes8.size() >= (1u << 24)) {
+        err = "BVH8 exceeds 2^24 nodes (traversal stack entries hold 24-bit node indices)";
+        return false;
     }
     out.nodes = std::move(nodes8);
     out.tri_order = std::move(tri_order);

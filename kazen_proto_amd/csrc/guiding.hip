@@ -157,112 +157,208 @@ __device__ __forceinline__ uint32_t lds_bf16(const uint32_t* buf, int row, int c
     return (col & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
-// One block = 64 samples (4 waves x 16).  Writes this block's gradient
-// contribution (canonical layout) to partial[blockIdx.x].
-__global__ __launch_bounds__(256) void k_guide_train(GuideNet net, const float* __restrict__ in6,
-                                                     const float* __restrict__ dirs,
-                                                     const float* __restrict__ weights,
-                                                     const uint32_t* n_dev, const float* scale_dev,
-                                                     float* __restrict__ partial) {
-    __shared__ uint32_t act[KP_G_LAYERS][64 * KP_G_ROW];   // layer inputs (bf16)
-    __shared__ uint32_t dyb[2][64 * KP_G_ROW];             // gradients (bf16)
-    __shared__ float outs[64 * KP_G_ROW];                  // network outputs (f32)
-    const uint32_t n = *n_dev;
-    const uint32_t base = blockIdx.x * 64;
-    if (base >= n) return;   // uniform per block
-    const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
-    const int g = lane >> 4, r = lane & 15;
-    float* part = partial + (size_t)blockIdx.x * KP_G_PARAMS;
-    const float scale = *scale_dev;
+// Weight-gradient tiles (16 x 16) of the network in a fixed enumeration:
+// layer 0 has 3 k-tiles (48 inputs) x 4 n-tiles, layers 1-3 4 x 4, layer 4
+// 4 x 2; 68 tiles, tile id % 4 = the wave that owns it, id / 4 = its slot in
+// that wave's accumulators.
+__host__ __device__ constexpr int g_ktiles(int l) { return l == 0 ? 3 : 4; }
+__host__ __device__ constexpr int g_tile_base(int l) {
+    return l == 0 ? 0 : g_tile_base(l - 1) + g_ktiles(l - 1) * (g_dim_out(l - 1) / 16);
+}
+#define KP_G_TILES 68
+#define KP_G_WTILES 17
 
-    if (tid < 64) {
-        const bool valid = base + tid < n;
-        g_write_features(act[0] + tid * KP_G_ROW, in6 + 6 * (size_t)(valid ? base + tid : 0), valid);
-    }
-    __syncthreads();
-    {
-        uint32_t* a[4] = {act[1] + 16 * wave * KP_G_ROW, act[2] + 16 * wave * KP_G_ROW,
-                          act[3] + 16 * wave * KP_G_ROW, act[4] + 16 * wave * KP_G_ROW};
-        g_forward_chunk(net, act[0] + 16 * wave * KP_G_ROW, a, outs + 16 * wave * KP_G_ROW, KP_G_ROW);
-    }
-    __syncthreads();
-    if (tid < 64) {
-        const uint32_t s = base + tid;
-        float gout[KP_G_OUT];
-        if (s < n) {
-            float o[KP_G_OUT];
-            for (int i = 0; i < KP_G_OUT; ++i) o[i] = outs[tid * KP_G_ROW + i];
-            g_output_grad(o, dirs[3 * (size_t)s], dirs[3 * (size_t)s + 1], dirs[3 * (size_t)s + 2],
-                          weights[s] * scale, gout);
-        } else {
-            for (int i = 0; i < KP_G_OUT; ++i) gout[i] = 0.0f;
-        }
-        uint32_t* row = dyb[0] + tid * KP_G_ROW;
-        for (int w = 0; w < 16; ++w) row[w] = pack_bf2(gout[2 * w], gout[2 * w + 1]);
-        for (int w = 16; w < 32; ++w) row[w] = 0u;
-    }
-    __syncthreads();
-    int cur = 0;
-    for (int l = KP_G_LAYERS - 1; l >= 0; --l) {
-        const int din = g_dim_in(l), dout = g_dim_out(l);
-        const uint32_t* X = act[l];
-        const uint32_t* dY = dyb[cur];
-        float* gW = part + g_layer_offset(l);
-        float* gb = gW + din * dout;
-        // weight gradient tiles: dW[k][n] = sum_s X[s][k] dY[s][n]
-        const int ntn = dout / 16, tiles = 4 * ntn;
-        for (int tile = wave; tile < tiles; tile += 4) {
-            const int kt = tile / ntn, nt = tile % ntn;
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+// dW[k][n] += sum over the group's 64 samples of X[s][k] dY[s][n]: A = X^T
+// (16 features x 32 samples), B = dY (32 samples x 16 outputs), both read as
+// one 16-byte word from the feature-major blocks.
+template <int L>
+__device__ __forceinline__ void g_dw_tiles(const uint32_t* XT, const uint32_t* dYT, int wave, int lane,
+                                           f32x4* acc) {
+    constexpr int ntn = g_dim_out(L) / 16;
+    const int g = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int kt = 0; kt < g_ktiles(L); ++kt)
+#pragma unroll
+        for (int nt = 0; nt < ntn; ++nt) {
+            const int id = g_tile_base(L) + kt * ntn + nt;
+            if ((id & 3) != wave) continue;
+            f32x4 a = acc[id >> 2];
+#pragma unroll
             for (int sc = 0; sc < 2; ++sc) {
-                uint32_t av[4], bv[4];
-                for (int w = 0; w < 4; ++w) {
-                    const int s0 = 32 * sc + 8 * g + 2 * w;
-                    av[w] = lds_bf16(X, s0, 16 * kt + r) | (lds_bf16(X, s0 + 1, 16 * kt + r) << 16);
-                    bv[w] = lds_bf16(dY, s0, 16 * nt + r) | (lds_bf16(dY, s0 + 1, 16 * nt + r) << 16);
-                }
-                acc = mfma16(u32x4{av[0], av[1], av[2], av[3]}, u32x4{bv[0], bv[1], bv[2], bv[3]}, acc);
+                const u32x4 av = *(const u32x4*)(XT + (16 * kt + r) * KP_G_TROW + 16 * sc + 4 * g);
+                const u32x4 bv = *(const u32x4*)(dYT + (16 * nt + r) * KP_G_TROW + 16 * sc + 4 * g);
+                a = mfma16(av, bv, a);
             }
+            acc[id >> 2] = a;
+        }
+}
+
+template <int L>
+__device__ __forceinline__ void g_dw_store(float* part, int wave, int lane, const f32x4* acc) {
+    constexpr int ntn = g_dim_out(L) / 16, dout = g_dim_out(L);
+    const int g = lane >> 4, r = lane & 15;
+    float* gW = part + g_layer_offset(L);
+#pragma unroll
+    for (int kt = 0; kt < g_ktiles(L); ++kt)
+#pragma unroll
+        for (int nt = 0; nt < ntn; ++nt) {
+            const int id = g_tile_base(L) + kt * ntn + nt;
+            if ((id & 3) != wave) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) gW[(16 * kt + 4 * g + j) * dout + 16 * nt + r] = acc[id >> 2][j];
+        }
+}
+
+// Back-propagation through layer L for one 64-sample group: accumulate its
+// weight and bias gradients, then (L > 0) write dX = (W dY) masked by the
+// ReLU of the layer input, row-major and feature-major, for this wave's 16
+// samples.
+template <int L>
+__device__ __forceinline__ void g_backward_layer(const GuideNet& net, uint32_t (*act)[64 * KP_G_ROW],
+                                                 uint32_t (*actT)[64 * KP_G_TROW],
+                                                 const uint32_t* dY, const uint32_t* dYT, uint32_t* dX,
+                                                 uint32_t* dXT, int tid, int wave, int lane, f32x4* acc,
+                                                 float* bacc) {
+    constexpr int dout = g_dim_out(L);
+    g_dw_tiles<L>(actT[L], dYT, wave, lane, acc);
+    if (tid < dout) {
+        float sb = bacc[L];
+        const uint32_t* row = dYT + tid * KP_G_TROW;
+#pragma unroll 8
+        for (int w = 0; w < 32; ++w) {
+            const uint32_t x = row[w];
+            sb += bf_lo(x);
+            sb += bf_hi(x);
+        }
+        bacc[L] = sb;
+    }
+    if constexpr (L > 0) {
+        const int g = lane >> 4, r = lane & 15;
+        constexpr int nch = dout / 32;
+        const uint32_t* dyrow = dY + (16 * wave + r) * KP_G_ROW;
+        const uint32_t* xrow = act[L] + (16 * wave + r) * KP_G_ROW;
+        uint32_t* dxrow = dX + (16 * wave + r) * KP_G_ROW;
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int c = 0; c < nch; ++c) {
+                const u32x4 b = *(const u32x4*)(dyrow + 16 * c + 4 * g);
+                d = mfma16(net.wb[(g_wb_base(L) + kt * nch + c) * 64 + lane], b, d);
+            }
+            const int k0 = 16 * kt + 4 * g;
+            float v[4];
+#pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int k = 16 * kt + 4 * g + j;
-                if (k < din) gW[k * dout + 16 * nt + r] = acc[j];
+                const uint32_t xw = xrow[(k0 + j) >> 1];
+                const float xv = ((k0 + j) & 1) ? bf_hi(xw) : bf_lo(xw);
+                v[j] = xv > 0.0f ? d[j] : 0.0f;
             }
+            dxrow[k0 / 2] = pack_bf2(v[0], v[1]);
+            dxrow[k0 / 2 + 1] = pack_bf2(v[2], v[3]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) g_store_t(dXT, k0 + j, 16 * wave + r, f2bf_bits(v[j]));
         }
-        if (tid < dout) {
-            float sb = 0.0f;
-            for (int s = 0; s < 64; ++s) {
-                const uint32_t w = dY[s * KP_G_ROW + (tid >> 1)];
-                sb += (tid & 1) ? bf_hi(w) : bf_lo(w);
-            }
-            gb[tid] = sb;
-        }
-        if (l > 0) {
-            // dX^T = W dY^T for this wave's 16 samples, masked by the ReLU
-            uint32_t* dX = dyb[cur ^ 1];
-            const int nch = dout / 32;
-            const uint32_t* dyrow = dY + (16 * wave + r) * KP_G_ROW;
-            const uint32_t* xrow = X + (16 * wave + r) * KP_G_ROW;
-            uint32_t* dxrow = dX + (16 * wave + r) * KP_G_ROW;
-            for (int kt = 0; kt < 4; ++kt) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (int c = 0; c < nch; ++c) {
-                    const u32x4 b = *(const u32x4*)(dyrow + 16 * c + 4 * g);
-                    acc = mfma16(net.wb[(g_wb_base(l) + kt * nch + c) * 64 + lane], b, acc);
-                }
-                const int k0 = 16 * kt + 4 * g;
-                float v[4];
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t xw = xrow[(k0 + j) >> 1];
-                    const float xv = ((k0 + j) & 1) ? bf_hi(xw) : bf_lo(xw);
-                    v[j] = xv > 0.0f ? acc[j] : 0.0f;
-                }
-                dxrow[k0 / 2] = pack_bf2(v[0], v[1]);
-                dxrow[k0 / 2 + 1] = pack_bf2(v[2], v[3]);
+    }
+}
+
+// Persistent training blocks: block b runs the 64-sample groups b, b + grid,
+// ... (4 waves x 16 samples each), keeps its weight-gradient tiles in the
+// matrix-core accumulators and its bias gradients in registers across groups,
+// and writes one partial gradient (canonical layout) to partial[b].
+__global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const float* __restrict__ in6,
+                                                        const float* __restrict__ dirs,
+                                                        const float* __restrict__ weights,
+                                                        const uint32_t* n_dev, const float* scale_dev,
+                                                        float* __restrict__ partial) {
+    __shared__ uint32_t act[KP_G_LAYERS][64 * KP_G_ROW];    // layer inputs, sample-major (bf16)
+    __shared__ uint32_t actT[KP_G_LAYERS][64 * KP_G_TROW];  // the same, feature-major
+    __shared__ uint32_t dyb[2][64 * KP_G_ROW];              // gradients, sample-major
+    __shared__ uint32_t dyT[2][64 * KP_G_TROW];             // gradients, feature-major
+    __shared__ float outs[64 * KP_G_ROW];                   // network outputs (f32)
+    const uint32_t n = *n_dev;
+    const uint32_t groups = (n + 63) / 64;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
+    const float scale = *scale_dev;
+    f32x4 acc[KP_G_WTILES];
+#pragma unroll
+    for (int i = 0; i < KP_G_WTILES; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc[KP_G_LAYERS] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+        const uint32_t base = grp * 64;
+        if (tid < 64) {
+            const bool valid = base + tid < n;
+            uint32_t* row = act[0] + tid * KP_G_ROW;
+            g_write_features(row, in6 + 6 * (size_t)(valid ? base + tid : 0), valid);
+#pragma unroll
+            for (int w = 0; w < 24; ++w) {
+                const uint32_t x = row[w];
+                g_store_t(actT[0], 2 * w, tid, x & 0xffffu);
+                g_store_t(actT[0], 2 * w + 1, tid, x >> 16);
             }
         }
         __syncthreads();
-        cur ^= 1;
+        {
+            uint32_t* a[4] = {act[1] + 16 * wave * KP_G_ROW, act[2] + 16 * wave * KP_G_ROW,
+                              act[3] + 16 * wave * KP_G_ROW, act[4] + 16 * wave * KP_G_ROW};
+            uint32_t* at[4] = {actT[1], actT[2], actT[3], actT[4]};
+            g_forward_chunk(net, act[0] + 16 * wave * KP_G_ROW, a, outs + 16 * wave * KP_G_ROW, KP_G_ROW,
+                            nullptr, at, 16 * wave);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const uint32_t s = base + tid;
+            float gout[KP_G_OUT];
+            if (s < n) {
+                float o[KP_G_OUT];
+                for (int i = 0; i < KP_G_OUT; ++i) o[i] = outs[tid * KP_G_ROW + i];
+                g_output_grad(o, dirs[3 * (size_t)s], dirs[3 * (size_t)s + 1], dirs[3 * (size_t)s + 2],
+                              weights[s] * scale, gout);
+            } else {
+                for (int i = 0; i < KP_G_OUT; ++i) gout[i] = 0.0f;
+            }
+            uint32_t* row = dyb[0] + tid * KP_G_ROW;
+            for (int w = 0; w < 16; ++w) {
+                row[w] = pack_bf2(gout[2 * w], gout[2 * w + 1]);
+                g_store_t(dyT[0], 2 * w, tid, f2bf_bits(gout[2 * w]));
+                g_store_t(dyT[0], 2 * w + 1, tid, f2bf_bits(gout[2 * w + 1]));
+            }
+            for (int w = 16; w < 32; ++w) row[w] = 0u;
+        }
+        __syncthreads();
+        g_backward_layer<4>(net, act, actT, dyb[0], dyT[0], dyb[1], dyT[1], tid, wave, lane, acc, bacc);
+        __syncthreads();
+        g_backward_layer<3>(net, act, actT, dyb[1], dyT[1], dyb[0], dyT[0], tid, wave, lane, acc, bacc);
+        __syncthreads();
+        g_backward_layer<2>(net, act, actT, dyb[0], dyT[0], dyb[1], dyT[1], tid, wave, lane, acc, bacc);
+        __syncthreads();
+        g_backward_layer<1>(net, act, actT, dyb[1], dyT[1], dyb[0], dyT[0], tid, wave, lane, acc, bacc);
+        __syncthreads();
+        g_backward_layer<0>(net, act, actT, dyb[0], dyT[0], nullptr, nullptr, tid, wave, lane, acc, bacc);
+        __syncthreads();
     }
+    float* part = partial + (size_t)blockIdx.x * KP_G_PARAMS;
+    g_dw_store<0>(part, wave, lane, acc);
+    g_dw_store<1>(part, wave, lane, acc);
+    g_dw_store<2>(part, wave, lane, acc);
+    g_dw_store<3>(part, wave, lane, acc);
+    g_dw_store<4>(part, wave, lane, acc);
+#pragma unroll
+    for (int l = 0; l < KP_G_LAYERS; ++l)
+        if (tid < g_dim_out(l)) part[g_layer_offset(l) + g_dim_in(l) * g_dim_out(l) + tid] = bacc[l];
+}
+
+// First stage of the gradient reduction: seg[s][p] = sum over partials
+// s*per .. (s+1)*per - 1 (fixed order).
+__global__ __launch_bounds__(256) void k_guide_reduce_seg(const float* __restrict__ partial, uint32_t nparts,
+                                                          uint32_t per, float* __restrict__ seg) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= KP_G_PARAMS) return;
+    const uint32_t b0 = blockIdx.y * per, b1 = b0 + per < nparts ? b0 + per : nparts;
+    float sum = 0.0f;
+    for (uint32_t b = b0; b < b1; ++b) sum += partial[(size_t)b * KP_G_PARAMS + p];
+    seg[(size_t)blockIdx.y * KP_G_PARAMS + p] = sum;
 }
 
 // One block: sums weights[0..n) in a fixed order (FP64), then writes the
@@ -296,13 +392,12 @@ __global__ __launch_bounds__(1024) void k_guide_normalise(const float* __restric
     }
 }
 
-// grad = sum of the block contributions (fixed order); optional Adam step.
-__global__ void k_guide_reduce_adam(const float* __restrict__ partial, const uint32_t* n_dev, float* grad,
+// grad = sum of the nb segment sums (fixed order); optional Adam step.
+__global__ void k_guide_reduce_adam(const float* __restrict__ partial, uint32_t nb, float* grad,
                                     float* params, float* m, float* v, int step, float lr, float beta1,
                                     float beta2, float eps, int apply) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= KP_G_PARAMS) return;
-    const uint32_t nb = (*n_dev + 63) / 64;
     float gsum = 0.0f;
     for (uint32_t b = 0; b < nb; ++b) gsum += partial[(size_t)b * KP_G_PARAMS + p];
     grad[p] = gsum;
@@ -325,7 +420,7 @@ void guide_state_free(kp_ctx* c) {
     if (!g) return;
     for (void* p : {(void*)g->params, (void*)g->m, (void*)g->v, (void*)g->grad, (void*)g->wf, (void*)g->wb,
                     (void*)g->partial, (void*)g->in6, (void*)g->dirs, (void*)g->weights, (void*)g->count,
-                    (void*)g->scale, (void*)g->rec_aux, (void*)g->rec_valid})
+                    (void*)g->scale, (void*)g->rec_aux, (void*)g->rec_valid, (void*)g->seg})
         if (p) (void)hipFree(p);
     delete g;
     c->guide = nullptr;
@@ -360,7 +455,9 @@ static int guide_alloc(kp_ctx* c) {
     KP_HIP_TRY(c, hipMalloc((void**)&g->grad, P));
     KP_HIP_TRY(c, hipMalloc((void**)&g->wf, KP_G_WF_FRAGS * 64 * sizeof(u32x4)));
     KP_HIP_TRY(c, hipMalloc((void**)&g->wb, KP_G_WB_FRAGS * 64 * sizeof(u32x4)));
-    KP_HIP_TRY(c, hipMalloc((void**)&g->partial, (size_t)(g->max_batch / 64) * P));
+    g->train_blocks = (uint32_t)(c->cus > 0 ? c->cus : 256);
+    KP_HIP_TRY(c, hipMalloc((void**)&g->partial, (size_t)g->train_blocks * P));
+    KP_HIP_TRY(c, hipMalloc((void**)&g->seg, (size_t)KP_G_SEGS * P));
     KP_HIP_TRY(c, hipMalloc((void**)&g->in6, (size_t)g->max_batch * 6 * sizeof(float)));
     KP_HIP_TRY(c, hipMalloc((void**)&g->dirs, (size_t)g->max_batch * 3 * sizeof(float)));
     KP_HIP_TRY(c, hipMalloc((void**)&g->weights, (size_t)g->max_batch * sizeof(float)));
@@ -390,13 +487,17 @@ static int guide_set_params(kp_ctx* c, const float* host) {
 int guide_train_device(kp_ctx* c, float lr, bool apply) {
     GuideState* g = c->guide;
     GuideNet net = guide_net(c);
-    const uint32_t blocks = g->max_batch / 64;
+    const uint32_t blocks = g->train_blocks;
     hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), 0, c->stream, net, g->in6, g->dirs, g->weights,
                        g->count, g->scale, g->partial);
     KP_HIP_TRY(c, hipGetLastError());
+    const uint32_t per = (blocks + KP_G_SEGS - 1) / KP_G_SEGS;
+    hipLaunchKernelGGL(k_guide_reduce_seg, dim3((KP_G_PARAMS + 255) / 256, KP_G_SEGS), dim3(256), 0, c->stream,
+                       g->partial, blocks, per, g->seg);
+    KP_HIP_TRY(c, hipGetLastError());
     if (apply) ++g->step;
     hipLaunchKernelGGL(k_guide_reduce_adam, dim3((KP_G_PARAMS + 255) / 256), dim3(256), 0, c->stream,
-                       g->partial, g->count, g->grad, g->params, g->m, g->v, g->step > 0 ? g->step : 1, lr,
+                       g->seg, (uint32_t)KP_G_SEGS, g->grad, g->params, g->m, g->v, g->step > 0 ? g->step : 1, lr,
                        g->beta1, g->beta2, g->eps, apply ? 1 : 0);
     KP_HIP_TRY(c, hipGetLastError());
     if (apply) return guide_pack(c);
@@ -570,6 +671,7 @@ int kp_guiding_configure(kp_ctx* c, const kp_guiding_desc* d) {
     g->guide_after = d->guide_after;
     g->max_records = d->max_records;
     g->pass_paths = d->pass_paths;
+    g->max_train_passes = d->train_passes;
     g->auto_box = d->bmin[0] == d->bmax[0] && d->bmin[1] == d->bmax[1] && d->bmin[2] == d->bmax[2];
     for (int i = 0; i < 3; ++i) {
         g->box_min[i] = d->bmin[i];

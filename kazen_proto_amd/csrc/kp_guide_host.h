@@ -7,6 +7,9 @@
 
 struct kp_ctx;
 
+// Segments of the two-stage gradient reduction.
+#define KP_G_SEGS 16
+
 struct GuideState {
     float* params = nullptr;   // FP32 masters, canonical layout
     float* m = nullptr;        // Adam moments
@@ -14,7 +17,9 @@ struct GuideState {
     float* grad = nullptr;     // last reduced gradient
     kp::u32x4* wf = nullptr;   // bf16 fragments
     kp::u32x4* wb = nullptr;
-    float* partial = nullptr;  // per-block gradient contributions
+    float* partial = nullptr;  // per-block gradient contributions [train_blocks][params]
+    float* seg = nullptr;      // first-stage sums [KP_G_SEGS][params]
+    uint32_t train_blocks = 256;
     uint32_t max_batch = 1u << 18;
     // training batch (also the in-render training records)
     float* in6 = nullptr;
@@ -28,8 +33,9 @@ struct GuideState {
     float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     float3 bmin{0, 0, 0}, binv{1, 1, 1};
     // guided rendering (kp_guiding_configure)
-    float bsdf_fraction = 0.5f, min_alpha = 0.1f, lr = 1e-3f;
+    float bsdf_fraction = 0.5f, min_alpha = 0.5f, lr = 1e-3f;
     uint32_t train_steps = 1, guide_after = 1, max_records = 1u << 18, pass_paths = 1u << 18;
+    uint32_t max_train_passes = 0;   // 0 = every pass trains
     bool auto_box = true;
     float box_min[3] = {0, 0, 0}, box_max[3] = {0, 0, 0};
     uint32_t train_passes = 0;   // passes whose records trained the network

@@ -12,9 +12,15 @@
 //   dword 6-7  meta[8]     uint8    leaf slot: (count << 5) | offset,
 //                                   count 1..7 triangles at prim_base+offset;
 //                                   inner slot: 1; empty slot: 0
-//   dword 8-31 q[6][8]     uint16   child boxes, planes lo.x lo.y lo.z hi.x
-//                                   hi.y hi.z, each 8 slots; decoded bound =
-//                                   p + q * 2^e  (conservative by construction)
+//   dword 8-31 q[6][8]     binary16 child boxes, planes lo.x lo.y lo.z hi.x
+//                                   hi.y hi.z, each 8 slots, holding integers
+//                                   0..2047 (exact in binary16, so the
+//                                   mixed-precision fma reads them without a
+//                                   conversion); decoded bound = p + q * 2^e
+//                                   (conservative by construction); an empty
+//                                   slot has lo = 2047, hi = 0 and meta 0.
+// Inner children sit in octant slots: bit k of the slot set = the child lies
+// on the high side of axis k (as nearly as the builder can arrange).
 // Triangle record: 48 bytes = three float4 {v0.xyz, prim}, {v1.xyz, 0},
 // {v2.xyz, 0}; `prim` is the caller-visible primitive index (bit pattern).
 #pragma once

@@ -1159,8 +1159,8 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     // training records: every rec_stride-th path (phase rotating per pass),
     // max_depth records each, at most max_records per pass
     uint32_t stride = 1, rec_slots_max = 0;
-    const bool train = guided && g->train_steps > 0;
-    if (train) {
+    const bool train_any = guided && g->train_steps > 0;
+    if (train_any) {
         const uint64_t D = d->max_depth;
         stride = (uint32_t)std::max<uint64_t>(1, ((uint64_t)n_paths * D + g->max_records - 1) / g->max_records);
         while (((uint64_t)n_paths + stride - 1) / stride * D > g->max_records) ++stride;
@@ -1168,6 +1168,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     }
     const uint32_t gen_blocks = (n_paths + 255) / 256;
     uint32_t steps = 0, guided_passes = 0;
+    bool train = false;
     auto t0 = std::chrono::steady_clock::now();
     unsigned long long zero2[2] = {0, 0};
     KP_HIP_TRY(c, hipMemcpyAsync(r->ps.totals, zero2, 16, hipMemcpyHostToDevice, c->stream));
@@ -1177,6 +1178,8 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
         if (guided) {
             a.guide_on = g->train_passes >= g->guide_after ? 1u : 0u;
             guided_passes += a.guide_on;
+            a.rec_cap = 0;
+            train = train_any && (g->max_train_passes == 0 || g->train_passes < g->max_train_passes);
             if (train) {
                 a.rec_stride = stride;
                 a.rec_phase = g->train_passes % stride;
@@ -1218,7 +1221,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     r->stats.train_steps = steps;
     r->stats.guided_passes = guided_passes;
     r->stats.train_records = 0;
-    if (train) KP_HIP_TRY(c, hipMemcpy(&r->stats.train_records, g->rec_valid, 4, hipMemcpyDeviceToHost));
+    if (steps) KP_HIP_TRY(c, hipMemcpy(&r->stats.train_records, g->rec_valid, 4, hipMemcpyDeviceToHost));
     return KP_OK;
 }
 
@@ -1301,8 +1304,8 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
         depth = std::max(depth, s->max_depth);
     }
     KP_HIP_TRY(c, hipSetDevice(c->device));
-    if (nt >= (1ull << 28) || nn >= (1ull << 31))
-        return fail(c, KP_ERR_INVALID, "kp_batch_create: more than 2^28 triangles in the batch");
+    if (nt >= (1ull << 28) || nn >= (1ull << 24))
+        return fail(c, KP_ERR_INVALID, "kp_batch_create: more than 2^28 triangles or 2^24 BVH8 nodes in the batch");
     BatchState* b = new BatchState();
     b->c = c;
     auto bail = [&](hipError_t e, const char* what) {

@@ -10,11 +10,14 @@
 //  - persistent lanes: every lane owns one ray at a time; finished lanes are
 //    refilled from a global ray counter, one atomic per wave (ballot +
 //    prefix count), so incoherent rays do not leave idle lanes behind;
-//  - a per-lane short stack in LDS (kStack entries of {ref, tnear}); deeper
-//    entries spill to a per-lane region in HBM, so any BVH depth works;
-//  - 128-byte nodes: eight 16-byte loads per visit, 16-bit child boxes decoded
-//    with one fma per plane; children sorted near-to-far by an 8-key sorting
-//    network on (tnear bits | slot);
+//  - a per-lane short stack in LDS (kStack group entries); deeper entries
+//    spill to a per-lane region in HBM, so any BVH depth works;
+//  - 128-byte nodes: eight 16-byte loads per visit (each lane loads its own
+//    near and far planes by the ray's octant), 16-bit child boxes decoded
+//    with one fma per plane; hit leaves are tested at once, the nearest hit
+//    inner child is visited next and the others go on the stack as one group,
+//    visited in octant order (children sit in octant slots) and culled as a
+//    group by their nearest entry distance;
 //  - watertight ray/triangle test (Woop, Benthin, Wald 2013) with contraction
 //    disabled, so its result is bit-identical to the host oracle's;
 //  - closest hit is the smallest t, ties broken by the lowest primitive
@@ -28,8 +31,6 @@
 
 namespace kp {
 
-// 1 + 2*gamma(3), gamma(n) = n*u/(1-n*u), u = 2^-24 (Ize 2013).
-#define KP_FAR_SCALE 1.00000036f
 
 struct RayPre {
     float ox, oy, oz, tmin, dx, dy, dz, tmax;
@@ -114,73 +115,124 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, const float4 a, const 
 
 __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
-__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
-    uint32_t lo = min(a, b), hi = max(a, b);
-    a = lo;
-    b = hi;
-}
-
-// Optimal 19-comparator sorting network for 8 keys.
-__device__ __forceinline__ void sort8(uint32_t* k) {
-    cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
-    cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
-    cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
-    cswap(k[2], k[4]); cswap(k[3], k[5]);
-    cswap(k[1], k[4]); cswap(k[3], k[6]);
-    cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
-}
-
-struct NodeRegs {
-    uint4 w[8];
+// Per-ray node-test constants: the ray direction's octant (bit k set when
+// direction k is negative) and the 16-byte word of each near / far plane in a
+// node (planes lo.x lo.y lo.z hi.x hi.y hi.z are words 2..7), so each lane
+// loads its own near and far planes instead of selecting them per child.
+struct NodeTest {
+    uint32_t oct;
+    uint32_t wn[3], wf[3];
 };
 
-__device__ __forceinline__ void load_node(const uint4* __restrict__ nodes, uint32_t idx, NodeRegs& n) {
-    const uint4* p = nodes + (size_t)idx * 8;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) n.w[i] = p[i];
+__device__ __forceinline__ NodeTest node_test_setup(const RayPre& r) {
+    NodeTest t;
+    const bool nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
+    t.oct = (nx ? 1u : 0u) | (ny ? 2u : 0u) | (nz ? 4u : 0u);
+    t.wn[0] = nx ? 5u : 2u; t.wf[0] = nx ? 2u : 5u;
+    t.wn[1] = ny ? 6u : 3u; t.wf[1] = ny ? 3u : 6u;
+    t.wn[2] = nz ? 7u : 4u; t.wf[2] = nz ? 4u : 7u;
+    return t;
 }
 
-__device__ __forceinline__ uint32_t q16(const uint4& plane, int slot) {
+// Plane value of a slot: a binary16 integer read straight from its half of
+// the word (the fma below becomes one mixed-precision fma, no conversion).
+__device__ __forceinline__ float qplane(const uint4& plane, int slot) {
     const uint32_t d = (slot >> 1) == 0 ? plane.x : (slot >> 1) == 1 ? plane.y : (slot >> 1) == 2 ? plane.z : plane.w;
-    return (slot & 1) ? (d >> 16) : (d & 0xFFFFu);
+    const unsigned short h = (slot & 1) ? (unsigned short)(d >> 16) : (unsigned short)(d & 0xFFFFu);
+    return (float)__builtin_bit_cast(_Float16, h);
 }
 
-// Tests the 8 children of a node; returns sort keys (0xFFFFFFFF = miss).
-__device__ __forceinline__ void test_children(const RayPre& r, const NodeRegs& n, float tmax,
-                                              uint32_t* key) {
-    const float px = __uint_as_float(n.w[0].x), py = __uint_as_float(n.w[0].y), pz = __uint_as_float(n.w[0].z);
-    const uint32_t e = n.w[0].w;
+// Relative widening of a far distance, 2*gamma(3) (Ize 2013): tf + |tf| * eps.
+#define KP_FAR_EPS 3.5762793e-7f
+
+// Bit s set for every slot whose meta byte is nonzero (a used slot).
+__device__ __forceinline__ uint32_t used_slots(uint32_t m0, uint32_t m1) {
+    auto nz4 = [](uint32_t w) {
+        w |= w >> 4;
+        w |= w >> 2;
+        w |= w >> 1;
+        w &= 0x01010101u;
+        return (w | (w >> 7) | (w >> 14) | (w >> 21)) & 0xFu;
+    };
+    return nz4(m0) | (nz4(m1) << 4);
+}
+
+// Result of one node visit.
+struct NodeHits {
+    uint4 w0, w1;        // header words (origin / exponents / inner mask, bases / meta)
+    uint32_t inner;      // inner children hit (slot bits)
+    uint32_t leaves;     // leaf children hit (slot bits)
+    uint32_t near_key;   // (tnear key | slot) of the nearest inner child hit
+    uint32_t next_key;   // the same for the second nearest (0xFFFFFFFF if none)
+};
+
+// Order key of a child entry distance: non-negative distances by their bits
+// (monotone), negative ones all first.
+__device__ __forceinline__ uint32_t tkey(float tn) {
+    return tn > 0.0f ? (__float_as_uint(tn) & 0xFFFFFFF8u) : 0u;
+}
+
+// Tests the 8 children of node `idx` against [tmin, tmax].
+__device__ __forceinline__ void visit_node(const uint4* __restrict__ nodes, uint32_t idx, const RayPre& r,
+                                           const NodeTest& nt, float tmax, NodeHits& h) {
+    const uint4* p = nodes + (size_t)idx * 8;
+    h.w0 = p[0];
+    h.w1 = p[1];
+    const uint4 nx = p[nt.wn[0]], fx = p[nt.wf[0]];
+    const uint4 ny = p[nt.wn[1]], fy = p[nt.wf[1]];
+    const uint4 nz = p[nt.wn[2]], fz = p[nt.wf[2]];
+    const float px = __uint_as_float(h.w0.x), py = __uint_as_float(h.w0.y), pz = __uint_as_float(h.w0.z);
+    const uint32_t e = h.w0.w;
     const int ex = (int)(int8_t)(e & 0xFF), ey = (int)(int8_t)((e >> 8) & 0xFF), ez = (int)(int8_t)((e >> 16) & 0xFF);
     const float ax = exp2i(ex) * r.ix, ay = exp2i(ey) * r.iy, az = exp2i(ez) * r.iz;
     const float bx = (px - r.ox) * r.ix, by = (py - r.oy) * r.iy, bz = (pz - r.oz) * r.iz;
-    // near/far planes per axis from the direction signs
-    const uint4 nx = r.ix >= 0.0f ? n.w[2] : n.w[5], fx = r.ix >= 0.0f ? n.w[5] : n.w[2];
-    const uint4 ny = r.iy >= 0.0f ? n.w[3] : n.w[6], fy = r.iy >= 0.0f ? n.w[6] : n.w[3];
-    const uint4 nz = r.iz >= 0.0f ? n.w[4] : n.w[7], fz = r.iz >= 0.0f ? n.w[7] : n.w[4];
+    const uint32_t imask = e >> 24, nin = ~imask;
+    uint32_t hit = 0u, k1 = 0xFFFFFFFFu, k2 = 0xFFFFFFFFu;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const float tnx = fmaf((float)q16(nx, s), ax, bx);
-        const float tfx = fmaf((float)q16(fx, s), ax, bx);
-        const float tny = fmaf((float)q16(ny, s), ay, by);
-        const float tfy = fmaf((float)q16(fy, s), ay, by);
-        const float tnz = fmaf((float)q16(nz, s), az, bz);
-        const float tfz = fmaf((float)q16(fz, s), az, bz);
+        const float tnx = fmaf(qplane(nx, s), ax, bx);
+        const float tfx = fmaf(qplane(fx, s), ax, bx);
+        const float tny = fmaf(qplane(ny, s), ay, by);
+        const float tfy = fmaf(qplane(fy, s), ay, by);
+        const float tnz = fmaf(qplane(nz, s), az, bz);
+        const float tfz = fmaf(qplane(fz, s), az, bz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        const uint32_t mw = s < 4 ? n.w[1].z : n.w[1].w;
-        const bool used = ((mw >> ((s & 3) * 8)) & 0xFFu) != 0u;   // empty slots never hit
-        const bool hit = used && tn <= tf * KP_FAR_SCALE;
-        key[s] = hit ? ((__float_as_uint(tn) & 0x7FFFFFF8u) | (uint32_t)s) : 0xFFFFFFFFu;
+        const bool hs = tn <= fmaf(fabsf(tf), KP_FAR_EPS, tf);
+        hit |= hs ? (1u << s) : 0u;
+        // key: entry distance bits (non-positive -> 0), slot, bit 31 for leaves
+        const uint32_t kv = (tkey(tn) | (uint32_t)s) | (((nin >> s) & 1u) << 31);
+        const uint32_t key = hs ? kv : 0xFFFFFFFFu;
+        k2 = min(k2, max(k1, key));
+        k1 = min(k1, key);
     }
+    hit &= used_slots(h.w1.z, h.w1.w);   // empty slots never hit
+    h.inner = hit & imask;
+    h.leaves = hit & ~imask;
+    h.near_key = k1;
+    h.next_key = k2;
 }
 
-__device__ __forceinline__ uint32_t child_ref(const NodeRegs& n, uint32_t slot) {
-    const uint32_t imask = n.w[0].w >> 24;
-    const uint32_t mw = slot < 4 ? n.w[1].z : n.w[1].w;
-    const uint32_t meta = (mw >> ((slot & 3u) * 8u)) & 0xFFu;
-    if ((imask >> slot) & 1u) return n.w[1].x + __popc(imask & ((1u << slot) - 1u));
-    const uint32_t first = n.w[1].y + (meta & 31u);
-    return KP_LEAF_FLAG | (first << 3) | ((meta >> 5) - 1u);
+__device__ __forceinline__ uint32_t inner_child(uint32_t child_base, uint32_t imask, uint32_t slot) {
+    return child_base + __popc(imask & ((1u << slot) - 1u));
+}
+
+// Slot bits reordered by rank = slot ^ oct (an involution).
+__device__ __forceinline__ uint32_t oct_permute(uint32_t m, uint32_t oct) {
+    if (oct & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+    if (oct & 2u) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+    if (oct & 4u) m = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
+    return m;
+}
+
+// Stack entry (a group of inner children of one node still to visit):
+//   x = child_base (24 bits) | inner mask << 24
+//   y = lower bound of the group's entry distances (float bits, low 8 bits
+//       cleared: a truncation toward zero, or -inf) | remaining children as
+//       octant-rank bits (8 bits).
+__device__ __forceinline__ uint32_t group_bound_bits(uint32_t key2) {
+    if (key2 == 0xFFFFFFFFu || key2 < 8u) return 0xFF800000u;   // none, or a negative distance: -inf
+    return key2 & 0xFFFFFF00u;
 }
 
 template <int S>
@@ -191,20 +243,20 @@ struct Stack {
     int sp;
     int tid;
 
-    __device__ __forceinline__ bool push(uint32_t ref, uint32_t tbits) {
+    __device__ __forceinline__ bool push(uint32_t a, uint32_t b) {
         if (sp >= S) {
             if ((uint32_t)(sp - S) >= cap) return false;
             spill[sp - S] = lds[sp & (S - 1)][tid];
         }
-        lds[sp & (S - 1)][tid] = make_uint2(ref, tbits);
+        lds[sp & (S - 1)][tid] = make_uint2(a, b);
         ++sp;
         return true;
     }
-    __device__ __forceinline__ uint2 pop() {
+    __device__ __forceinline__ uint2 top() const { return lds[(sp - 1) & (S - 1)][tid]; }
+    __device__ __forceinline__ void set_top_y(uint32_t y) { lds[(sp - 1) & (S - 1)][tid].y = y; }
+    __device__ __forceinline__ void drop() {
         --sp;
-        const uint2 e = lds[sp & (S - 1)][tid];
         if (sp >= S) lds[sp & (S - 1)][tid] = spill[sp - S];
-        return e;
     }
 };
 
@@ -268,11 +320,16 @@ struct RayFeeder {
     }
 };
 
-template <bool kCount>
-__global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
+// Persistent traversal shared by closest hit (kAny = false) and occlusion
+// (kAny = true).  A node visit tests the 8 children; hit leaf children are
+// tested at once; the nearest hit inner child is visited next and the other
+// hit inner children are pushed as one group entry, taken later in octant
+// order (the builder puts children in octant slots) and skipped as a whole
+// when their nearest entry distance lies beyond the closest hit so far.
+template <bool kAny, bool kCount>
+__device__ __forceinline__ void traverse(const TraceArgs& args) {
     __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
     const int tid = threadIdx.x;
-    const int lane = __lane_id();
     const uint32_t gl = blockIdx.x * KP_TRACE_BLOCK + tid;
     Stack<kStack> st;
     st.lds = lds;
@@ -286,10 +343,11 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
 
     uint32_t ray_id = KP_INVALID;
     RayPre r;
+    NodeTest nt;
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     uint32_t best_p = KP_INVALID, cur = KP_INVALID, steps = 0;
+    bool occl = false;
     uint64_t cnt_nodes = 0, cnt_tris = 0;
-    (void)lane;
     RayFeeder feed;
     feed.init(args.counter, args.rays, args.roots, n_rays);
 
@@ -297,9 +355,11 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
         if (ray_id == KP_INVALID && feed.pending) {
             ray_id = feed.pid;
             setup_ray(feed.p0, feed.p1, r);
+            nt = node_test_setup(r);
             best_t = r.tmax;
             best_p = KP_INVALID;
             bu = bv = 0.0f;
+            occl = false;
             cur = ray_valid(feed.p0, feed.p1) ? feed.proot : KP_INVALID;   // root
             st.sp = 0;
             steps = 0;
@@ -310,47 +370,67 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
         if (ray_id != KP_INVALID) {
             for (;;) {
                 bool done = false;
-                if (cur == KP_INVALID) {
-                    done = true;   // invalid ray: reported as a miss
-                } else if (cur & KP_LEAF_FLAG) {
-                    const uint32_t first = (cur & ~KP_LEAF_FLAG) >> 3;
-                    const uint32_t count = (cur & 7u) + 1u;
-                    if (kCount) cnt_tris += count;
-                    for (uint32_t i = 0; i < count; ++i) {
-                        const float4* tp = tris + 3 * (size_t)(first + i);
-                        const float4 a = tp[0], b = tp[1], c = tp[2];
-                        float t, u, v;
-                        if (tri_test(r, a, b, c, best_t, t, u, v)) {
-                            const uint32_t p = __float_as_uint(a.w);
-                            if (t < best_t || p < best_p) { best_t = t; best_p = p; bu = u; bv = v; }
-                        }
-                    }
-                    cur = KP_INVALID;
-                } else {
-                    NodeRegs n;
-                    load_node(nodes, cur, n);
+                if (cur != KP_INVALID) {
+                    NodeHits h;
+                    visit_node(nodes, cur, r, nt, best_t, h);
                     if (kCount) cnt_nodes++;
-                    uint32_t key[8];
-                    test_children(r, n, best_t, key);
-                    sort8(key);
                     cur = KP_INVALID;
-                    if (key[0] != 0xFFFFFFFFu) {
-                        bool ok = true;
-#pragma unroll
-                        for (int j = 7; j >= 1; --j)
-                            if (key[j] != 0xFFFFFFFFu)
-                                ok = ok && st.push(child_ref(n, key[j] & 7u), key[j] & ~7u);
-                        if (!ok) {
-                            atomicOr(args.err, 1u);
-                            done = true;
+                    // leaf children: (count << 5) | offset from prim_base
+                    uint32_t lm = h.leaves;
+                    while (lm) {
+                        const uint32_t s = __ffs(lm) - 1u;
+                        lm &= lm - 1u;
+                        const uint32_t mw = s < 4u ? h.w1.z : h.w1.w;
+                        const uint32_t meta = (mw >> ((s & 3u) * 8u)) & 0xFFu;
+                        const uint32_t first = h.w1.y + (meta & 31u), count = meta >> 5;
+                        if (kCount) cnt_tris += count;
+                        for (uint32_t i = 0; i < count; ++i) {
+                            const float4* tp = tris + 3 * (size_t)(first + i);
+                            const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+                            float t, u, v;
+                            if (tri_test(r, ta, tb, tc, best_t, t, u, v)) {
+                                if (kAny) {
+                                    occl = true;
+                                    break;
+                                }
+                                const uint32_t pr = __float_as_uint(ta.w);
+                                if (t < best_t || pr < best_p) { best_t = t; best_p = pr; bu = u; bv = v; }
+                            }
                         }
-                        cur = child_ref(n, key[0] & 7u);
+                        if (kAny && occl) break;
+                    }
+                    if (kAny && occl) {
+                        done = true;
+                    } else if (h.inner) {
+                        const uint32_t imask = h.w0.w >> 24;
+                        const uint32_t s1 = h.near_key & 7u;
+                        const uint32_t rest = h.inner & ~(1u << s1);
+                        if (rest) {
+                            const uint32_t gx = (h.w1.x & 0x00FFFFFFu) | (imask << 24);
+                            const uint32_t gy = group_bound_bits(h.next_key) | oct_permute(rest, nt.oct);
+                            if (!st.push(gx, gy)) {
+                                atomicOr(args.err, 1u);
+                                done = true;
+                            }
+                        }
+                        cur = inner_child(h.w1.x, imask, s1);
                     }
                 }
                 if (!done && cur == KP_INVALID) {
                     while (st.sp > 0) {
-                        const uint2 e = st.pop();
-                        if (__uint_as_float(e.y) <= best_t) { cur = e.x; break; }
+                        const uint2 e = st.top();
+                        const float bound = __uint_as_float(e.y & 0xFFFFFF00u);
+                        if (!kAny && bound > best_t) {   // every child of the group is beyond the hit
+                            st.drop();
+                            continue;
+                        }
+                        uint32_t pm = e.y & 0xFFu;
+                        const uint32_t s = (__ffs(pm) - 1u) ^ nt.oct;
+                        pm &= pm - 1u;
+                        if (pm) st.set_top_y((e.y & 0xFFFFFF00u) | pm);
+                        else st.drop();
+                        cur = inner_child(e.x & 0x00FFFFFFu, e.x >> 24, s);
+                        break;
                     }
                     if (cur == KP_INVALID) done = true;
                 }
@@ -359,10 +439,14 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
                     done = true;
                 }
                 if (done) {
-                    float4 h;
-                    if (best_p == KP_INVALID) h = make_float4(__int_as_float(0x7f800000), __uint_as_float(KP_INVALID), 0.0f, 0.0f);
-                    else h = make_float4(best_t, __uint_as_float(best_p), bu, bv);
-                    args.hits[ray_id] = h;
+                    if (kAny) {
+                        args.occl[ray_id] = occl ? 1u : 0u;
+                    } else {
+                        float4 hh;
+                        if (best_p == KP_INVALID) hh = make_float4(__int_as_float(0x7f800000), __uint_as_float(KP_INVALID), 0.0f, 0.0f);
+                        else hh = make_float4(best_t, __uint_as_float(best_p), bu, bv);
+                        args.hits[ray_id] = hh;
+                    }
                     ray_id = KP_INVALID;
                     break;
                 }
@@ -376,90 +460,13 @@ __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
     }
 }
 
-__global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
-    __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
-    const int tid = threadIdx.x;
-    const int lane = __lane_id();
-    const uint32_t gl = blockIdx.x * KP_TRACE_BLOCK + tid;
-    Stack<kStack> st;
-    st.lds = lds;
-    st.spill = args.spill + (size_t)gl * args.spill_cap;
-    st.cap = args.spill_cap;
-    st.sp = 0;
-    st.tid = tid;
-    const uint4* __restrict__ nodes = args.nodes;
-    const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n;   // device-side count for queues
-    const float4* __restrict__ tris = args.tris;
+template <bool kCount>
+__global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
+    traverse<false, kCount>(args);
+}
 
-    uint32_t ray_id = KP_INVALID;
-    RayPre r;
-    uint32_t cur = KP_INVALID, steps = 0;
-    (void)lane;
-    RayFeeder feed;
-    feed.init(args.counter, args.rays, args.roots, n_rays);
-    for (;;) {
-        if (ray_id == KP_INVALID && feed.pending) {
-            ray_id = feed.pid;
-            setup_ray(feed.p0, feed.p1, r);
-            cur = ray_valid(feed.p0, feed.p1) ? feed.proot : KP_INVALID;
-            st.sp = 0;
-            steps = 0;
-            feed.pending = false;
-        }
-        feed.assign();
-        if (__ballot(ray_id != KP_INVALID || feed.pending) == 0) break;
-        if (ray_id != KP_INVALID) {
-            for (;;) {
-                bool hit = false, done = false;
-                if (cur == KP_INVALID) {
-                    done = true;   // invalid ray: reported as not occluded
-                } else if (cur & KP_LEAF_FLAG) {
-                    const uint32_t first = (cur & ~KP_LEAF_FLAG) >> 3;
-                    const uint32_t count = (cur & 7u) + 1u;
-                    for (uint32_t i = 0; i < count && !hit; ++i) {
-                        const float4* tp = tris + 3 * (size_t)(first + i);
-                        float t, u, v;
-                        hit = tri_test(r, tp[0], tp[1], tp[2], r.tmax, t, u, v);
-                    }
-                    cur = KP_INVALID;
-                } else {
-                    NodeRegs n;
-                    load_node(nodes, cur, n);
-                    uint32_t key[8];
-                    test_children(r, n, r.tmax, key);
-                    cur = KP_INVALID;
-                    bool ok = true;
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        if (key[s] == 0xFFFFFFFFu) continue;
-                        const uint32_t ref = child_ref(n, (uint32_t)s);
-                        if (cur == KP_INVALID) cur = ref;
-                        else ok = ok && st.push(ref, 0u);
-                    }
-                    if (!ok) {
-                        atomicOr(args.err, 1u);
-                        hit = true;   // report occluded rather than a false "visible"
-                    }
-                }
-                if (hit) done = true;
-                if (!done && cur == KP_INVALID) {
-                    if (st.sp > 0) cur = st.pop().x;
-                    else done = true;
-                }
-                if (!done && ++steps > args.max_steps) {
-                    atomicOr(args.err, 2u);
-                    hit = true;
-                    done = true;
-                }
-                if (done) {
-                    args.occl[ray_id] = hit ? 1u : 0u;
-                    ray_id = KP_INVALID;
-                    break;
-                }
-                if (__popcll(__ballot(true)) < args.refill_below) break;
-            }
-        }
-    }
+__global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
+    traverse<true, false>(args);
 }
 
 __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,

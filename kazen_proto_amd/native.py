@@ -79,7 +79,7 @@ class BatchTuning(ctypes.Structure):
 class GuidingDesc(ctypes.Structure):
     _fields_ = [("bsdf_fraction", ctypes.c_float), ("min_alpha", ctypes.c_float),
                 ("lr", ctypes.c_float), ("train_steps", _u32), ("guide_after", _u32),
-                ("max_records", _u32), ("pass_paths", _u32), ("reserved", _u32),
+                ("max_records", _u32), ("pass_paths", _u32), ("train_passes", _u32),
                 ("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3)]
 
 
@@ -446,8 +446,8 @@ class Context:
                                                 in6.shape[0], lr, int(apply), _np_ptr(g)))
         return g
 
-    def guiding_configure(self, bsdf_fraction=0.5, min_alpha=0.1, lr=1e-3, train_steps=1, guide_after=1,
-                          max_records=1 << 18, pass_paths=1 << 18, box=None):
+    def guiding_configure(self, bsdf_fraction=0.5, min_alpha=0.5, lr=1e-3, train_steps=1, guide_after=1,
+                          max_records=1 << 18, pass_paths=1 << 18, box=None, train_passes=0):
         """Guided-render settings (kp_guiding_configure); box = (lo, hi) or None for the scene's bounds."""
         d = GuidingDesc()
         d.bsdf_fraction = bsdf_fraction
@@ -457,6 +457,7 @@ class Context:
         d.guide_after = guide_after
         d.max_records = max_records
         d.pass_paths = pass_paths
+        d.train_passes = train_passes
         if box is not None:
             d.bmin[:] = [float(x) for x in box[0]]
             d.bmax[:] = [float(x) for x in box[1]]

@@ -429,6 +429,15 @@ inline bool bvh_occluded_one(const or_bvh& b, const float* ray) {
 }  // namespace
 
 // ================================================================ C API ====
+namespace {
+// Value of a binary16 bit pattern holding a non-negative integer (BVH8 planes).
+double half_int(uint16_t h) {
+    const int e = (h >> 10) & 0x1F, m = h & 0x3FF;
+    if (e == 0) return std::ldexp((double)m, -24);
+    return std::ldexp(1.0 + m / 1024.0, e - 15);
+}
+}  // namespace
+
 extern "C" {
 
 void or_pcg32_u32(uint64_t initstate, uint64_t initseq, uint32_t* out, int n) {
@@ -541,84 +550,119 @@ void or_bvh8_closest(const void* nodes_v, uint64_t n_nodes, const void* tris_v, 
                      const float* rays, uint32_t n_rays, float* hits, uint64_t* nodes_visited,
                      uint64_t* tris_tested, int threads) {
     // Node record: p[3] f32 | e[3] i8, imask u8 | child_base u32 | prim_base u32
-    //              | meta[8] u8 | q[6][8] u16 (lo.xyz, hi.xyz planes).
+    //              | meta[8] u8 | q[6][8] binary16 integers (lo.xyz, hi.xyz planes).
+    // Visit order of the HIP kernel (kazen_proto_amd/csrc/trace.hip traverse):
+    // hit leaf children at once, the nearest hit inner child next, the other
+    // hit inner children as one group taken in octant order and dropped when
+    // the second-nearest entry distance exceeds the closest hit.  Child boxes
+    // are tested exactly in FP64 (the kernel's FP32 test is conservative), so
+    // the counts can differ slightly from the kernel's.
     const uint8_t* nodes = (const uint8_t*)nodes_v;
     const float* tris = (const float*)tris_v;
     std::atomic<uint64_t> tot_nodes{0}, tot_tris{0};
     parallel_chunks(n_rays, threads, 256, [&](uint32_t s, uint32_t e) {
         uint64_t ln = 0, lt = 0;
-        std::vector<std::pair<uint32_t, double>> stack;
+        struct Group { uint32_t base, imask, pm; float bound; };
+        std::vector<Group> stack;
         for (uint32_t i = s; i < e; ++i) {
             const float* ray = rays + 8 * (size_t)i;
             if (!ray_valid(ray)) { write_miss(hits + 4 * (size_t)i); continue; }
             WRay r = make_wray(ray);
+            uint32_t oct = 0;
+            for (int k = 0; k < 3; ++k)
+                if (std::signbit(r.d[k])) oct |= 1u << k;
             float best_t = r.tmax, bu = 0, bv = 0;
             uint32_t best_p = 0xFFFFFFFFu;
             stack.clear();
-            stack.push_back({0u, -INFINITY});
-            while (!stack.empty()) {
-                auto [ref, tn] = stack.back();
-                stack.pop_back();
-                if (tn > (double)best_t) continue;
-                if (ref & 0x80000000u) {
-                    uint32_t first = (ref & 0x7FFFFFFFu) >> 3, count = (ref & 7u) + 1u;
-                    for (uint32_t k = 0; k < count; ++k) {
-                        if (first + k >= n_tris) continue;
-                        const float* tr = tris + 12 * (size_t)(first + k);
-                        ++lt;
-                        float t, u, v;
-                        if (wt_hit(r, tr, tr + 4, tr + 8, best_t, t, u, v)) {
-                            uint32_t p;
-                            std::memcpy(&p, &tr[3], 4);
-                            if (t < best_t || p < best_p) { best_t = t; best_p = p; bu = u; bv = v; }
+            uint32_t cur = 0;
+            for (;;) {
+                if (cur != 0xFFFFFFFFu && cur < n_nodes) {
+                    ++ln;
+                    const uint8_t* nd = nodes + 128 * (size_t)cur;
+                    cur = 0xFFFFFFFFu;
+                    float p[3];
+                    std::memcpy(p, nd, 12);
+                    int ex[3] = {(int8_t)nd[12], (int8_t)nd[13], (int8_t)nd[14]};
+                    const uint32_t imask = nd[15];
+                    uint32_t child_base, prim_base;
+                    std::memcpy(&child_base, nd + 16, 4);
+                    std::memcpy(&prim_base, nd + 20, 4);
+                    const uint8_t* meta = nd + 24;
+                    uint16_t q[6][8];
+                    std::memcpy(q, nd + 32, 96);
+                    uint32_t hitm = 0, k1 = 0xFFFFFFFFu, k2 = 0xFFFFFFFFu;
+                    for (int sl = 0; sl < 8; ++sl) {
+                        if (meta[sl] == 0) continue;
+                        double tn0 = (double)r.tmin, tf0 = (double)best_t;
+                        for (int k = 0; k < 3; ++k) {
+                            const double sc = std::ldexp(1.0, ex[k]);
+                            const double lo = (double)p[k] + half_int(q[k][sl]) * sc;
+                            const double hi = (double)p[k] + half_int(q[3 + k][sl]) * sc;
+                            const double d = (double)r.d[k], o = (double)r.o[k];
+                            if (d == 0.0) {
+                                if (o < lo || o > hi) { tn0 = 1.0; tf0 = 0.0; }
+                                continue;
+                            }
+                            const double t0 = (lo - o) / d, t1 = (hi - o) / d;
+                            tn0 = std::max(tn0, std::min(t0, t1));
+                            tf0 = std::min(tf0, std::max(t0, t1));
+                        }
+                        if (!(tn0 <= tf0)) continue;
+                        hitm |= 1u << sl;
+                        if ((imask >> sl) & 1u) {
+                            const float tnf = (float)tn0;
+                            uint32_t bits;
+                            std::memcpy(&bits, &tnf, 4);
+                            const uint32_t key = (tnf > 0.0f ? (bits & 0xFFFFFFF8u) : 0u) | (uint32_t)sl;
+                            k2 = std::min(k2, std::max(k1, key));
+                            k1 = std::min(k1, key);
                         }
                     }
-                    continue;
-                }
-                if (ref >= n_nodes) continue;
-                ++ln;
-                const uint8_t* nd = nodes + 128 * (size_t)ref;
-                float p[3];
-                std::memcpy(p, nd, 12);
-                int ex[3] = {(int8_t)nd[12], (int8_t)nd[13], (int8_t)nd[14]};
-                uint8_t imask = nd[15];
-                uint32_t child_base, prim_base;
-                std::memcpy(&child_base, nd + 16, 4);
-                std::memcpy(&prim_base, nd + 20, 4);
-                const uint8_t* meta = nd + 24;
-                uint16_t q[6][8];
-                std::memcpy(q, nd + 32, 96);
-                std::vector<std::pair<double, uint32_t>> hitc;
-                for (int sl = 0; sl < 8; ++sl) {
-                    if (meta[sl] == 0) continue;
-                    double tn0 = (double)r.tmin, tf0 = (double)best_t;
-                    for (int k = 0; k < 3; ++k) {
-                        double sc = std::ldexp(1.0, ex[k]);
-                        double lo = (double)p[k] + q[k][sl] * sc, hi = (double)p[k] + q[3 + k][sl] * sc;
-                        double d = (double)r.d[k], o = (double)r.o[k];
-                        if (d == 0.0) {
-                            if (o < lo || o > hi) { tn0 = 1.0; tf0 = 0.0; }
-                            continue;
+                    // leaf children in slot order
+                    for (int sl = 0; sl < 8; ++sl) {
+                        if (!((hitm >> sl) & 1u) || ((imask >> sl) & 1u)) continue;
+                        const uint32_t first = prim_base + (meta[sl] & 31u), count = meta[sl] >> 5;
+                        for (uint32_t k = 0; k < count; ++k) {
+                            if (first + k >= n_tris) continue;
+                            const float* tr = tris + 12 * (size_t)(first + k);
+                            ++lt;
+                            float t, u, v;
+                            if (wt_hit(r, tr, tr + 4, tr + 8, best_t, t, u, v)) {
+                                uint32_t pr;
+                                std::memcpy(&pr, &tr[3], 4);
+                                if (t < best_t || pr < best_p) { best_t = t; best_p = pr; bu = u; bv = v; }
+                            }
                         }
-                        double t0 = (lo - o) / d, t1 = (hi - o) / d;
-                        tn0 = std::max(tn0, std::min(t0, t1));
-                        tf0 = std::min(tf0, std::max(t0, t1));
                     }
-                    if (!(tn0 <= tf0)) continue;
-                    uint32_t ref2;
-                    if ((imask >> sl) & 1u) {
-                        uint32_t rank = (uint32_t)__builtin_popcount((unsigned)(imask & ((1u << sl) - 1u)));
-                        ref2 = child_base + rank;
-                    } else {
-                        uint32_t first = prim_base + (meta[sl] & 31u);
-                        uint32_t count = meta[sl] >> 5;
-                        ref2 = 0x80000000u | (first << 3) | (count - 1u);
+                    const uint32_t inner = hitm & imask;
+                    if (inner) {
+                        const uint32_t s1 = k1 & 7u;
+                        const uint32_t rest = inner & ~(1u << s1);
+                        if (rest) {
+                            uint32_t pm = 0;
+                            for (uint32_t sl = 0; sl < 8; ++sl)
+                                if ((rest >> sl) & 1u) pm |= 1u << (sl ^ oct);
+                            uint32_t bb = (k2 < 8u) ? 0xFF800000u : (k2 & 0xFFFFFF00u);
+                            float bound;
+                            std::memcpy(&bound, &bb, 4);
+                            stack.push_back({child_base, imask, pm, bound});
+                        }
+                        cur = child_base + (uint32_t)__builtin_popcount(imask & ((1u << s1) - 1u));
                     }
-                    hitc.push_back({tn0, ref2});
                 }
-                std::sort(hitc.begin(), hitc.end(),
-                          [](const auto& a, const auto& b) { return a.first > b.first; });
-                for (auto& h : hitc) stack.push_back({h.second, h.first});
+                if (cur == 0xFFFFFFFFu) {
+                    while (!stack.empty()) {
+                        Group& g = stack.back();
+                        if (g.bound > best_t) { stack.pop_back(); continue; }
+                        const uint32_t rank = (uint32_t)__builtin_ctz(g.pm);
+                        const uint32_t sl = rank ^ oct;
+                        g.pm &= g.pm - 1u;
+                        cur = g.base + (uint32_t)__builtin_popcount(g.imask & ((1u << sl) - 1u));
+                        if (g.pm == 0) stack.pop_back();
+                        break;
+                    }
+                    if (cur == 0xFFFFFFFFu) break;
+                }
             }
             float* h = hits + 4 * (size_t)i;
             if (best_p == 0xFFFFFFFFu) write_miss(h);

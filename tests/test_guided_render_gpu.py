@@ -92,23 +92,29 @@ def _slit_scene(res, spp):
 
 
 def test_trained_guide_beats_bsdf_sampling_at_equal_spp():
-    res, spp = 96, 64
+    # Ablation of the learned part: train the network online during a
+    # 64-spp guided render, freeze it, then render fresh samples with it and
+    # compare against plain BSDF sampling at the same sample count.
+    res, spp = 96, 256
     s = _slit_scene(res, spp)
     ref_ctx = native.Context(0)
     ref_ctx.load_scene(s)
-    ref = ref_ctx.render(desc(s, spp=4096, seed=99)).astype(np.float64)
-    plain = ref_ctx.render(desc(s, spp=spp))
+    ref = ref_ctx.render(desc(s, spp=8192, seed=99)).astype(np.float64)
+    plain = ref_ctx.render(desc(s, spp=spp, seed=5))
     ref_ctx.close()
-    c = guided_ctx(s, bsdf_fraction=0.5, min_alpha=0.1, lr=2e-3, train_steps=4, guide_after=4,
-                   pass_paths=res * res)
-    guided = c.render(desc(s, spp=spp, guided=True))
+    cfg = dict(bsdf_fraction=0.5, min_alpha=0.5, lr=5e-3, train_steps=16, guide_after=4, pass_paths=res * res)
+    c = guided_ctx(s, **cfg)
+    c.render(desc(s, spp=64, guided=True, seed=3))
     st = c.render_stats()
+    c.guiding_configure(**dict(cfg, train_steps=0, guide_after=0))
+    guided = c.render(desc(s, spp=spp, guided=True, seed=5))
     c.close()
     e_plain, e_guided = rel_mse(plain, ref), rel_mse(guided, ref)
-    print("slit room %dx%d @%d spp: relMSE plain %.4f guided %.4f (train steps %d, records %d)"
-          % (res, res, spp, e_plain, e_guided, st["train_steps"], st["train_records"]))
+    print("slit room %dx%d @%d spp: relMSE plain %.4f guided (frozen after %d steps, %d records) %.4f"
+          % (res, res, spp, e_plain, st["train_steps"], st["train_records"], e_guided))
+    assert st["train_steps"] == 64 * 16 and st["train_records"] > 0
     assert np.isfinite(guided).all()
-    assert e_guided < 0.85 * e_plain
+    assert e_guided < 0.9 * e_plain
 
 
 def test_slit_room_gpu_reference_matches_oracle():

@@ -40,7 +40,9 @@ def _decode(nodes):
     imask = n[:, 15]
     base = n[:, 16:24].copy().view(np.uint32)
     meta = n[:, 24:32]
-    q = n[:, 32:128].copy().view(np.uint16).reshape(-1, 6, 8).astype(np.float64)
+    # planes are binary16 integers 0..2047 (exact)
+    q = n[:, 32:128].copy().view(np.float16).reshape(-1, 6, 8).astype(np.float64)
+    assert np.all(q == np.round(q)) and q.min() >= 0 and q.max() <= 2047
     sc = np.ldexp(1.0, e)
     lo = p[:, :, None].astype(np.float64) + q[:, 0:3] * sc[:, :, None]
     hi = p[:, :, None].astype(np.float64) + q[:, 3:6] * sc[:, :, None]
