@@ -83,7 +83,7 @@ def guiding_equal_error(args):
             c.guiding_init(args.seed_guide)
             c.guiding_configure(bsdf_fraction=0.5, min_alpha=0.5, lr=args.guiding_lr,
                                 train_steps=args.guiding_steps, guide_after=args.guiding_after,
-                                max_records=1 << 18, pass_paths=1 << 18,
+                                max_records=1 << 18, pass_paths=args.guiding_pass_paths,
                                 train_passes=args.guiding_train_passes)
         c.film_reset(res, res)
         spp = 0
@@ -244,10 +244,12 @@ def main():
     ap.add_argument("--guiding-target", type=float, default=0.01)
     ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
     ap.add_argument("--guiding-lr", type=float, default=2e-3)
-    ap.add_argument("--guiding-steps", type=int, default=4, help="Adam steps per pass")
+    ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
+    ap.add_argument("--guiding-pass-paths", type=int, default=1 << 21,
+                    help="paths per guided pass (each pass trains once)")
     ap.add_argument("--guiding-train-passes", type=int, default=64,
                     help="passes that train the network before it is frozen (0 = all)")
-    ap.add_argument("--guiding-after", type=int, default=4, help="training passes before guided sampling")
+    ap.add_argument("--guiding-after", type=int, default=2, help="training passes before guided sampling")
     ap.add_argument("--seed-guide", type=int, default=17)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -121,16 +121,14 @@ __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) 
 // loads its own near and far planes instead of selecting them per child.
 struct NodeTest {
     uint32_t oct;
-    uint32_t wn[3], wf[3];
+    uint32_t words;   // 4 bits each: near x, far x, near y, far y, near z, far z
 };
 
 __device__ __forceinline__ NodeTest node_test_setup(const RayPre& r) {
     NodeTest t;
     const bool nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
     t.oct = (nx ? 1u : 0u) | (ny ? 2u : 0u) | (nz ? 4u : 0u);
-    t.wn[0] = nx ? 5u : 2u; t.wf[0] = nx ? 2u : 5u;
-    t.wn[1] = ny ? 6u : 3u; t.wf[1] = ny ? 3u : 6u;
-    t.wn[2] = nz ? 7u : 4u; t.wf[2] = nz ? 4u : 7u;
+    t.words = (nx ? 0x25u : 0x52u) | ((ny ? 0x36u : 0x63u) << 8) | ((nz ? 0x47u : 0x74u) << 16);
     return t;
 }
 
@@ -178,9 +176,10 @@ __device__ __forceinline__ void visit_node(const uint4* __restrict__ nodes, uint
     const uint4* p = nodes + (size_t)idx * 8;
     h.w0 = p[0];
     h.w1 = p[1];
-    const uint4 nx = p[nt.wn[0]], fx = p[nt.wf[0]];
-    const uint4 ny = p[nt.wn[1]], fy = p[nt.wf[1]];
-    const uint4 nz = p[nt.wn[2]], fz = p[nt.wf[2]];
+    const uint32_t w = nt.words;
+    const uint4 nx = p[w & 0xFu], fx = p[(w >> 4) & 0xFu];
+    const uint4 ny = p[(w >> 8) & 0xFu], fy = p[(w >> 12) & 0xFu];
+    const uint4 nz = p[(w >> 16) & 0xFu], fz = p[(w >> 20) & 0xFu];
     const float px = __uint_as_float(h.w0.x), py = __uint_as_float(h.w0.y), pz = __uint_as_float(h.w0.z);
     const uint32_t e = h.w0.w;
     const int ex = (int)(int8_t)(e & 0xFF), ey = (int)(int8_t)((e >> 8) & 0xFF), ez = (int)(int8_t)((e >> 16) & 0xFF);

@@ -144,6 +144,18 @@ def test_terrain_matches_oracle_statistics():
     assert e < 1.6 * e0
 
 
+def test_ingested_xml_scene_per_path_agreement(tmp_path):
+    # a scene in the reference's XML / OBJ format, through the loader
+    from kazen_proto_amd import scene_xml
+    from test_scene_xml import write_scene
+    s = scene_xml.load_xml_scene(write_scene(str(tmp_path)), sphere_lon=16, sphere_lat=8)
+    s.spp, s.max_depth, s.rr_depth, s.seed = 1, 5, 3, 2
+    frac, g, o = _per_path(s)
+    print("ingested xml scene: %.4f of pixels agree path by path" % frac)
+    assert frac >= 0.95
+    assert scene_xml.orient_film(g, s).shape == g.shape
+
+
 def test_film_accumulate_resumes_bit_for_bit_samples():
     s = scenes.cornell_box(width=32, height=32, spp=8)
     c = gpu_ctx(s)

@@ -1,0 +1,161 @@
+"""Scene ingest (kazen_proto_amd/scene_xml.py): the reference's XML scene
+format and OBJ meshes, checked on a scene written here in that format and,
+where the reference tree is present, on its own Cornell-box scene files."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from kazen_proto_amd import scene_xml, scenes
+
+REF_SCENES = "/root/reference/resource/scene"
+
+XML = """<Scene>
+    <Film width="int 40" height="int 30"/>
+    <Camera position="float3 0 0 15" lookat="float3 0 0 -10" up="float3 0 1 0" fov="float 60" scale="float3 -1 1 1"/>
+    <Materials>
+        <ShaderGroupBegin name="red"/>
+            <Shader name="matte" layer="layer1"><Parameter Cs="float3 0.7 0.1 0.1"/></Shader>
+        <ShaderGroupEnd/>
+        <ShaderGroupBegin name="white"/>
+            <Shader name="diffuse" layer="layer2"><Parameter Cs="float3 0.8 0.8 0.8"/></Shader>
+        <ShaderGroupEnd/>
+        <ShaderGroupBegin name="chrome"/>
+            <Shader name="mirror" layer="layer3"><Parameter Cs="float3 0.9 0.9 0.9"/></Shader>
+        <ShaderGroupEnd/>
+        <ShaderGroupBegin name="glass"/>
+            <Shader name="dielectric" layer="layer4"><Parameter eta="float 1.33"/></Shader>
+        <ShaderGroupEnd/>
+        <ShaderGroupBegin name="glossy"/>
+            <Shader name="rough" layer="layer5"><Parameter kd="float3 0.5 0.5 0.5"/><Parameter alpha="float 0.1"/></Shader>
+        <ShaderGroupEnd/>
+        <ShaderGroupBegin name="lamp"/>
+            <Shader name="emitter" layer="layer6"><Parameter Cs="float3 12 12 12"/></Shader>
+        <ShaderGroupEnd/>
+        <ShaderGroupBegin name="skin"/>
+            <Shader name="wax" layer="layer7"><Parameter Cs="float3 0.6 0.5 0.4"/></Shader>
+        <ShaderGroupEnd/>
+    </Materials>
+    <Objects>
+        <Triangle verta="float3 -10 -10 0" vertb="float3 -10 -10 -20" vertc="float3 -10 10 -20" shader_name="string red"/>
+        <Triangle verta="float3 -10 10 -20" vertb="float3 -10 10 0" vertc="float3 -10 -10 0" shader_name="string red"/>
+        <Triangle verta="float3 -10 -10 -20" vertb="float3 -10 -10 0" vertc="float3 10 -10 -20" shader_name="string white"/>
+        <Triangle verta="float3 10 -10 -20" vertb="float3 -10 -10 0" vertc="float3 10 -10 0" shader_name="string white"/>
+        <Triangle verta="float3 -10 10 -20" vertb="float3 -10 -10 -20" vertc="float3 10 10 -20" shader_name="string white"/>
+        <Triangle verta="float3 10 10 -20" vertb="float3 -10 -10 -20" vertc="float3 10 -10 -20" shader_name="string white"/>
+        <Sphere center_n_radius="float4 0 0 0 4" translate="func_translate -5 -6 -10" shader_name="string chrome"/>
+        <Sphere center_n_radius="float4 0 0 0 2.5" translate="func_translate 5 -7.5 -8" shader_name="string glass"/>
+        <Mesh filename="string box.obj" shader_name="glossy" translate="float3 0 -9 -15" rotate="float4 0 1 0 1.5707963" scale="float3 2 1 1"/>
+        <Mesh filename="string box.obj" shader_name="skin" translate="float3 6 -9 -16"/>
+        <Quad center="float3 0 9.5 -10" dir="float3 0 -1 0" vertical_vec="float3 0 0 1" horizontal_vec="float3 1 0 0" half_width="float 2" half_height="float 1" shader_name="string lamp" is_light="bool true"/>
+    </Objects>
+    <PathIntegrator/>
+</Scene>
+"""
+
+# unit cube, quads fanned into triangles, 1-based and negative indices mixed
+OBJ = """# cube
+v -1 -1 -1
+v 1 -1 -1
+v 1 1 -1
+v -1 1 -1
+v -1 -1 1
+v 1 -1 1
+v 1 1 1
+v -1 1 1
+f 1 2 3 4
+f 5/1 8/1 7/1 6/1
+f 1 5 6 2
+f 4 3 7 8
+f -8 -4 -1 -5
+f 2/1/1 6/1/1 7/1/1 3/1/1
+"""
+
+
+def write_scene(d):
+    p = os.path.join(d, "scene.xml")
+    with open(p, "w") as f:
+        f.write(XML)
+    with open(os.path.join(d, "box.obj"), "w") as f:
+        f.write(OBJ)
+    return p
+
+
+def test_parse_own_scene(tmp_path):
+    s = scene_xml.load_xml_scene(write_scene(str(tmp_path)), sphere_lon=16, sphere_lat=8)
+    assert (s.width, s.height) == (40, 30)
+    assert s.flip_x
+    # horizontal 60 degrees -> vertical with tan(v/2) = tan(30 deg) * 30/40
+    assert math.isclose(math.tan(math.radians(s.fov_deg) / 2), math.tan(math.radians(30)) * 0.75, rel_tol=1e-9)
+    assert s.eye == (0.0, 0.0, 15.0) and s.lookat == (0.0, 0.0, -10.0)
+    sphere = 16 * 8 * 2 - 16
+    assert s.n_tris == 6 + 2 * sphere + 2 * 12 + 2
+    by_name = {}
+    for tris, mi in zip(s.meshes, s.mesh_material):
+        by_name.setdefault(mi, []).append(tris)
+    mats = s.materials
+    kinds = sorted((m["type"], m["dist"]) for m in mats)
+    assert len(mats) == 7
+    lamp = [m for m in mats if m["Le"][0] > 0][0]
+    assert lamp["Le"] == (12.0, 12.0, 12.0)
+    glass = [m for m in mats if m["type"] == scenes.DIELECTRIC][0]
+    assert math.isclose(glass["eta"], 1.33) and glass["alpha"] <= 1e-3
+    glossy = [m for m in mats if m["type"] == scenes.CONDUCTOR and m["dist"] == scenes.BECKMANN][0]
+    assert math.isclose(glossy["alpha"], 0.1) and math.isclose(glossy["F0"][0], 0.02)
+    chrome = [m for m in mats if m["type"] == scenes.CONDUCTOR and m["dist"] == scenes.GGX][0]
+    assert math.isclose(chrome["F0"][0], 0.9)
+    assert any("wax" in w for w in s.warnings)
+    assert kinds.count((scenes.LAMBERT, scenes.GGX)) == 4   # red, white, lamp, wax
+    # the sphere translate and the cube's scale-rotate-translate chain
+    sph = s.meshes[6].reshape(-1, 3)
+    assert np.allclose(sph.mean(0), (-5, -6, -10), atol=0.5)
+    assert np.allclose(np.linalg.norm(sph - (-5, -6, -10), axis=1), 4.0, atol=1e-5)
+    cube = s.meshes[8].reshape(-1, 3)
+    # scale x by 2, then rotate 90 degrees about y (x -> -z), then translate
+    lo, hi = cube.min(0), cube.max(0)
+    assert np.allclose(lo, (-1, -10, -17), atol=1e-5) and np.allclose(hi, (1, -8, -13), atol=1e-5)
+    # the light quad faces its dir (down) and spans 4 x 2
+    q = s.meshes[10].reshape(-1, 3, 3)
+    n = np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0])
+    assert np.all(n[:, 1] < 0)
+    assert math.isclose(0.5 * np.linalg.norm(n, axis=1).sum(), 8.0, rel_tol=1e-5)
+
+
+def test_obj_faces_are_fanned(tmp_path):
+    p = os.path.join(str(tmp_path), "box.obj")
+    with open(p, "w") as f:
+        f.write(OBJ)
+    t = scene_xml.load_obj(p)
+    assert t.shape == (12, 9)
+    v = t.reshape(-1, 3, 3)
+    area = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+    assert np.allclose(area, 2.0)
+
+
+def test_own_scene_renders_with_oracle(tmp_path):
+    s = scene_xml.load_xml_scene(write_scene(str(tmp_path)), sphere_lon=16, sphere_lat=8)
+    s.spp, s.max_depth, s.rr_depth, s.seed = 8, 5, 3, 1
+    mats = [oracle.material(m["type"], m["dist"], m["albedo"], m["F0"], m["alpha"], m["eta"], m["Le"])
+            for m in s.materials]
+    sc = oracle.Scene(s.soup(), s.tri_material(), s.tri_light(), mats)
+    d = oracle.RenderDesc()
+    d.width, d.height, d.spp, d.max_depth, d.rr_depth, d.seed = s.width, s.height, 8, 5, 3, 1
+    d.eye[:], d.lookat[:], d.up[:] = s.eye, s.lookat, s.up
+    d.tan_half_fov = s.tan_half_fov
+    img = sc.render(d)
+    assert np.isfinite(img).all() and img.mean() > 0.01
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="reference scene files not present")
+def test_reference_cornell_mis_scene():
+    s = scene_xml.load_xml_scene(os.path.join(REF_SCENES, "cbox", "cbox_mis.xml"))
+    assert (s.width, s.height) == (800, 600) and s.flip_x
+    assert len(s.meshes) == 6 and len(s.materials) == 6
+    types = [s.materials[m]["type"] for m in s.mesh_material]
+    assert types == [scenes.LAMBERT, scenes.LAMBERT, scenes.LAMBERT, scenes.CONDUCTOR, scenes.DIELECTRIC,
+                     scenes.LAMBERT]
+    assert s.materials[s.mesh_material[5]]["Le"] == (40.0, 40.0, 40.0)
+    assert s.meshes[5].shape[0] == 2 and s.n_tris > 2000
+    assert not s.warnings
