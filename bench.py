@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
     ap.add_argument("--guiding-pass-paths", type=int, default=1 << 21,
                     help="paths per guided pass (each pass trains once)")
-    ap.add_argument("--guiding-train-passes", type=int, default=64,
+    ap.add_argument("--guiding-train-passes", type=int, default=256,
                     help="passes that train the network before it is frozen (0 = all)")
     ap.add_argument("--guiding-after", type=int, default=2, help="training passes before guided sampling")
     ap.add_argument("--seed-guide", type=int, default=17)
