@@ -319,12 +319,48 @@ struct RayFeeder {
     }
 };
 
+// Lanes holding postponed leaf children before the wave tests them together.
+constexpr int kLeafBatch = 16;
+
+// Tests the leaf children `lm` of a node (prim_base, meta words m0 / m1):
+// closest hit updates best_*, any-hit sets occl and stops.
+template <bool kAny, bool kCount>
+__device__ __forceinline__ void test_leaves(const float4* __restrict__ tris, const RayPre& r, uint32_t lm,
+                                            uint32_t prim_base, uint32_t m0, uint32_t m1, float& best_t,
+                                            uint32_t& best_p, float& bu, float& bv, bool& occl,
+                                            uint64_t& cnt_tris) {
+    while (lm) {
+        const uint32_t s = __ffs(lm) - 1u;
+        lm &= lm - 1u;
+        const uint32_t mw = s < 4u ? m0 : m1;
+        const uint32_t meta = (mw >> ((s & 3u) * 8u)) & 0xFFu;
+        const uint32_t first = prim_base + (meta & 31u), count = meta >> 5;
+        if (kCount) cnt_tris += count;
+        for (uint32_t i = 0; i < count; ++i) {
+            const float4* tp = tris + 3 * (size_t)(first + i);
+            const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+            float t, u, v;
+            if (tri_test(r, ta, tb, tc, best_t, t, u, v)) {
+                if (kAny) {
+                    occl = true;
+                    return;
+                }
+                const uint32_t pr = __float_as_uint(ta.w);
+                if (t < best_t || pr < best_p) { best_t = t; best_p = pr; bu = u; bv = v; }
+            }
+        }
+    }
+}
+
 // Persistent traversal shared by closest hit (kAny = false) and occlusion
-// (kAny = true).  A node visit tests the 8 children; hit leaf children are
-// tested at once; the nearest hit inner child is visited next and the other
-// hit inner children are pushed as one group entry, taken later in octant
-// order (the builder puts children in octant slots) and skipped as a whole
-// when their nearest entry distance lies beyond the closest hit so far.
+// (kAny = true).  A node visit tests the 8 children; the nearest hit inner
+// child is visited next and the other hit inner children are pushed as one
+// group entry, taken later in octant order (the builder puts children in
+// octant slots) and skipped as a whole when their nearest entry distance lies
+// beyond the closest hit so far.  Hit leaf children are postponed (one set
+// per lane) and tested when at least kLeafBatch lanes of the wave hold some,
+// or a lane has nothing else to do, so triangle tests run with many lanes
+// active instead of one or two per iteration.
 template <bool kAny, bool kCount>
 __device__ __forceinline__ void traverse(const TraceArgs& args) {
     __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
@@ -345,6 +381,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
     NodeTest nt;
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     uint32_t best_p = KP_INVALID, cur = KP_INVALID, steps = 0;
+    uint32_t pend = 0u, pbase = 0u, pm0 = 0u, pm1 = 0u;   // postponed leaf children
     bool occl = false;
     uint64_t cnt_nodes = 0, cnt_tris = 0;
     RayFeeder feed;
@@ -359,6 +396,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
             best_p = KP_INVALID;
             bu = bv = 0.0f;
             occl = false;
+            pend = 0u;
             cur = ray_valid(feed.p0, feed.p1) ? feed.proot : KP_INVALID;   // root
             st.sp = 0;
             steps = 0;
@@ -368,39 +406,24 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
         if (__ballot(ray_id != KP_INVALID || feed.pending) == 0) break;
         if (ray_id != KP_INVALID) {
             for (;;) {
-                bool done = false;
+                bool fault = false;
                 if (cur != KP_INVALID) {
                     NodeHits h;
                     visit_node(nodes, cur, r, nt, best_t, h);
                     if (kCount) cnt_nodes++;
                     cur = KP_INVALID;
-                    // leaf children: (count << 5) | offset from prim_base
-                    uint32_t lm = h.leaves;
-                    while (lm) {
-                        const uint32_t s = __ffs(lm) - 1u;
-                        lm &= lm - 1u;
-                        const uint32_t mw = s < 4u ? h.w1.z : h.w1.w;
-                        const uint32_t meta = (mw >> ((s & 3u) * 8u)) & 0xFFu;
-                        const uint32_t first = h.w1.y + (meta & 31u), count = meta >> 5;
-                        if (kCount) cnt_tris += count;
-                        for (uint32_t i = 0; i < count; ++i) {
-                            const float4* tp = tris + 3 * (size_t)(first + i);
-                            const float4 ta = tp[0], tb = tp[1], tc = tp[2];
-                            float t, u, v;
-                            if (tri_test(r, ta, tb, tc, best_t, t, u, v)) {
-                                if (kAny) {
-                                    occl = true;
-                                    break;
-                                }
-                                const uint32_t pr = __float_as_uint(ta.w);
-                                if (t < best_t || pr < best_p) { best_t = t; best_p = pr; bu = u; bv = v; }
-                            }
+                    if (h.leaves) {
+                        if (pend == 0u) {
+                            pend = h.leaves;
+                            pbase = h.w1.y;
+                            pm0 = h.w1.z;
+                            pm1 = h.w1.w;
+                        } else {   // this lane already holds a set: test these now
+                            test_leaves<kAny, kCount>(tris, r, h.leaves, h.w1.y, h.w1.z, h.w1.w, best_t, best_p, bu,
+                                                      bv, occl, cnt_tris);
                         }
-                        if (kAny && occl) break;
                     }
-                    if (kAny && occl) {
-                        done = true;
-                    } else if (h.inner) {
+                    if (h.inner && !(kAny && occl)) {
                         const uint32_t imask = h.w0.w >> 24;
                         const uint32_t s1 = h.near_key & 7u;
                         const uint32_t rest = h.inner & ~(1u << s1);
@@ -409,13 +432,13 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                             const uint32_t gy = group_bound_bits(h.next_key) | oct_permute(rest, nt.oct);
                             if (!st.push(gx, gy)) {
                                 atomicOr(args.err, 1u);
-                                done = true;
+                                fault = true;
                             }
                         }
                         cur = inner_child(h.w1.x, imask, s1);
                     }
                 }
-                if (!done && cur == KP_INVALID) {
+                if (!fault && cur == KP_INVALID && !(kAny && occl)) {
                     while (st.sp > 0) {
                         const uint2 e = st.top();
                         const float bound = __uint_as_float(e.y & 0xFFFFFF00u);
@@ -431,8 +454,18 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                         cur = inner_child(e.x & 0x00FFFFFFu, e.x >> 24, s);
                         break;
                     }
-                    if (cur == KP_INVALID) done = true;
                 }
+                // postponed leaves: tested together once enough lanes hold some,
+                // or when a lane has no node left to visit
+                const bool stuck = pend != 0u && (cur == KP_INVALID || fault);
+                if (__ballot(stuck) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch) {
+                    if (pend != 0u && !(kAny && occl)) {
+                        test_leaves<kAny, kCount>(tris, r, pend, pbase, pm0, pm1, best_t, best_p, bu, bv, occl,
+                                                  cnt_tris);
+                    }
+                    pend = 0u;
+                }
+                bool done = fault || (kAny && occl) || (cur == KP_INVALID && pend == 0u);
                 if (!done && ++steps > args.max_steps) {
                     atomicOr(args.err, 2u);
                     done = true;
@@ -447,6 +480,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                         args.hits[ray_id] = hh;
                     }
                     ray_id = KP_INVALID;
+                    pend = 0u;
                     break;
                 }
                 if (__popcll(__ballot(true)) < args.refill_below) break;
