@@ -108,6 +108,32 @@ int kp_mesh_add(kp_ctx* ctx, const float* xyz, uint32_t n_verts, const uint32_t*
 int kp_bvh_build(kp_ctx* ctx, const kp_build_opts* opts);
 int kp_get_stats(kp_ctx* ctx, kp_stats* out);
 
+/* Instancing (two-level BVH). A prototype mesh is added once in object space
+ * with kp_blas_add and placed any number of times with kp_instance_add
+ * (row-major 3x4 object-to-world matrix, invertible). With instances present,
+ * kp_bvh_build builds one BVH8 per prototype (and one over the kp_mesh_add
+ * meshes, placed once with the identity) and a top-level BVH8 over the
+ * instances' world boxes; a ray enters each instance it reaches in the
+ * instance's object space (o' = M o + m, d' = M d in FP32, so t keeps its
+ * meaning). Primitive ids: the kp_mesh_add meshes first, then every
+ * instance's primitives in instance order. The integrator shades instances
+ * like any mesh (material per prototype, an emissive prototype makes each
+ * instance a light). Replaces Accelerator::add_instances
+ * (src/core/accel.h:28-31; EmbreeAccel::add_instances
+ * src/core/accel.cpp:317-346). */
+int kp_blas_add(kp_ctx* ctx, const float* xyz, uint32_t n_verts, const uint32_t* idx, uint32_t n_tris,
+                uint32_t* blas_id);
+int kp_blas_set_material(kp_ctx* ctx, uint32_t blas_id, uint32_t material_id);
+int kp_instance_add(kp_ctx* ctx, uint32_t blas_id, const float* xform12, uint32_t* instance_id);
+/* Removes every mesh, prototype and instance and the BVH (materials, film
+ * and guiding state stay), so a scene can be rebuilt from scratch.
+ * (The reference rebuilds by constructing a new Embree scene; no counterpart
+ * call.) */
+int kp_geometry_clear(kp_ctx* ctx);
+/* The prototype and both matrices of an instance (the world-to-object one is
+ * the FP32 matrix the traversal applies). Any output may be NULL. */
+int kp_instance_get(kp_ctx* ctx, uint32_t instance_id, uint32_t* blas_id, float* o2w12, float* w2o12);
+
 /* Queries ---------------------------------------------------------------- */
 /* Closest hit for n rays. Replaces HitableInterface::intersect(const Ray&,
  * Intersection&) / intersect(const Ray&, float&) (src/core/hitable.h:12-13)

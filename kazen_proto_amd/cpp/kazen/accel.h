@@ -6,6 +6,8 @@
 // batch of one; new callers should use the batched overloads, which are the
 // fast path (one kernel launch for the whole batch).
 #pragma once
+#include <functional>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -27,7 +29,19 @@ struct TriangleMesh {
     std::string shader_name;
     bool is_light = false;
     uint32_t light_id = 0;
+    uint32_t material_id = 0;   // closure-table entry in the context (set by SceneGPU::add_mesh)
 };
+
+inline Mat4f operator*(const Mat4f& a, const Mat4f& b) {
+    Mat4f c;
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            float acc = 0.0f;
+            for (int j = 0; j < 4; ++j) acc += a.m[r][j] * b.m[j][k];
+            c.m[r][k] = acc;
+        }
+    return c;
+}
 
 class Accelerator : public HitableInterface {
 public:
@@ -55,25 +69,87 @@ public:
     kp_ctx* context() const { return ctx_; }
     const std::vector<std::shared_ptr<TriangleMesh>>& meshes() const { return meshes_; }
 
-    void add_trianglemesh(std::shared_ptr<TriangleMesh>& t) override {
-        std::vector<float> xyz(3 * t->verts.size());
-        for (size_t i = 0; i < t->verts.size(); ++i)
-            for (int k = 0; k < 3; ++k) xyz[3 * i + k] = t->verts[i][k];
-        std::vector<uint32_t> idx(3 * t->indice.size());
-        for (size_t i = 0; i < t->indice.size(); ++i)
-            for (int k = 0; k < 3; ++k) idx[3 * i + k] = (uint32_t)t->indice[i][k];
-        float xf[12];
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 4; ++c) xf[4 * r + c] = t->local_to_world.m[r][c];
-        uint32_t id = 0;
-        check(kp_mesh_add(ctx_, xyz.data(), (uint32_t)t->verts.size(), idx.data(),
-                          (uint32_t)t->indice.size(), xf, &id));
-        first_prim_.push_back(total_tris_);
-        total_tris_ += (uint32_t)t->indice.size();
-        meshes_.push_back(t);
+    // Meshes are kept until build(), which decides how each is placed.
+    void add_trianglemesh(std::shared_ptr<TriangleMesh>& t) override { meshes_.push_back(t); }
+
+    // Names a group of meshes and groups placed with `trans` (EmbreeAccel::
+    // add_instances, src/core/accel.cpp:317-346: a subscene wrapped in an
+    // instance); with root = true the names are the scene root, as build().
+    void add_instances(const std::string& name, const std::vector<std::string>& names,
+                       const Mat4f& trans = Mat4f(), bool root = false) override {
+        if (root) {
+            build(names);
+            return;
+        }
+        groups_[name] = Group{names, trans};
     }
 
-    void build(const std::vector<std::string>&) override { check(kp_bvh_build(ctx_, nullptr)); }
+    // Places the named meshes and groups (all meshes when `names` is empty)
+    // and builds the BVH. A mesh placed once goes into the one-level soup
+    // under its full transform; a mesh placed several times becomes one
+    // prototype with an instance per placement (two-level traversal).
+    void build(const std::vector<std::string>& names) override {
+        std::vector<std::pair<uint32_t, Mat4f>> placed;
+        if (names.empty()) {
+            for (uint32_t i = 0; i < meshes_.size(); ++i) placed.push_back({i, meshes_[i]->local_to_world});
+        } else {
+            std::map<std::string, uint32_t> by_name;
+            for (uint32_t i = 0; i < meshes_.size(); ++i) by_name[meshes_[i]->name] = i;
+            std::function<void(const std::string&, const Mat4f&, int)> expand = [&](const std::string& n,
+                                                                                  const Mat4f& parent, int depth) {
+                if (depth > 64) throw KpError(KP_ERR_INVALID, "instance groups nest deeper than 64 (a cycle?)");
+                auto m = by_name.find(n);
+                if (m != by_name.end()) {
+                    placed.push_back({m->second, parent * meshes_[m->second]->local_to_world});
+                    return;
+                }
+                auto g = groups_.find(n);
+                if (g == groups_.end()) throw KpError(KP_ERR_INVALID, "build: unknown mesh or group " + n);
+                for (const auto& child : g->second.names) expand(child, parent * g->second.trans, depth + 1);
+            };
+            for (const auto& n : names) expand(n, Mat4f(), 0);
+        }
+        std::vector<uint32_t> uses(meshes_.size(), 0);
+        for (const auto& p : placed) uses[p.first]++;
+        check(kp_geometry_clear(ctx_));
+        places_.clear();
+        uint32_t next = 0;
+        for (const auto& p : placed) {
+            if (uses[p.first] != 1) continue;
+            const TriangleMesh& m = *meshes_[p.first];
+            std::vector<float> xyz;
+            std::vector<uint32_t> idx;
+            flatten(m, xyz, idx);
+            float xf[12];
+            rows(p.second, xf);
+            uint32_t id = 0;
+            check(kp_mesh_add(ctx_, xyz.data(), (uint32_t)m.verts.size(), idx.data(), (uint32_t)m.indice.size(), xf,
+                              &id));
+            check(kp_mesh_set_material(ctx_, id, m.material_id));
+            places_.push_back({next, p.first, p.second});
+            next += (uint32_t)m.indice.size();
+        }
+        std::vector<uint32_t> proto(meshes_.size(), 0xFFFFFFFFu);
+        for (const auto& p : placed) {
+            if (uses[p.first] < 2) continue;
+            const TriangleMesh& m = *meshes_[p.first];
+            if (proto[p.first] == 0xFFFFFFFFu) {
+                std::vector<float> xyz;
+                std::vector<uint32_t> idx;
+                flatten(m, xyz, idx);
+                check(kp_blas_add(ctx_, xyz.data(), (uint32_t)m.verts.size(), idx.data(), (uint32_t)m.indice.size(),
+                                  &proto[p.first]));
+                check(kp_blas_set_material(ctx_, proto[p.first], m.material_id));
+            }
+            float xf[12];
+            rows(p.second, xf);
+            check(kp_instance_add(ctx_, proto[p.first], xf, nullptr));
+            places_.push_back({next, p.first, p.second});
+            next += (uint32_t)m.indice.size();
+        }
+        if (places_.empty()) throw KpError(KP_ERR_INVALID, "build: nothing to place");
+        check(kp_bvh_build(ctx_, nullptr));
+    }
 
     bool intersect(const Ray& r, Intersection& isect) const override {
         std::vector<Intersection> out(1);
@@ -128,23 +204,37 @@ private:
         if (rc != KP_OK) throw KpError(rc, kp_last_error(ctx_));
     }
 
-    // Global primitive id -> (mesh, local id), then the fields post_hit fills
-    // (TriangleMesh::post_hit, src/core/shape.cpp:615-683, flat normals).
+    static void flatten(const TriangleMesh& m, std::vector<float>& xyz, std::vector<uint32_t>& idx) {
+        xyz.resize(3 * m.verts.size());
+        for (size_t i = 0; i < m.verts.size(); ++i)
+            for (int k = 0; k < 3; ++k) xyz[3 * i + k] = m.verts[i][k];
+        idx.resize(3 * m.indice.size());
+        for (size_t i = 0; i < m.indice.size(); ++i)
+            for (int k = 0; k < 3; ++k) idx[3 * i + k] = (uint32_t)m.indice[i][k];
+    }
+    static void rows(const Mat4f& x, float* xf) {
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) xf[4 * r + c] = x.m[r][c];
+    }
+
+    // Global primitive id -> (placement, mesh, local id), then the fields
+    // post_hit fills (TriangleMesh::post_hit, src/core/shape.cpp:615-683,
+    // flat normals) in world space.
     void fill(const Ray& r, const kp_hit& h, Intersection& is) const {
-        size_t lo = 0, hi = first_prim_.size();
+        size_t lo = 0, hi = places_.size();
         while (hi - lo > 1) {
             size_t mid = (lo + hi) / 2;
-            if (first_prim_[mid] <= h.prim) lo = mid;
+            if (places_[mid].first_prim <= h.prim) lo = mid;
             else hi = mid;
         }
-        const TriangleMesh& m = *meshes_[lo];
-        const uint32_t local = h.prim - first_prim_[lo];
+        const Place& pl = places_[lo];
+        const TriangleMesh& m = *meshes_[pl.mesh];
+        const uint32_t local = h.prim - pl.first_prim;
         Vec3f v[3];
         for (int k = 0; k < 3; ++k) {
             const Vec3f& p = m.verts[m.indice[local][k]];
             for (int rr = 0; rr < 3; ++rr)
-                v[k][rr] = m.local_to_world.m[rr][0] * p[0] + m.local_to_world.m[rr][1] * p[1] +
-                           m.local_to_world.m[rr][2] * p[2] + m.local_to_world.m[rr][3];
+                v[k][rr] = pl.xf.m[rr][0] * p[0] + pl.xf.m[rr][1] * p[1] + pl.xf.m[rr][2] * p[2] + pl.xf.m[rr][3];
         }
         const float b0 = 1.0f - h.u - h.v;
         is.P = v[0] * b0 + v[1] * h.u + v[2] * h.v;
@@ -156,15 +246,25 @@ private:
         is.backface = dot(r.direction, is.N) > 0.0f;
         is.is_light = m.is_light;
         is.light_id = m.light_id;
-        is.geom_id = lo;
+        is.geom_id = pl.mesh;
         is.prim_id = local;
         is.shader_name = m.shader_name;
     }
 
+    struct Group {
+        std::vector<std::string> names;
+        Mat4f trans;
+    };
+    struct Place {
+        uint32_t first_prim;   // first global primitive id of this placement
+        uint32_t mesh;
+        Mat4f xf;              // object to world
+    };
+
     kp_ctx* ctx_ = nullptr;
     std::vector<std::shared_ptr<TriangleMesh>> meshes_;
-    std::vector<uint32_t> first_prim_;
-    uint32_t total_tris_ = 0;
+    std::map<std::string, Group> groups_;
+    std::vector<Place> places_;   // in primitive-id order
 };
 
 }  // namespace kazen

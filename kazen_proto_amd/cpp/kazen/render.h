@@ -72,10 +72,10 @@ public:
         m->name = name;
         m->shader_name = shader_name;
         m->is_light = is_light;
-        accelerator->add_trianglemesh(m);
         auto it = materials.find(shader_name);
         if (it == materials.end()) throw KpError(KP_ERR_INVALID, "unknown shader/material: " + shader_name);
-        check(kp_mesh_set_material(ctx(), (uint32_t)(accelerator->meshes().size() - 1), it->second));
+        m->material_id = it->second;
+        accelerator->add_trianglemesh(m);
     }
     void build_bvh(const std::vector<std::string>& names = {}) { accelerator->build(names); }
 

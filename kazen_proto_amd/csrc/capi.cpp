@@ -1,4 +1,5 @@
 // C ABI of the HIP intersector: context, scene upload, BVH build, queries.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -40,7 +41,7 @@ int ensure_buffer(kp_ctx* c, void** buf, size_t* cap, size_t bytes) {
     return KP_OK;
 }
 
-static TraceArgs base_args(kp_ctx* c) {
+TraceArgs trace_base(kp_ctx* c) {
     TraceArgs a;
     std::memset(&a, 0, sizeof(a));
     a.nodes = c->d_nodes;
@@ -52,12 +53,17 @@ static TraceArgs base_args(kp_ctx* c) {
     a.refill_below = c->refill_below;
     a.counters = c->d_counters;
     a.max_steps = c->max_steps;
+    if (c->two_level) {
+        a.insts = c->d_insts;
+        a.tlas_inst = c->d_tlas_inst;
+        a.n_tlas = c->n_tlas_nodes;
+    }
     return a;
 }
 
 int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, bool count) {
     if (n == 0) return KP_OK;
-    TraceArgs a = base_args(c);
+    TraceArgs a = trace_base(c);
     a.rays = rays;
     a.hits = hits;
     a.n = n;
@@ -68,7 +74,7 @@ int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, b
 
 int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out) {
     if (n == 0) return KP_OK;
-    TraceArgs a = base_args(c);
+    TraceArgs a = trace_base(c);
     a.rays = rays;
     a.occl = out;
     a.n = n;
@@ -81,10 +87,205 @@ static void free_bvh(kp_ctx* c) {
     if (c->d_nodes) (void)hipFree(c->d_nodes);
     if (c->d_tris) (void)hipFree(c->d_tris);
     if (c->d_spill) (void)hipFree(c->d_spill);
+    if (c->d_insts) (void)hipFree(c->d_insts);
+    if (c->d_tlas_inst) (void)hipFree(c->d_tlas_inst);
     c->d_nodes = nullptr;
     c->d_tris = nullptr;
     c->d_spill = nullptr;
+    c->d_insts = nullptr;
+    c->d_tlas_inst = nullptr;
+    c->two_level = false;
+    c->n_tlas_nodes = 0;
     c->built = false;
+}
+
+// Row-major 3x4 affine inverse in double; false if singular or non-finite.
+static bool affine_inverse(const float* m, double* inv) {
+    double a[9];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) a[3 * r + k] = m[4 * r + k];
+    for (int i = 0; i < 12; ++i)
+        if (!std::isfinite(m[i])) return false;
+    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double scale = 0.0;
+    for (double v : a) scale = std::max(scale, std::fabs(v));
+    if (!(std::fabs(det) > 1e-12 * scale * scale * scale)) return false;
+    const double id = 1.0 / det;
+    double b[9];
+    b[0] = c00 * id;
+    b[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    b[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    b[3] = c01 * id;
+    b[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    b[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    b[6] = c02 * id;
+    b[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    b[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    for (int r = 0; r < 3; ++r) {
+        inv[4 * r + 0] = b[3 * r + 0];
+        inv[4 * r + 1] = b[3 * r + 1];
+        inv[4 * r + 2] = b[3 * r + 2];
+        inv[4 * r + 3] = -(b[3 * r + 0] * m[3] + b[3 * r + 1] * m[7] + b[3 * r + 2] * m[11]);
+    }
+    return true;
+}
+
+static void box_of(const float* tris9, uint32_t n, double lo[3], double hi[3]) {
+    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+    for (size_t i = 0; i < 3 * (size_t)n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], (double)tris9[3 * i + k]);
+            hi[k] = std::max(hi[k], (double)tris9[3 * i + k]);
+        }
+}
+
+// Two-level build: one BVH8 per prototype (unit 0 = the kp_mesh_add meshes,
+// placed once with the identity, if any), a top-level BVH8 over the placed
+// units' world boxes (leaf size 1), all nodes in one array [top | unit 0 |
+// unit 1 ...] with child and primitive bases rebased, and per placement a
+// record {world-to-object rows, root node, primitive base}.
+static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
+    auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n_world = (uint32_t)(c->soup.size() / 9);
+    const uint32_t n_blas = (uint32_t)c->blas_first.size();
+    const bool has_world = n_world > 0;
+    // units: [world] + prototypes
+    std::vector<Bvh8Host> units(n_blas + (has_world ? 1 : 0));
+    std::vector<const float*> src(units.size());
+    std::vector<uint32_t> ucount(units.size());
+    if (has_world) { src[0] = c->soup.data(); ucount[0] = n_world; }
+    for (uint32_t b = 0; b < n_blas; ++b) {
+        src[b + (has_world ? 1 : 0)] = &c->blas_soup[9 * (size_t)c->blas_first[b]];
+        ucount[b + (has_world ? 1 : 0)] = c->blas_count[b];
+    }
+    std::string err;
+    for (size_t u = 0; u < units.size(); ++u)
+        if (!build_bvh8(src[u], ucount[u], bo, units[u], err))
+            return fail(c, KP_ERR_INVALID, "kp_bvh_build (prototype " + std::to_string(u) + "): " + err);
+    // placements: world (identity), then every instance
+    const uint32_t n_inst = (uint32_t)c->inst_blas.size();
+    const uint32_t n_place = n_inst + (has_world ? 1 : 0);
+    std::vector<float> tlas_tris(9 * (size_t)n_place);
+    std::vector<float> rec(16 * (size_t)n_place, 0.0f);
+    std::vector<uint32_t> place_unit(n_place);
+    std::vector<uint64_t> place_base(n_place);
+    uint64_t prim_next = n_world;
+    for (uint32_t k = 0; k < n_place; ++k) {
+        double lo[3], hi[3];
+        float* r = &rec[16 * (size_t)k];
+        if (has_world && k == 0) {
+            place_unit[k] = 0;
+            place_base[k] = 0;
+            box_of(c->soup.data(), n_world, lo, hi);
+            r[0] = 1.0f; r[5] = 1.0f; r[10] = 1.0f;
+        } else {
+            const uint32_t i = k - (has_world ? 1 : 0);
+            const uint32_t b = c->inst_blas[i];
+            place_unit[k] = b + (has_world ? 1 : 0);
+            place_base[k] = prim_next;
+            prim_next += c->blas_count[b];
+            double olo[3], ohi[3];
+            box_of(&c->blas_soup[9 * (size_t)c->blas_first[b]], c->blas_count[b], olo, ohi);
+            const float* m = &c->inst_o2w[12 * (size_t)i];
+            for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+            for (int corner = 0; corner < 8; ++corner) {
+                const double p[3] = {(corner & 1) ? ohi[0] : olo[0], (corner & 2) ? ohi[1] : olo[1],
+                                     (corner & 4) ? ohi[2] : olo[2]};
+                for (int a = 0; a < 3; ++a) {
+                    const double w = (double)m[4 * a] * p[0] + (double)m[4 * a + 1] * p[1] +
+                                     (double)m[4 * a + 2] * p[2] + (double)m[4 * a + 3];
+                    lo[a] = std::min(lo[a], w);
+                    hi[a] = std::max(hi[a], w);
+                }
+            }
+            std::memcpy(r, &c->inst_w2o[12 * (size_t)i], 12 * sizeof(float));
+        }
+        // the box as a degenerate triangle (lo, hi, lo), widened outward by
+        // one float step so the rounding to float stays conservative
+        float* t = &tlas_tris[9 * (size_t)k];
+        for (int a = 0; a < 3; ++a) {
+            const float fl = std::nextafter((float)lo[a], -INFINITY), fh = std::nextafter((float)hi[a], INFINITY);
+            t[a] = fl; t[3 + a] = fh; t[6 + a] = fl;
+        }
+    }
+    if (prim_next >= 0xFFFFFFFFull)
+        return fail(c, KP_ERR_INVALID, "kp_bvh_build: instanced scene exceeds 2^32-1 primitives");
+    BuildOptions tb = bo;
+    tb.leaf_size = 1;
+    Bvh8Host top;
+    if (!build_bvh8(tlas_tris.data(), n_place, tb, top, err))
+        return fail(c, KP_ERR_INVALID, "kp_bvh_build (top level): " + err);
+    // merge
+    uint64_t total_nodes = top.nodes.size(), total_tris = 0;
+    std::vector<uint32_t> node_off(units.size()), tri_off(units.size());
+    for (size_t u = 0; u < units.size(); ++u) {
+        node_off[u] = (uint32_t)total_nodes;
+        tri_off[u] = (uint32_t)total_tris;
+        total_nodes += units[u].nodes.size();
+        total_tris += ucount[u];
+        if (total_nodes >= (1ull << 24))
+            return fail(c, KP_ERR_INVALID, "kp_bvh_build: instanced BVH exceeds 2^24 nodes");
+        if (total_tris >= (1ull << 28))
+            return fail(c, KP_ERR_INVALID, "kp_bvh_build: prototypes exceed 2^28 triangles");
+    }
+    std::vector<KpNode8> nodes(total_nodes);
+    std::memcpy(nodes.data(), top.nodes.data(), top.nodes.size() * sizeof(KpNode8));
+    std::vector<float> tris(12 * (size_t)total_tris);
+    uint64_t steps = 2ull * top.nodes.size() + n_place + 64u;
+    uint32_t max_unit_depth = 0;
+    for (size_t u = 0; u < units.size(); ++u) {
+        for (size_t i = 0; i < units[u].nodes.size(); ++i) {
+            KpNode8 nd = units[u].nodes[i];
+            nd.child_base += node_off[u];
+            nd.prim_base += tri_off[u];
+            nodes[node_off[u] + i] = nd;
+        }
+        std::memcpy(&tris[12 * (size_t)tri_off[u]], units[u].tris.data(), units[u].tris.size() * sizeof(float));
+        max_unit_depth = std::max(max_unit_depth, units[u].max_depth);
+    }
+    for (uint32_t k = 0; k < n_place; ++k) {
+        const uint32_t u = place_unit[k];
+        steps += 2ull * units[u].nodes.size() + 2u;
+        float* r = &rec[16 * (size_t)k];
+        const uint32_t root = node_off[u], base = (uint32_t)place_base[k];
+        std::memcpy(&r[12], &root, 4);
+        std::memcpy(&r[13], &base, 4);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    *seconds = std::chrono::duration<double>(t1 - t0).count();
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_bvh(c);
+    const size_t nb = nodes.size() * sizeof(KpNode8), tbytes = tris.size() * sizeof(float);
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tbytes));
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_insts, rec.size() * sizeof(float)));
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_tlas_inst, top.tri_order.size() * 4));
+    KP_HIP_TRY(c, hipMemcpy(c->d_nodes, nodes.data(), nb, hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMemcpy(c->d_tris, tris.data(), tbytes, hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMemcpy(c->d_insts, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMemcpy(c->d_tlas_inst, top.tri_order.data(), top.tri_order.size() * 4, hipMemcpyHostToDevice));
+    // Stack bound: at most 8 entries per top-level node on the path (instance
+    // entries and one group), the return marker, 7 per prototype level.
+    c->spill_cap = 8u * top.max_depth + 7u * max_unit_depth + 16u;
+    const uint32_t grid = c->grid_closest > c->grid_occluded ? c->grid_closest : c->grid_occluded;
+    c->spill_lanes = (size_t)grid * KP_TRACE_BLOCK;
+    KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
+    c->two_level = true;
+    c->n_tlas_nodes = (uint32_t)top.nodes.size();
+    c->n_world_prims = n_world;
+    c->n_tris_built = (uint32_t)prim_next;
+    c->n_tris_stored = (uint32_t)total_tris;
+    c->n_nodes = (uint32_t)nodes.size();
+    c->n_bvh2 = top.n_bvh2_nodes;
+    for (const auto& u : units) c->n_bvh2 += u.n_bvh2_nodes;
+    c->max_depth = top.max_depth + max_unit_depth;
+    // A traversal visits every top-level node once and, per placement entered,
+    // each node of its prototype once.
+    c->max_steps = (uint32_t)std::min<uint64_t>(steps, 0xFFFFFFF0ull);
+    c->built = true;
+    render_scene_dirty(c);
+    return KP_OK;
 }
 
 }  // namespace kp
@@ -198,6 +399,75 @@ int kp_mesh_add(kp_ctx* c, const float* xyz, uint32_t n_verts, const uint32_t* i
     return KP_OK;
 }
 
+int kp_blas_add(kp_ctx* c, const float* xyz, uint32_t n_verts, const uint32_t* idx, uint32_t n_tris,
+                uint32_t* blas_id) {
+    if (!c) return KP_ERR_INVALID;
+    if (!xyz || !idx || n_verts == 0 || n_tris == 0)
+        return kp::fail(c, KP_ERR_INVALID, "kp_blas_add: empty mesh or NULL arrays");
+    if ((uint64_t)c->blas_soup.size() / 9 + n_tris >= (1ull << 28))
+        return kp::fail(c, KP_ERR_INVALID, "kp_blas_add: prototypes exceed 2^28 triangles");
+    for (uint64_t i = 0; i < 3ull * n_tris; ++i)
+        if (idx[i] >= n_verts) return kp::fail(c, KP_ERR_INVALID, "kp_blas_add: vertex index out of range");
+    const size_t base = c->blas_soup.size();
+    c->blas_soup.resize(base + 9 * (size_t)n_tris);
+    for (uint32_t t = 0; t < n_tris; ++t)
+        for (int v = 0; v < 3; ++v)
+            std::memcpy(&c->blas_soup[base + 9 * (size_t)t + 3 * v], xyz + 3 * (size_t)idx[3 * (size_t)t + v],
+                        3 * sizeof(float));
+    const uint32_t id = (uint32_t)c->blas_first.size();
+    c->blas_first.push_back((uint32_t)(base / 9));
+    c->blas_count.push_back(n_tris);
+    c->blas_material.push_back(0);
+    if (blas_id) *blas_id = id;
+    return KP_OK;
+}
+
+int kp_instance_add(kp_ctx* c, uint32_t blas_id, const float* xform12, uint32_t* instance_id) {
+    if (!c) return KP_ERR_INVALID;
+    if (blas_id >= c->blas_first.size()) return kp::fail(c, KP_ERR_INVALID, "kp_instance_add: unknown prototype");
+    if (!xform12) return kp::fail(c, KP_ERR_INVALID, "kp_instance_add: NULL transform");
+    double inv[12];
+    if (!kp::affine_inverse(xform12, inv))
+        return kp::fail(c, KP_ERR_INVALID, "kp_instance_add: transform not finite or not invertible");
+    const uint32_t id = (uint32_t)c->inst_blas.size();
+    c->inst_blas.push_back(blas_id);
+    c->inst_o2w.insert(c->inst_o2w.end(), xform12, xform12 + 12);
+    for (int i = 0; i < 12; ++i) c->inst_w2o.push_back((float)inv[i]);
+    if (instance_id) *instance_id = id;
+    return KP_OK;
+}
+
+int kp_geometry_clear(kp_ctx* c) {
+    if (!c) return KP_ERR_INVALID;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    kp::free_bvh(c);
+    c->soup.clear();
+    c->mesh_first.clear();
+    c->mesh_count.clear();
+    c->mesh_material.clear();
+    c->blas_soup.clear();
+    c->blas_first.clear();
+    c->blas_count.clear();
+    c->blas_material.clear();
+    c->inst_o2w.clear();
+    c->inst_w2o.clear();
+    c->inst_blas.clear();
+    c->n_tris_built = c->n_tris_stored = c->n_world_prims = 0;
+    c->n_nodes = c->n_bvh2 = c->max_depth = 0;
+    kp::render_scene_dirty(c);
+    return KP_OK;
+}
+
+int kp_instance_get(kp_ctx* c, uint32_t instance_id, uint32_t* blas_id, float* o2w12, float* w2o12) {
+    if (!c) return KP_ERR_INVALID;
+    if (instance_id >= c->inst_blas.size()) return kp::fail(c, KP_ERR_INVALID, "kp_instance_get: unknown instance");
+    if (blas_id) *blas_id = c->inst_blas[instance_id];
+    if (o2w12) std::memcpy(o2w12, &c->inst_o2w[12 * (size_t)instance_id], 12 * sizeof(float));
+    if (w2o12) std::memcpy(w2o12, &c->inst_w2o[12 * (size_t)instance_id], 12 * sizeof(float));
+    return KP_OK;
+}
+
 int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     if (!c) return KP_ERR_INVALID;
     KP_HIP_TRY(c, hipSetDevice(c->device));
@@ -206,6 +476,12 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     if (o) {
         if (o->leaf_size) bo.leaf_size = o->leaf_size;
         if (o->sah_bins) bo.sah_bins = o->sah_bins;
+    }
+    if (!c->inst_blas.empty()) {
+        double secs = 0.0;
+        const int rc = kp::build_two_level(c, bo, &secs);
+        if (rc == KP_OK) c->build_seconds = secs;
+        return rc;
     }
     kp::Bvh8Host h;
     std::string err;
@@ -226,6 +502,8 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     c->spill_lanes = (size_t)grid * KP_TRACE_BLOCK;
     KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
     c->n_tris_built = n;
+    c->n_tris_stored = n;
+    c->n_world_prims = n;
     c->n_nodes = (uint32_t)h.nodes.size();
     c->n_bvh2 = h.n_bvh2_nodes;
     c->max_depth = h.max_depth;
@@ -243,7 +521,8 @@ int kp_get_stats(kp_ctx* c, kp_stats* s) {
     s->n_tris = c->n_tris_built;
     s->n_nodes = c->n_nodes;
     s->n_bvh2_nodes = c->n_bvh2;
-    s->bvh_bytes = (uint64_t)c->n_nodes * KP_NODE_BYTES + (uint64_t)c->n_tris_built * KP_TRI_BYTES;
+    s->bvh_bytes = (uint64_t)c->n_nodes * KP_NODE_BYTES + (uint64_t)c->n_tris_stored * KP_TRI_BYTES +
+                   (c->two_level ? (uint64_t)(c->inst_blas.size() + 1) * 68u : 0u);
     s->max_depth = c->max_depth;
     s->spill_entries = c->spill_cap;
     s->grid_closest = c->grid_closest;

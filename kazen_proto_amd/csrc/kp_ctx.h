@@ -22,9 +22,21 @@ struct kp_ctx {
     std::vector<uint32_t> mesh_count;
     std::vector<uint32_t> mesh_material;
 
+    // instancing: prototype meshes (object space) and their placements
+    std::vector<float> blas_soup;
+    std::vector<uint32_t> blas_first, blas_count, blas_material;
+    std::vector<float> inst_o2w, inst_w2o;   // 12 per instance, row-major 3x4
+    std::vector<uint32_t> inst_blas;
+
     // BVH on the device
     bool built = false;
-    uint32_t n_tris_built = 0;
+    bool two_level = false;          // built with instances (top-level BVH + prototypes)
+    uint32_t n_tlas_nodes = 0;       // nodes [0, n_tlas_nodes) form the top level
+    uint32_t n_world_prims = 0;      // primitives of the kp_mesh_add meshes
+    float4* d_insts = nullptr;       // instance records (4 float4 each)
+    uint32_t* d_tlas_inst = nullptr; // top-level leaf position -> instance
+    uint32_t n_tris_built = 0;       // primitive ids (instanced: every placement counted)
+    uint32_t n_tris_stored = 0;      // triangles stored on the device
     uint32_t n_nodes = 0, n_bvh2 = 0, max_depth = 0;
     double build_seconds = 0.0;
     uint4* d_nodes = nullptr;
@@ -60,6 +72,8 @@ int ensure_buffer(kp_ctx* c, void** buf, size_t* cap, size_t bytes);
 // Trace helpers shared with the integrator (device pointers, async).
 int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, bool count);
 int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out);
+// Traversal arguments of the context's BVH (one- or two-level).
+TraceArgs trace_base(kp_ctx* c);
 void render_state_free(kp_ctx* c);
 void guide_state_free(kp_ctx* c);
 // Marks the integrator's device copy of the scene stale (meshes changed).

@@ -25,6 +25,12 @@ struct TraceArgs {
     uint32_t max_steps;     // per-ray step limit (> any legitimate traversal)
     unsigned long long* counters;   // [nodes visited, triangles tested]
     const uint32_t* roots;  // if set, root node of each ray (batched scenes); else node 0
+    // two-level traversal (instances): nodes [0, n_tlas) are the top level,
+    // whose leaf children name instances through tlas_inst; insts holds per
+    // instance the world-to-object rows and {root node, primitive base}
+    const float4* insts;
+    const uint32_t* tlas_inst;
+    uint32_t n_tlas;
 };
 
 hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStream_t s);

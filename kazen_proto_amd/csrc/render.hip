@@ -785,12 +785,45 @@ static int prepare_scene(kp_ctx* c) {
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_scene(r);
     const uint32_t n = c->n_tris_built;
+    if (c->two_level && n >= (1u << 28))
+        return fail(c, KP_ERR_INVALID, "render: instanced scene exceeds 2^28 placed triangles");
+    // World-space triangles of every primitive id: the kp_mesh_add meshes as
+    // stored, then each instance's prototype under its object-to-world matrix
+    // (shading records are per placed triangle; only traversal is instanced).
+    std::vector<float> placed;
+    if (c->two_level) {
+        placed.resize(9 * (size_t)n);
+        std::memcpy(placed.data(), c->soup.data(), c->soup.size() * sizeof(float));
+        size_t o = c->soup.size();
+        for (size_t k = 0; k < c->inst_blas.size(); ++k) {
+            const uint32_t b = c->inst_blas[k];
+            const float* m = &c->inst_o2w[12 * k];
+            const float* src = &c->blas_soup[9 * (size_t)c->blas_first[b]];
+            for (size_t v = 0; v < 3 * (size_t)c->blas_count[b]; ++v, o += 3)
+                for (int a = 0; a < 3; ++a)
+                    placed[o + a] = (float)((double)m[4 * a] * src[3 * v] + (double)m[4 * a + 1] * src[3 * v + 1] +
+                                            (double)m[4 * a + 2] * src[3 * v + 2] + (double)m[4 * a + 3]);
+        }
+    }
+    const float* world = c->two_level ? placed.data() : c->soup.data();
+    // meshes as (first primitive, count, material): kp_mesh_add meshes, then instances
+    std::vector<uint32_t> g_first(c->mesh_first), g_count(c->mesh_count), g_mat(c->mesh_material);
+    if (c->two_level) {
+        uint32_t next = c->n_world_prims;
+        for (size_t k = 0; k < c->inst_blas.size(); ++k) {
+            const uint32_t b = c->inst_blas[k];
+            g_first.push_back(next);
+            g_count.push_back(c->blas_count[b]);
+            g_mat.push_back(c->blas_material[b]);
+            next += c->blas_count[b];
+        }
+    }
     std::vector<float4> st(3 * (size_t)n);
     std::vector<uint32_t> light_tris;
     std::vector<float> cdf;
     std::vector<LightRec> lights;
-    for (size_t mi = 0; mi < c->mesh_first.size(); ++mi) {
-        const uint32_t mat = c->mesh_material[mi];
+    for (size_t mi = 0; mi < g_first.size(); ++mi) {
+        const uint32_t mat = g_mat[mi];
         if (mat >= r->mats.size()) return fail(c, KP_ERR_INVALID, "mesh material id out of range");
         const Material& m = r->mats[mat];
         const bool emissive = m.Le[0] != 0.0f || m.Le[1] != 0.0f || m.Le[2] != 0.0f;
@@ -799,13 +832,13 @@ static int prepare_scene(kp_ctx* c) {
             lid = (int32_t)lights.size();
             LightRec lr{};
             lr.first = (uint32_t)light_tris.size();
-            lr.count = c->mesh_count[mi];
+            lr.count = g_count[mi];
             lr.cdf_first = (uint32_t)cdf.size();
             cdf.push_back(0.0f);
             for (uint32_t t = 0; t < lr.count; ++t) {
-                const uint32_t p = c->mesh_first[mi] + t;
+                const uint32_t p = g_first[mi] + t;
                 light_tris.push_back(p);
-                cdf.push_back(cdf.back() + tri_area(&c->soup[9 * (size_t)p]));
+                cdf.push_back(cdf.back() + tri_area(&world[9 * (size_t)p]));
             }
             const float sum = cdf.back();
             lr.area = sum;
@@ -818,9 +851,9 @@ static int prepare_scene(kp_ctx* c) {
             lr.Le[2] = m.Le[2];
             lights.push_back(lr);
         }
-        for (uint32_t t = 0; t < c->mesh_count[mi]; ++t) {
-            const uint32_t p = c->mesh_first[mi] + t;
-            const float* v = &c->soup[9 * (size_t)p];
+        for (uint32_t t = 0; t < g_count[mi]; ++t) {
+            const uint32_t p = g_first[mi] + t;
+            const float* v = &world[9 * (size_t)p];
             uint32_t matbits = mat;
             float fm, fl;
             std::memcpy(&fm, &matbits, 4);
@@ -836,8 +869,8 @@ static int prepare_scene(kp_ctx* c) {
     }
     for (size_t i = 0; i < 3 * (size_t)n; ++i)
         for (int a = 0; a < 3; ++a) {
-            r->scene_lo[a] = std::fmin(r->scene_lo[a], c->soup[3 * i + a]);
-            r->scene_hi[a] = std::fmax(r->scene_hi[a], c->soup[3 * i + a]);
+            r->scene_lo[a] = std::fmin(r->scene_lo[a], world[3 * i + a]);
+            r->scene_hi[a] = std::fmax(r->scene_hi[a], world[3 * i + a]);
         }
     KP_HIP_TRY(c, hipMalloc((void**)&r->stris, st.size() * sizeof(float4)));
     KP_HIP_TRY(c, hipMemcpy(r->stris, st.data(), st.size() * sizeof(float4), hipMemcpyHostToDevice));
@@ -972,15 +1005,7 @@ struct TraceCfg {
 static TraceCfg ctx_trace_cfg(kp_ctx* c) {
     TraceCfg t;
     std::memset(&t, 0, sizeof(t));
-    t.base.nodes = c->d_nodes;
-    t.base.tris = c->d_tris;
-    t.base.counter = c->d_words;
-    t.base.err = c->d_words + 1;
-    t.base.spill = c->d_spill;
-    t.base.spill_cap = c->spill_cap;
-    t.base.refill_below = c->refill_below;
-    t.base.max_steps = c->max_steps;
-    t.base.counters = c->d_counters;
+    t.base = trace_base(c);
     t.grid_closest = c->grid_closest;
     t.grid_occluded = c->grid_occluded;
     t.shade_blocks = (uint32_t)c->cus * 8u;
@@ -1290,6 +1315,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
         if (!s) return fail(c, KP_ERR_INVALID, "kp_batch_create: NULL scene context");
         if (s->device != c->device) return fail(c, KP_ERR_INVALID, "kp_batch_create: scenes on another device");
         if (!s->built) return fail(c, KP_ERR_STATE, "kp_batch_create: scene without kp_bvh_build");
+        if (s->two_level) return fail(c, KP_ERR_INVALID, "kp_batch_create: instanced scenes cannot be batched");
         KP_HIP_TRY(c, hipSetDevice(s->device));
         int rc = prepare_scene(s);
         if (rc) return fail(c, rc, std::string("kp_batch_create: scene ") + std::to_string(i) + ": " + s->err);
@@ -1645,6 +1671,14 @@ int kp_mesh_set_material(kp_ctx* c, uint32_t mesh_id, uint32_t material_id) {
     if (!c) return KP_ERR_INVALID;
     if (mesh_id >= c->mesh_material.size()) return kp::fail(c, KP_ERR_INVALID, "bad mesh id");
     c->mesh_material[mesh_id] = material_id;
+    kp::render_scene_dirty(c);
+    return KP_OK;
+}
+
+int kp_blas_set_material(kp_ctx* c, uint32_t blas_id, uint32_t material_id) {
+    if (!c) return KP_ERR_INVALID;
+    if (blas_id >= c->blas_material.size()) return kp::fail(c, KP_ERR_INVALID, "bad prototype id");
+    c->blas_material[blas_id] = material_id;
     kp::render_scene_dirty(c);
     return KP_OK;
 }

@@ -234,6 +234,48 @@ __device__ __forceinline__ uint32_t group_bound_bits(uint32_t key2) {
     return key2 & 0xFFFFFF00u;
 }
 
+// Entry distance of one child slot of node `idx` (the same arithmetic as
+// visit_node), for a top-level leaf child that names an instance.
+__device__ __forceinline__ float slot_tnear(const uint4* __restrict__ nodes, uint32_t idx, uint32_t slot,
+                                            const RayPre& r, const NodeTest& nt) {
+    const uint4* p = nodes + (size_t)idx * 8;
+    const uint4 w0 = p[0];
+    const uint32_t w = nt.words;
+    const uint4 nx = p[w & 0xFu], ny = p[(w >> 8) & 0xFu], nz = p[(w >> 16) & 0xFu];
+    const uint32_t e = w0.w;
+    const int ex = (int)(int8_t)(e & 0xFF), ey = (int)(int8_t)((e >> 8) & 0xFF), ez = (int)(int8_t)((e >> 16) & 0xFF);
+    const float ax = exp2i(ex) * r.ix, ay = exp2i(ey) * r.iy, az = exp2i(ez) * r.iz;
+    const float bx = (__uint_as_float(w0.x) - r.ox) * r.ix, by = (__uint_as_float(w0.y) - r.oy) * r.iy,
+                bz = (__uint_as_float(w0.z) - r.oz) * r.iz;
+    const int s = (int)slot;
+    const float tnx = fmaf(qplane(nx, s), ax, bx);
+    const float tny = fmaf(qplane(ny, s), ay, by);
+    const float tnz = fmaf(qplane(nz, s), az, bz);
+    return fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+}
+
+// World ray to an instance's object space: o' = M o + m, d' = M d with the
+// row-major 3x4 world-to-object matrix in inst[0..2] (FP32, no contraction,
+// the same order as the oracle); the interval [tmin, tmax] is unchanged.
+__device__ __forceinline__ void ray_to_object(const float4* __restrict__ inst, const float4 a, const float4 b,
+                                              float4& oa, float4& ob) {
+    const float4 m0 = inst[0], m1 = inst[1], m2 = inst[2];
+    oa.x = ((m0.x * a.x + m0.y * a.y) + m0.z * a.z) + m0.w;
+    oa.y = ((m1.x * a.x + m1.y * a.y) + m1.z * a.z) + m1.w;
+    oa.z = ((m2.x * a.x + m2.y * a.y) + m2.z * a.z) + m2.w;
+    oa.w = a.w;
+    ob.x = (m0.x * b.x + m0.y * b.y) + m0.z * b.z;
+    ob.y = (m1.x * b.x + m1.y * b.y) + m1.z * b.z;
+    ob.z = (m2.x * b.x + m2.y * b.y) + m2.z * b.z;
+    ob.w = b.w;
+}
+
+// Special stack entries of the two-level traversal (low byte of y zero; a
+// group entry always has remaining-child bits there):
+//   instance entry  x = instance record, y = bound of its box entry distance
+//   return marker   x = KP_INVALID, y = -inf: back to the world-space ray
+constexpr uint32_t kReturnMarker = 0xFFFFFFFFu;
+
 template <int S>
 struct Stack {
     uint2 (*lds)[KP_TRACE_BLOCK];
@@ -328,7 +370,7 @@ template <bool kAny, bool kCount>
 __device__ __forceinline__ void test_leaves(const float4* __restrict__ tris, const RayPre& r, uint32_t lm,
                                             uint32_t prim_base, uint32_t m0, uint32_t m1, float& best_t,
                                             uint32_t& best_p, float& bu, float& bv, bool& occl,
-                                            uint64_t& cnt_tris) {
+                                            uint64_t& cnt_tris, uint32_t prim_off = 0u) {
     while (lm) {
         const uint32_t s = __ffs(lm) - 1u;
         lm &= lm - 1u;
@@ -345,7 +387,7 @@ __device__ __forceinline__ void test_leaves(const float4* __restrict__ tris, con
                     occl = true;
                     return;
                 }
-                const uint32_t pr = __float_as_uint(ta.w);
+                const uint32_t pr = __float_as_uint(ta.w) + prim_off;
                 if (t < best_t || pr < best_p) { best_t = t; best_p = pr; bu = u; bv = v; }
             }
         }
@@ -361,7 +403,15 @@ __device__ __forceinline__ void test_leaves(const float4* __restrict__ tris, con
 // per lane) and tested when at least kLeafBatch lanes of the wave hold some,
 // or a lane has nothing else to do, so triangle tests run with many lanes
 // active instead of one or two per iteration.
-template <bool kAny, bool kCount>
+//
+// Two-level variant (kInst): nodes below args.n_tlas form the top level,
+// whose leaf children are instances.  A hit instance is pushed as an instance
+// entry (culled later like a group by its entry distance); taking it pushes a
+// return marker, moves the ray to the instance's object space and continues
+// at the instance's root; reaching the marker restores the world-space ray
+// (refetched from the ray buffer).  Leaf children are tested at once here (a
+// postponed set would mix ray spaces).
+template <bool kAny, bool kCount, bool kInst>
 __device__ __forceinline__ void traverse(const TraceArgs& args) {
     __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
     const int tid = threadIdx.x;
@@ -382,6 +432,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     uint32_t best_p = KP_INVALID, cur = KP_INVALID, steps = 0;
     uint32_t pend = 0u, pbase = 0u, pm0 = 0u, pm1 = 0u;   // postponed leaf children
+    uint32_t inst_pbase = 0u;                              // primitive base of the instance entered
     bool occl = false;
     uint64_t cnt_nodes = 0, cnt_tris = 0;
     RayFeeder feed;
@@ -397,6 +448,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
             bu = bv = 0.0f;
             occl = false;
             pend = 0u;
+            inst_pbase = 0u;
             cur = ray_valid(feed.p0, feed.p1) ? feed.proot : KP_INVALID;   // root
             st.sp = 0;
             steps = 0;
@@ -411,8 +463,30 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                     NodeHits h;
                     visit_node(nodes, cur, r, nt, best_t, h);
                     if (kCount) cnt_nodes++;
+                    const uint32_t vis = cur;
                     cur = KP_INVALID;
-                    if (h.leaves) {
+                    if (kInst && h.leaves && vis < args.n_tlas) {
+                        // top-level leaves: push every instance they name
+                        uint32_t lm = h.leaves;
+                        while (lm && !fault) {
+                            const uint32_t s = __ffs(lm) - 1u;
+                            lm &= lm - 1u;
+                            const uint32_t mw = s < 4u ? h.w1.z : h.w1.w;
+                            const uint32_t meta = (mw >> ((s & 3u) * 8u)) & 0xFFu;
+                            const uint32_t first = h.w1.y + (meta & 31u), count = meta >> 5;
+                            const uint32_t gy = group_bound_bits(tkey(slot_tnear(nodes, vis, s, r, nt)));
+                            for (uint32_t i = 0; i < count; ++i) {
+                                if (!st.push(args.tlas_inst[first + i], gy)) {
+                                    atomicOr(args.err, 1u);
+                                    fault = true;
+                                    break;
+                                }
+                            }
+                        }
+                    } else if (kInst && h.leaves) {
+                        test_leaves<kAny, kCount>(tris, r, h.leaves, h.w1.y, h.w1.z, h.w1.w, best_t, best_p, bu, bv,
+                                                  occl, cnt_tris, inst_pbase);
+                    } else if (h.leaves) {
                         if (pend == 0u) {
                             pend = h.leaves;
                             pbase = h.w1.y;
@@ -423,7 +497,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                                                       bv, occl, cnt_tris);
                         }
                     }
-                    if (h.inner && !(kAny && occl)) {
+                    if (h.inner && !fault && !(kAny && occl)) {
                         const uint32_t imask = h.w0.w >> 24;
                         const uint32_t s1 = h.near_key & 7u;
                         const uint32_t rest = h.inner & ~(1u << s1);
@@ -446,6 +520,31 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                             st.drop();
                             continue;
                         }
+                        if (kInst && (e.y & 0xFFu) == 0u) {
+                            st.drop();
+                            const float4 wa = args.rays[2 * (size_t)ray_id], wb = args.rays[2 * (size_t)ray_id + 1];
+                            if (e.x == kReturnMarker) {   // leave the instance: world-space ray again
+                                setup_ray(wa, wb, r);
+                                nt = node_test_setup(r);
+                                inst_pbase = 0u;
+                                continue;
+                            }
+                            const float4* inst = args.insts + 4 * (size_t)e.x;
+                            float4 oa, ob;
+                            ray_to_object(inst, wa, wb, oa, ob);
+                            if (!ray_valid(oa, ob)) continue;   // degenerate in object space: misses
+                            if (!st.push(kReturnMarker, 0xFF800000u)) {
+                                atomicOr(args.err, 1u);
+                                fault = true;
+                                break;
+                            }
+                            const float4 rec = inst[3];
+                            setup_ray(oa, ob, r);
+                            nt = node_test_setup(r);
+                            inst_pbase = __float_as_uint(rec.y);
+                            cur = __float_as_uint(rec.x);
+                            break;
+                        }
                         uint32_t pm = e.y & 0xFFu;
                         const uint32_t s = (__ffs(pm) - 1u) ^ nt.oct;
                         pm &= pm - 1u;
@@ -458,7 +557,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                 // postponed leaves: tested together once enough lanes hold some,
                 // or when a lane has no node left to visit
                 const bool stuck = pend != 0u && (cur == KP_INVALID || fault);
-                if (__ballot(stuck) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch) {
+                if (!kInst && (__ballot(stuck) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch)) {
                     if (pend != 0u && !(kAny && occl)) {
                         test_leaves<kAny, kCount>(tris, r, pend, pbase, pm0, pm1, best_t, best_p, bu, bv, occl,
                                                   cnt_tris);
@@ -495,11 +594,20 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
 
 template <bool kCount>
 __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest(TraceArgs args) {
-    traverse<false, kCount>(args);
+    traverse<false, kCount, false>(args);
 }
 
 __global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded(TraceArgs args) {
-    traverse<true, false>(args);
+    traverse<true, false, false>(args);
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(KP_TRACE_BLOCK) void k_closest_inst(TraceArgs args) {
+    traverse<false, kCount, true>(args);
+}
+
+__global__ __launch_bounds__(KP_TRACE_BLOCK) void k_occluded_inst(TraceArgs args) {
+    traverse<true, false, true>(args);
 }
 
 __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
@@ -524,6 +632,11 @@ __global__ void k_reset_u32(uint32_t* p, uint32_t n) {
 // ---------------------------------------------------------------- launch --
 hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
+    if (a.insts) {
+        if (count) hipLaunchKernelGGL(k_closest_inst<true>, dim3(grid), dim3(KP_TRACE_BLOCK), 0, s, a);
+        else hipLaunchKernelGGL(k_closest_inst<false>, dim3(grid), dim3(KP_TRACE_BLOCK), 0, s, a);
+        return hipGetLastError();
+    }
     if (count) hipLaunchKernelGGL(k_closest<true>, dim3(grid), dim3(KP_TRACE_BLOCK), 0, s, a);
     else hipLaunchKernelGGL(k_closest<false>, dim3(grid), dim3(KP_TRACE_BLOCK), 0, s, a);
     return hipGetLastError();
@@ -531,7 +644,8 @@ hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStre
 
 hipError_t launch_occluded(const TraceArgs& a, uint32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
-    hipLaunchKernelGGL(k_occluded, dim3(grid), dim3(KP_TRACE_BLOCK), 0, s, a);
+    if (a.insts) hipLaunchKernelGGL(k_occluded_inst, dim3(grid), dim3(KP_TRACE_BLOCK), 0, s, a);
+    else hipLaunchKernelGGL(k_occluded, dim3(grid), dim3(KP_TRACE_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

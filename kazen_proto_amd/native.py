@@ -122,6 +122,11 @@ SIGNATURES = {
     "kp_synchronize": (_i32, [_p]),
     "kp_mesh_add": (_i32, [_p, _p, _u32, _p, _u32, _p, ctypes.POINTER(_u32)]),
     "kp_bvh_build": (_i32, [_p, _p]),
+    "kp_blas_add": (_i32, [_p, _p, _u32, _p, _u32, ctypes.POINTER(_u32)]),
+    "kp_blas_set_material": (_i32, [_p, _u32, _u32]),
+    "kp_instance_add": (_i32, [_p, _u32, _p, ctypes.POINTER(_u32)]),
+    "kp_instance_get": (_i32, [_p, _u32, ctypes.POINTER(_u32), _p, _p]),
+    "kp_geometry_clear": (_i32, [_p]),
     "kp_get_stats": (_i32, [_p, _p]),
     "kp_intersect_closest": (_i32, [_p, _p, _u32, _p, _i32]),
     "kp_occluded": (_i32, [_p, _p, _u32, _p, _i32]),
@@ -305,6 +310,42 @@ class Context:
         n = tris9.shape[0]
         return self.mesh_add(tris9.reshape(-1, 3), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3))
 
+    def blas_add(self, verts, idx):
+        """Adds a prototype mesh (object space) for instancing; returns its id."""
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        bid = _u32()
+        self._check(lib().kp_blas_add(self._h, _np_ptr(verts), verts.shape[0], _np_ptr(idx), idx.shape[0],
+                                      ctypes.byref(bid)))
+        return bid.value
+
+    def blas_add_soup(self, tris9):
+        tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+        n = tris9.shape[0]
+        return self.blas_add(tris9.reshape(-1, 3), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3))
+
+    def blas_set_material(self, blas_id, material_id):
+        self._check(lib().kp_blas_set_material(self._h, blas_id, material_id))
+
+    def instance_add(self, blas_id, o2w):
+        """Places prototype blas_id with a row-major 3x4 object-to-world matrix."""
+        xf = np.ascontiguousarray(o2w, np.float32).reshape(12)
+        iid = _u32()
+        self._check(lib().kp_instance_add(self._h, blas_id, _np_ptr(xf), ctypes.byref(iid)))
+        return iid.value
+
+    def instance_get(self, instance_id):
+        """(prototype id, object-to-world 3x4, world-to-object 3x4 as traversed)."""
+        b = _u32()
+        o2w = np.zeros((3, 4), np.float32)
+        w2o = np.zeros((3, 4), np.float32)
+        self._check(lib().kp_instance_get(self._h, instance_id, ctypes.byref(b), _np_ptr(o2w), _np_ptr(w2o)))
+        return b.value, o2w, w2o
+
+    def geometry_clear(self):
+        """Drops every mesh, prototype, instance and the BVH (materials stay)."""
+        self._check(lib().kp_geometry_clear(self._h))
+
     def bvh_build(self, leaf_size=4, sah_bins=32):
         o = BuildOpts()
         o.leaf_size = leaf_size
@@ -368,6 +409,13 @@ class Context:
         for tris, mi in zip(scene.meshes, scene.mesh_material):
             mesh = self.mesh_add_soup(tris)
             self._check(lib().kp_mesh_set_material(self._h, mesh, mids[mi]))
+        protos = []
+        for tris, mi in zip(getattr(scene, "prototypes", []), getattr(scene, "prototype_material", [])):
+            b = self.blas_add_soup(tris)
+            self.blas_set_material(b, mids[mi])
+            protos.append(b)
+        for pi, o2w in getattr(scene, "instances", []):
+            self.instance_add(protos[pi], o2w)
         self.bvh_build(leaf_size=leaf_size)
 
     def film_reset(self, width, height):

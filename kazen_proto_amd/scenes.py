@@ -39,26 +39,51 @@ class Scene:
     rr_depth: int = 3
     seed: int = 0
     name: str = ""
+    # instancing: prototype meshes in object space, placed by (prototype,
+    # row-major 3x4 object-to-world matrix); primitive ids run over the meshes,
+    # then over each instance's prototype in instance order
+    prototypes: list = field(default_factory=list)
+    prototype_material: list = field(default_factory=list)
+    instances: list = field(default_factory=list)
 
     def add_mesh(self, tris9, mat_index):
         self.meshes.append(np.ascontiguousarray(tris9, np.float32).reshape(-1, 9))
         self.mesh_material.append(int(mat_index))
 
+    def add_prototype(self, tris9, mat_index):
+        self.prototypes.append(np.ascontiguousarray(tris9, np.float32).reshape(-1, 9))
+        self.prototype_material.append(int(mat_index))
+        return len(self.prototypes) - 1
+
+    def add_instance(self, proto, o2w):
+        self.instances.append((int(proto), np.asarray(o2w, np.float32).reshape(3, 4)))
+        return len(self.instances) - 1
+
+    def placed_meshes(self):
+        """(world-space [n,9] triangles, material) of every mesh and instance,
+        in primitive-id order (instances transformed in float64)."""
+        out = list(zip(self.meshes, self.mesh_material))
+        for pi, o2w in self.instances:
+            m = np.asarray(o2w, np.float64)
+            v = self.prototypes[pi].reshape(-1, 3).astype(np.float64) @ m[:, :3].T + m[:, 3]
+            out.append((v.reshape(-1, 9).astype(np.float32), self.prototype_material[pi]))
+        return out
+
     @property
     def n_tris(self):
-        return int(sum(m.shape[0] for m in self.meshes))
+        return int(sum(m.shape[0] for m, _ in self.placed_meshes()))
 
     def soup(self):
-        return np.concatenate(self.meshes, axis=0)
+        return np.concatenate([m for m, _ in self.placed_meshes()], axis=0)
 
     def tri_material(self):
-        return np.concatenate([np.full(m.shape[0], mi, np.int32)
-                               for m, mi in zip(self.meshes, self.mesh_material)])
+        return np.concatenate([np.full(m.shape[0], mi, np.int32) for m, mi in self.placed_meshes()])
 
     def tri_light(self):
-        """Light id per triangle (-1: not emissive); one light per emissive mesh."""
+        """Light id per triangle (-1: not emissive); one light per emissive mesh
+        or instance."""
         out, lid = [], 0
-        for m, mi in zip(self.meshes, self.mesh_material):
+        for m, mi in self.placed_meshes():
             if any(v != 0.0 for v in self.materials[mi]["Le"]):
                 out.append(np.full(m.shape[0], lid, np.int32))
                 lid += 1

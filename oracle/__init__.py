@@ -100,6 +100,7 @@ def lib():
             "or_render": (_i32, [_p, _p, _p, _p, _i32]),
             "or_integrate_rays": (_i32, [_p, _p, _u32, _p, _p, _i32]),
             "or_render_guided": (_i32, [_p, _p, _p, _p, _p, _i32]),
+            "or_instanced_closest": (None, [_p, _p, _p, _p, _p, _p, _u32, _p, _u32, _p, _i32]),
             "or_integrate_rays_guided": (_i32, [_p, _p, _u32, _p, _p, _p, _i32]),
         }
         for name, (res, args) in sig.items():
@@ -387,3 +388,21 @@ class Scene:
         if rc != 0:
             raise RuntimeError("or_integrate_rays failed (%d)" % rc)
         return out
+
+
+def instanced_closest(meshes, instances, rays, threads=0):
+    """Two-level brute-force closest hit.  meshes: list of [n, 9] object-space
+    triangle arrays; instances: list of (mesh index, world-to-object 3x4
+    matrix); global prim of instance k = sum of the earlier instances' mesh
+    sizes + local index.  Returns hits [n, 4] (t, prim bits, u, v)."""
+    tris = np.ascontiguousarray(np.concatenate([np.asarray(m, np.float32).reshape(-1, 9) for m in meshes]))
+    counts = np.array([np.asarray(m).reshape(-1, 9).shape[0] for m in meshes], np.uint32)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.uint32)
+    w2o = np.ascontiguousarray(np.array([np.asarray(x, np.float32).reshape(12) for _, x in instances], np.float32))
+    imesh = np.array([m for m, _ in instances], np.uint32)
+    base = np.concatenate([[0], np.cumsum(counts[imesh])[:-1]]).astype(np.uint32)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    hits = np.zeros((rays.shape[0], 4), np.float32)
+    lib().or_instanced_closest(_ptr(tris), _ptr(first), _ptr(counts), _ptr(w2o), _ptr(imesh), _ptr(base),
+                               len(instances), _ptr(rays), rays.shape[0], _ptr(hits), threads)
+    return hits

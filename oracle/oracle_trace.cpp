@@ -481,6 +481,50 @@ void or_closest_bruteforce(const float* tris, uint32_t n_tris, const float* rays
     });
 }
 
+// Two-level closest hit: each instance tests its mesh with the ray moved to
+// object space by the instance's world-to-object matrix (Embree instances:
+// rtcSetGeometryTransform on a subscene, src/core/accel.cpp:304-313, placed by
+// EmbreeAccel::add_instances, accel.cpp:317-346). Brute force over instances
+// and triangles; the world-space t is kept (the direction is transformed
+// unnormalised).
+void or_instanced_closest(const float* mesh_tris, const uint32_t* mesh_first, const uint32_t* mesh_count,
+                          const float* inst_w2o, const uint32_t* inst_mesh, const uint32_t* inst_base,
+                          uint32_t n_inst, const float* rays, uint32_t n_rays, float* hits, int threads) {
+    parallel_chunks(n_rays, threads, 16, [&](uint32_t b, uint32_t e) {
+        for (uint32_t i = b; i < e; ++i) {
+            const float* wr = rays + 8 * (size_t)i;
+            if (!ray_valid(wr)) { write_miss(hits + 4 * (size_t)i); continue; }
+            float best_t = wr[7], bu = 0, bv = 0;
+            uint32_t best_p = 0xFFFFFFFFu;
+            for (uint32_t k = 0; k < n_inst; ++k) {
+                const float* m = inst_w2o + 12 * (size_t)k;
+                float orr[8];
+                for (int a = 0; a < 3; ++a) {
+                    const float* row = m + 4 * a;
+                    orr[a] = ((row[0] * wr[0] + row[1] * wr[1]) + row[2] * wr[2]) + row[3];
+                    orr[4 + a] = (row[0] * wr[4] + row[1] * wr[5]) + row[2] * wr[6];
+                }
+                orr[3] = wr[3];
+                orr[7] = wr[7];
+                if (!ray_valid(orr)) continue;
+                WRay r = make_wray(orr);
+                const uint32_t mesh = inst_mesh[k];
+                for (uint32_t p = 0; p < mesh_count[mesh]; ++p) {
+                    const float* tv = mesh_tris + 9 * (size_t)(mesh_first[mesh] + p);
+                    float t, u, v;
+                    if (wt_hit(r, tv, tv + 3, tv + 6, best_t, t, u, v)) {
+                        const uint32_t gp = inst_base[k] + p;
+                        if (t < best_t || gp < best_p) { best_t = t; best_p = gp; bu = u; bv = v; }
+                    }
+                }
+            }
+            float* h = hits + 4 * (size_t)i;
+            if (best_p == 0xFFFFFFFFu) write_miss(h);
+            else write_hit(h, best_t, best_p, bu, bv);
+        }
+    });
+}
+
 uint32_t or_count_near_ties(const float* tris, uint32_t n_tris, const float* rays,
                             uint32_t n_rays, int threads) {
     std::atomic<uint32_t> ties{0};

@@ -80,6 +80,37 @@ int main() {
         for (float v : sc.film.rgb) mean += v;
         mean /= (double)sc.film.rgb.size();
         std::printf("render %d %d %.9g\n", ok ? 1 : 0, cb.calls, mean);
+
+        // 4. Accelerator::add_instances: a tile mesh placed directly, through
+        //    a translated group and through a rotated group that nests it;
+        //    the ground placed once. Vertical rays from y = 5 above each.
+        SceneGPU ins;
+        ins.add_material("diffuse", lambert(0.5f));
+        ins.add_mesh(Mat4f(), {Vec3f(-0.25f, 0, -0.25f), Vec3f(0.25f, 0, -0.25f), Vec3f(0.25f, 0, 0.25f),
+                               Vec3f(-0.25f, 0, 0.25f)},
+                     {}, {}, {Vec3i(0, 1, 2), Vec3i(0, 2, 3)}, "tile", "diffuse");
+        ins.add_mesh(Mat4f(), {Vec3f(-5, -1, -5), Vec3f(5, -1, -5), Vec3f(5, -1, 5), Vec3f(-5, -1, 5)}, {}, {},
+                     {Vec3i(0, 1, 2), Vec3i(0, 2, 3)}, "ground", "diffuse");
+        Mat4f a;
+        a.m[0][3] = -1.0f;
+        a.m[1][3] = 0.5f;
+        Mat4f b;   // 90 degrees about y, then (0, 1, 1.5)
+        b.m[0][0] = 0.0f; b.m[0][2] = 1.0f;
+        b.m[2][0] = -1.0f; b.m[2][2] = 0.0f;
+        b.m[1][3] = 1.0f;
+        b.m[2][3] = 1.5f;
+        ins.accelerator->add_instances("pair", {"tile"}, a);
+        ins.accelerator->add_instances("pair2", {"tile", "pair"}, b);
+        for (int pass = 0; pass < 2; ++pass) {   // a second build starts from scratch
+            ins.build_bvh({"ground", "tile", "pair", "pair2"});
+            const float xz[5][2] = {{0.05f, 0.03f}, {-0.95f, 0.03f}, {0.05f, 1.53f}, {0.05f, 2.53f}, {3.0f, 3.0f}};
+            for (int i = 0; i < 5; ++i) {
+                Intersection h;
+                const bool got = ins.accelerator->intersect(Ray(Vec3f(xz[i][0], 5.0f, xz[i][1]), Vec3f(0, -1, 0), 0.0f), h);
+                std::printf("inst_hit %d %d %d %zu %zu %.9g %.9g %.9g %.9g\n", pass, i, got ? 1 : 0, h.geom_id,
+                            h.prim_id, h.ray_t, h.P[0], h.P[1], h.P[2]);
+            }
+        }
         return 0;
     } catch (const KpError& e) {
         std::printf("error %d %s\n", e.code, e.what());

@@ -65,3 +65,14 @@ def test_adapter_program(tmp_path):
     assert ok == "1" and calls == "1" and float(mean) > 0.0
     pfm = (tmp_path / "out.pfm").read_bytes()
     assert pfm.startswith(b"PF\n16 16\n") and len(pfm) >= 16 * 16 * 12
+
+    # Accelerator::add_instances: tile (mesh 0) placed four times through
+    # nested groups, ground (mesh 1) once; both builds give the same hits
+    want = [(0, 5.0, 0.0), (0, 4.5, 0.5), (0, 4.0, 1.0), (0, 3.5, 1.5), (1, 6.0, -1.0)]
+    rows = out["inst_hit"]
+    assert len(rows) == 10
+    for r in rows:
+        p, i, got, geom, prim, t, px, py, pz = r
+        g, tw, yw = want[int(i)]
+        assert got == "1" and int(geom) == g and int(prim) in (0, 1)
+        assert abs(float(t) - tw) < 1e-5 and abs(float(py) - yw) < 1e-5
