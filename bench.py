@@ -132,20 +132,11 @@ def guiding_equal_error(args):
             "plain_throughput": plain_rate, "wall_seconds": round(time.time() - t0, 1)}
 
 
-def path_trace_leg(args):
-    """Plain wavefront path trace of the terrain workload (2048^2 fBm
-    heightfield, 256 spheres, three material families, 4 area lights) at
-    args.pt_res^2 x args.pt_spp, depth 8: samples/s and rays/s over the
-    whole kp_render_accumulate (all bounces, shading, shadow rays, film)."""
-    from kazen_proto_amd import native, scenes
-    t0 = time.time()
-    s = scenes.terrain_scene(grid=args.pt_grid, width=args.pt_res, height=args.pt_res, spp=args.pt_spp)
-    gen = time.time() - t0
+def _render_terrain(s, instances):
+    from kazen_proto_amd import native
     c = native.Context(0)
-    c.load_scene(s)
+    c.load_scene(s, instances=instances)
     st = c.stats()
-    print("[bench] terrain %d tris (generated in %.1fs), BVH build %.2fs" % (s.n_tris, gen, st["build_seconds"]),
-          file=sys.stderr, flush=True)
     c.film_reset(s.width, s.height)
     c.render_accumulate(native.make_render_desc(s, spp=1))      # warm-up (scene upload, path buffers)
     c.film_reset(s.width, s.height)
@@ -154,15 +145,36 @@ def path_trace_leg(args):
     img = c.film_read(s.width, s.height)
     c.close()
     rays = rs["extension_rays"] + rs["shadow_rays"]
-    out = {"workload": "terrain %dx%d heightfield + 256 spheres, %dx%d, %d spp, depth 8"
+    return {"seconds": round(rs["seconds"], 4),
+            "samples_per_s": round(rs["paths"] / rs["seconds"], 1),
+            "mrays_per_s": round(rays / rs["seconds"] / 1e6, 2),
+            "extension_rays": int(rs["extension_rays"]), "shadow_rays": int(rs["shadow_rays"]),
+            "passes": int(rs["passes"]), "paths_per_pass": int(rs["paths_per_pass"]),
+            "bvh_build_seconds": round(st["build_seconds"], 3), "bvh_nodes": int(st["n_nodes"]),
+            "two_level": bool(st["two_level"]), "bvh_bytes": int(st["bvh_bytes"]),
+            "image_mean": round(float(img.mean()), 5), "finite": bool(np.isfinite(img).all())}
+
+
+def path_trace_leg(args):
+    """Plain wavefront path trace of the terrain workload (2048^2 fBm
+    heightfield, 256 instanced spheres, three material families, 4 area
+    lights) at args.pt_res^2 x args.pt_spp, depth 8: samples/s and rays/s
+    over the whole kp_render_accumulate (all bounces, shading, shadow rays,
+    film).  The spheres are one prototype placed 256 times: the default build
+    bakes the placements into the one-level BVH (they fit the bake budget);
+    the same scene with two-level traversal (shared sphere BVH) is rendered
+    beside it."""
+    from kazen_proto_amd import native, scenes
+    t0 = time.time()
+    s = scenes.terrain_scene(grid=args.pt_grid, width=args.pt_res, height=args.pt_res, spp=args.pt_spp,
+                             instanced=True)
+    gen = time.time() - t0
+    print("[bench] terrain %d tris (generated in %.1fs)" % (s.n_tris, gen), file=sys.stderr, flush=True)
+    out = {"workload": "terrain %dx%d heightfield + 256 instanced spheres, %dx%d, %d spp, depth 8"
                        % (args.pt_grid, args.pt_grid, s.width, s.height, s.spp),
-           "triangles": int(s.n_tris), "seconds": round(rs["seconds"], 4),
-           "samples_per_s": round(rs["paths"] / rs["seconds"], 1),
-           "mrays_per_s": round(rays / rs["seconds"] / 1e6, 2),
-           "extension_rays": int(rs["extension_rays"]), "shadow_rays": int(rs["shadow_rays"]),
-           "passes": int(rs["passes"]), "paths_per_pass": int(rs["paths_per_pass"]),
-           "bvh_build_seconds": round(st["build_seconds"], 3), "image_mean": round(float(img.mean()), 5),
-           "finite": bool(np.isfinite(img).all())}
+           "triangles": int(s.n_tris)}
+    out.update(_render_terrain(s, native.INSTANCES_AUTO))
+    out["two_level_traversal"] = _render_terrain(s, native.INSTANCES_TWO_LEVEL)
     print("[bench] path trace: %s" % out, file=sys.stderr, flush=True)
     return out
 

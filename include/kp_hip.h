@@ -58,10 +58,22 @@ typedef struct {
     float u, v;
 } kp_hit;
 
+/* How kp_bvh_build treats instances (kp_instance_add). Baking transforms
+ * every placement's triangles into the one-level BVH (on the host, FP64 to
+ * FP32), which traverses fastest; two-level traversal shares each prototype's
+ * BVH and triangles among its placements, which saves memory. Primitive ids
+ * are the same either way. AUTO bakes while the placed instance triangles
+ * number at most KP_BAKE_BUDGET. */
+#define KP_INSTANCES_AUTO 0u
+#define KP_INSTANCES_TWO_LEVEL 1u
+#define KP_INSTANCES_BAKE 2u
+#define KP_BAKE_BUDGET (1u << 24)
+
 typedef struct {
     uint32_t leaf_size;   /* max triangles per leaf, 1..4 (default 4) */
     uint32_t sah_bins;    /* SAH bins, 4..64 (default 32) */
-    uint32_t reserved[6];
+    uint32_t instances;   /* KP_INSTANCES_* (default AUTO) */
+    uint32_t reserved[5];
 } kp_build_opts;
 
 typedef struct {
@@ -75,7 +87,8 @@ typedef struct {
     uint32_t grid_occluded;
     double build_seconds;      /* host BVH build wall time */
     uint32_t device_cus;
-    uint32_t reserved[7];
+    uint32_t two_level;        /* 1: instances traversed two-level; 0: one-level (baked) */
+    uint32_t reserved[6];
 } kp_stats;
 
 /* Context ---------------------------------------------------------------- */
@@ -108,14 +121,14 @@ int kp_mesh_add(kp_ctx* ctx, const float* xyz, uint32_t n_verts, const uint32_t*
 int kp_bvh_build(kp_ctx* ctx, const kp_build_opts* opts);
 int kp_get_stats(kp_ctx* ctx, kp_stats* out);
 
-/* Instancing (two-level BVH). A prototype mesh is added once in object space
- * with kp_blas_add and placed any number of times with kp_instance_add
- * (row-major 3x4 object-to-world matrix, invertible). With instances present,
- * kp_bvh_build builds one BVH8 per prototype (and one over the kp_mesh_add
- * meshes, placed once with the identity) and a top-level BVH8 over the
- * instances' world boxes; a ray enters each instance it reaches in the
+/* Instancing. A prototype mesh is added once in object space with
+ * kp_blas_add and placed any number of times with kp_instance_add (row-major
+ * 3x4 object-to-world matrix, invertible). kp_bvh_build either bakes the
+ * placements (kp_build_opts.instances) or builds one BVH8 per prototype (and
+ * one over the kp_mesh_add meshes, traversed first) and a top-level BVH8 over
+ * the instances' world boxes; a ray enters each instance it reaches in the
  * instance's object space (o' = M o + m, d' = M d in FP32, so t keeps its
- * meaning). Primitive ids: the kp_mesh_add meshes first, then every
+ * meaning; an exactly-identity placement uses the ray as given). Primitive ids: the kp_mesh_add meshes first, then every
  * instance's primitives in instance order. The integrator shades instances
  * like any mesh (material per prototype, an emissive prototype makes each
  * instance a light). Replaces Accelerator::add_instances
@@ -125,6 +138,9 @@ int kp_blas_add(kp_ctx* ctx, const float* xyz, uint32_t n_verts, const uint32_t*
                 uint32_t* blas_id);
 int kp_blas_set_material(kp_ctx* ctx, uint32_t blas_id, uint32_t material_id);
 int kp_instance_add(kp_ctx* ctx, uint32_t blas_id, const float* xform12, uint32_t* instance_id);
+/* Material of one instance, overriding its prototype's
+ * (kp_blas_set_material) for the integrator. */
+int kp_instance_set_material(kp_ctx* ctx, uint32_t instance_id, uint32_t material_id);
 /* Removes every mesh, prototype and instance and the BVH (materials, film
  * and guiding state stay), so a scene can be rebuilt from scratch.
  * (The reference rebuilds by constructing a new Embree scene; no counterpart

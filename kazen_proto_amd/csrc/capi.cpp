@@ -57,6 +57,7 @@ TraceArgs trace_base(kp_ctx* c) {
         a.insts = c->d_insts;
         a.tlas_inst = c->d_tlas_inst;
         a.n_tlas = c->n_tlas_nodes;
+        a.world_root = c->world_root;
     }
     return a;
 }
@@ -96,6 +97,7 @@ static void free_bvh(kp_ctx* c) {
     c->d_tlas_inst = nullptr;
     c->two_level = false;
     c->n_tlas_nodes = 0;
+    c->world_root = KP_INVALID;
     c->built = false;
 }
 
@@ -131,6 +133,27 @@ static bool affine_inverse(const float* m, double* inv) {
     return true;
 }
 
+void bake_placed(const kp_ctx* c, std::vector<float>& out) {
+    size_t n = c->soup.size();
+    for (uint32_t b : c->inst_blas) n += 9 * (size_t)c->blas_count[b];
+    out.resize(n);
+    std::memcpy(out.data(), c->soup.data(), c->soup.size() * sizeof(float));
+    size_t o = c->soup.size();
+    for (size_t k = 0; k < c->inst_blas.size(); ++k) {
+        const uint32_t b = c->inst_blas[k];
+        const float* m = &c->inst_o2w[12 * k];
+        const float* src = &c->blas_soup[9 * (size_t)c->blas_first[b]];
+        for (size_t v = 0; v < 3 * (size_t)c->blas_count[b]; ++v, o += 3)
+            for (int a = 0; a < 3; ++a) {
+                double w = (double)m[4 * a] * (double)src[3 * v];
+                w = w + (double)m[4 * a + 1] * (double)src[3 * v + 1];
+                w = w + (double)m[4 * a + 2] * (double)src[3 * v + 2];
+                w = w + (double)m[4 * a + 3];
+                out[o + a] = (float)w;
+            }
+    }
+}
+
 static void box_of(const float* tris9, uint32_t n, double lo[3], double hi[3]) {
     for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
     for (size_t i = 0; i < 3 * (size_t)n; ++i)
@@ -141,66 +164,58 @@ static void box_of(const float* tris9, uint32_t n, double lo[3], double hi[3]) {
 }
 
 // Two-level build: one BVH8 per prototype (unit 0 = the kp_mesh_add meshes,
-// placed once with the identity, if any), a top-level BVH8 over the placed
-// units' world boxes (leaf size 1), all nodes in one array [top | unit 0 |
-// unit 1 ...] with child and primitive bases rebased, and per placement a
-// record {world-to-object rows, root node, primitive base}.
+// if any, traversed first in world space), a top-level BVH8 over the
+// instances' world boxes (leaf size 1), all nodes in one array [top | unit 0 |
+// unit 1 ...] with child and primitive bases rebased, and per instance a
+// record {world-to-object rows, root node, primitive base, flags}.
 static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     auto t0 = std::chrono::steady_clock::now();
     const uint32_t n_world = (uint32_t)(c->soup.size() / 9);
     const uint32_t n_blas = (uint32_t)c->blas_first.size();
     const bool has_world = n_world > 0;
+    const uint32_t ub = has_world ? 1u : 0u;   // unit of prototype 0
     // units: [world] + prototypes
-    std::vector<Bvh8Host> units(n_blas + (has_world ? 1 : 0));
+    std::vector<Bvh8Host> units(n_blas + ub);
     std::vector<const float*> src(units.size());
     std::vector<uint32_t> ucount(units.size());
     if (has_world) { src[0] = c->soup.data(); ucount[0] = n_world; }
     for (uint32_t b = 0; b < n_blas; ++b) {
-        src[b + (has_world ? 1 : 0)] = &c->blas_soup[9 * (size_t)c->blas_first[b]];
-        ucount[b + (has_world ? 1 : 0)] = c->blas_count[b];
+        src[b + ub] = &c->blas_soup[9 * (size_t)c->blas_first[b]];
+        ucount[b + ub] = c->blas_count[b];
     }
     std::string err;
     for (size_t u = 0; u < units.size(); ++u)
         if (!build_bvh8(src[u], ucount[u], bo, units[u], err))
             return fail(c, KP_ERR_INVALID, "kp_bvh_build (prototype " + std::to_string(u) + "): " + err);
-    // placements: world (identity), then every instance
+    // one record per instance
     const uint32_t n_inst = (uint32_t)c->inst_blas.size();
-    const uint32_t n_place = n_inst + (has_world ? 1 : 0);
-    std::vector<float> tlas_tris(9 * (size_t)n_place);
-    std::vector<float> rec(16 * (size_t)n_place, 0.0f);
-    std::vector<uint32_t> place_unit(n_place);
-    std::vector<uint64_t> place_base(n_place);
+    std::vector<float> tlas_tris(9 * (size_t)n_inst);
+    std::vector<float> rec(16 * (size_t)n_inst, 0.0f);
+    std::vector<uint32_t> place_unit(n_inst);
+    std::vector<uint64_t> place_base(n_inst);
     uint64_t prim_next = n_world;
-    for (uint32_t k = 0; k < n_place; ++k) {
+    for (uint32_t k = 0; k < n_inst; ++k) {
         double lo[3], hi[3];
         float* r = &rec[16 * (size_t)k];
-        if (has_world && k == 0) {
-            place_unit[k] = 0;
-            place_base[k] = 0;
-            box_of(c->soup.data(), n_world, lo, hi);
-            r[0] = 1.0f; r[5] = 1.0f; r[10] = 1.0f;
-        } else {
-            const uint32_t i = k - (has_world ? 1 : 0);
-            const uint32_t b = c->inst_blas[i];
-            place_unit[k] = b + (has_world ? 1 : 0);
-            place_base[k] = prim_next;
-            prim_next += c->blas_count[b];
-            double olo[3], ohi[3];
-            box_of(&c->blas_soup[9 * (size_t)c->blas_first[b]], c->blas_count[b], olo, ohi);
-            const float* m = &c->inst_o2w[12 * (size_t)i];
-            for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
-            for (int corner = 0; corner < 8; ++corner) {
-                const double p[3] = {(corner & 1) ? ohi[0] : olo[0], (corner & 2) ? ohi[1] : olo[1],
-                                     (corner & 4) ? ohi[2] : olo[2]};
-                for (int a = 0; a < 3; ++a) {
-                    const double w = (double)m[4 * a] * p[0] + (double)m[4 * a + 1] * p[1] +
-                                     (double)m[4 * a + 2] * p[2] + (double)m[4 * a + 3];
-                    lo[a] = std::min(lo[a], w);
-                    hi[a] = std::max(hi[a], w);
-                }
+        const uint32_t b = c->inst_blas[k];
+        place_unit[k] = b + ub;
+        place_base[k] = prim_next;
+        prim_next += c->blas_count[b];
+        double olo[3], ohi[3];
+        box_of(&c->blas_soup[9 * (size_t)c->blas_first[b]], c->blas_count[b], olo, ohi);
+        const float* m = &c->inst_o2w[12 * (size_t)k];
+        for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+        for (int corner = 0; corner < 8; ++corner) {
+            const double p[3] = {(corner & 1) ? ohi[0] : olo[0], (corner & 2) ? ohi[1] : olo[1],
+                                 (corner & 4) ? ohi[2] : olo[2]};
+            for (int a = 0; a < 3; ++a) {
+                const double w = (double)m[4 * a] * p[0] + (double)m[4 * a + 1] * p[1] +
+                                 (double)m[4 * a + 2] * p[2] + (double)m[4 * a + 3];
+                lo[a] = std::min(lo[a], w);
+                hi[a] = std::max(hi[a], w);
             }
-            std::memcpy(r, &c->inst_w2o[12 * (size_t)i], 12 * sizeof(float));
         }
+        std::memcpy(r, &c->inst_w2o[12 * (size_t)k], 12 * sizeof(float));
         // the box as a degenerate triangle (lo, hi, lo), widened outward by
         // one float step so the rounding to float stays conservative
         float* t = &tlas_tris[9 * (size_t)k];
@@ -214,7 +229,7 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     BuildOptions tb = bo;
     tb.leaf_size = 1;
     Bvh8Host top;
-    if (!build_bvh8(tlas_tris.data(), n_place, tb, top, err))
+    if (!build_bvh8(tlas_tris.data(), n_inst, tb, top, err))
         return fail(c, KP_ERR_INVALID, "kp_bvh_build (top level): " + err);
     // merge
     uint64_t total_nodes = top.nodes.size(), total_tris = 0;
@@ -232,7 +247,7 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     std::vector<KpNode8> nodes(total_nodes);
     std::memcpy(nodes.data(), top.nodes.data(), top.nodes.size() * sizeof(KpNode8));
     std::vector<float> tris(12 * (size_t)total_tris);
-    uint64_t steps = 2ull * top.nodes.size() + n_place + 64u;
+    uint64_t steps = 2ull * top.nodes.size() + n_inst + 64u + (has_world ? 2ull * units[0].nodes.size() : 0u);
     uint32_t max_unit_depth = 0;
     for (size_t u = 0; u < units.size(); ++u) {
         for (size_t i = 0; i < units[u].nodes.size(); ++i) {
@@ -244,13 +259,17 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
         std::memcpy(&tris[12 * (size_t)tri_off[u]], units[u].tris.data(), units[u].tris.size() * sizeof(float));
         max_unit_depth = std::max(max_unit_depth, units[u].max_depth);
     }
-    for (uint32_t k = 0; k < n_place; ++k) {
+    static const float eye[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    for (uint32_t k = 0; k < n_inst; ++k) {
+        float* r = &rec[16 * (size_t)k];
         const uint32_t u = place_unit[k];
         steps += 2ull * units[u].nodes.size() + 2u;
-        float* r = &rec[16 * (size_t)k];
         const uint32_t root = node_off[u], base = (uint32_t)place_base[k];
+        // an exact identity world-to-object matrix: the ray is used as is
+        const uint32_t flags = std::memcmp(r, eye, sizeof(eye)) == 0 ? 1u : 0u;
         std::memcpy(&r[12], &root, 4);
         std::memcpy(&r[13], &base, 4);
+        std::memcpy(&r[14], &flags, 4);
     }
     auto t1 = std::chrono::steady_clock::now();
     *seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -265,13 +284,16 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     KP_HIP_TRY(c, hipMemcpy(c->d_tris, tris.data(), tbytes, hipMemcpyHostToDevice));
     KP_HIP_TRY(c, hipMemcpy(c->d_insts, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
     KP_HIP_TRY(c, hipMemcpy(c->d_tlas_inst, top.tri_order.data(), top.tri_order.size() * 4, hipMemcpyHostToDevice));
-    // Stack bound: at most 8 entries per top-level node on the path (instance
-    // entries and one group), the return marker, 7 per prototype level.
+    // Stack bound: the top-level marker, at most 8 entries
+    // per top-level node on the path (instance entries and one group), the
+    // return marker, 7 per prototype level.
+    // Postponed leaves need no stack entries.
     c->spill_cap = 8u * top.max_depth + 7u * max_unit_depth + 16u;
     const uint32_t grid = c->grid_closest > c->grid_occluded ? c->grid_closest : c->grid_occluded;
     c->spill_lanes = (size_t)grid * KP_TRACE_BLOCK;
     KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
     c->two_level = true;
+    c->world_root = has_world ? node_off[0] : KP_INVALID;
     c->n_tlas_nodes = (uint32_t)top.nodes.size();
     c->n_world_prims = n_world;
     c->n_tris_built = (uint32_t)prim_next;
@@ -280,8 +302,8 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     c->n_bvh2 = top.n_bvh2_nodes;
     for (const auto& u : units) c->n_bvh2 += u.n_bvh2_nodes;
     c->max_depth = top.max_depth + max_unit_depth;
-    // A traversal visits every top-level node once and, per placement entered,
-    // each node of its prototype once.
+    // A traversal visits every node of the meshes' BVH and of the top level
+    // once and, per instance entered, each node of its prototype once.
     c->max_steps = (uint32_t)std::min<uint64_t>(steps, 0xFFFFFFF0ull);
     c->built = true;
     render_scene_dirty(c);
@@ -431,6 +453,7 @@ int kp_instance_add(kp_ctx* c, uint32_t blas_id, const float* xform12, uint32_t*
         return kp::fail(c, KP_ERR_INVALID, "kp_instance_add: transform not finite or not invertible");
     const uint32_t id = (uint32_t)c->inst_blas.size();
     c->inst_blas.push_back(blas_id);
+    c->inst_material.push_back(KP_INVALID);
     c->inst_o2w.insert(c->inst_o2w.end(), xform12, xform12 + 12);
     for (int i = 0; i < 12; ++i) c->inst_w2o.push_back((float)inv[i]);
     if (instance_id) *instance_id = id;
@@ -453,6 +476,7 @@ int kp_geometry_clear(kp_ctx* c) {
     c->inst_o2w.clear();
     c->inst_w2o.clear();
     c->inst_blas.clear();
+    c->inst_material.clear();
     c->n_tris_built = c->n_tris_stored = c->n_world_prims = 0;
     c->n_nodes = c->n_bvh2 = c->max_depth = 0;
     kp::render_scene_dirty(c);
@@ -477,16 +501,31 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
         if (o->leaf_size) bo.leaf_size = o->leaf_size;
         if (o->sah_bins) bo.sah_bins = o->sah_bins;
     }
+    std::vector<float> baked;
+    const float* soup = c->soup.data();
+    uint32_t n_all = n;
     if (!c->inst_blas.empty()) {
-        double secs = 0.0;
-        const int rc = kp::build_two_level(c, bo, &secs);
-        if (rc == KP_OK) c->build_seconds = secs;
-        return rc;
+        const uint32_t mode = o ? o->instances : KP_INSTANCES_AUTO;
+        if (mode > KP_INSTANCES_BAKE) return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: bad instances mode");
+        uint64_t placed = 0;
+        for (uint32_t b : c->inst_blas) placed += c->blas_count[b];
+        const bool two = mode == KP_INSTANCES_TWO_LEVEL || (mode == KP_INSTANCES_AUTO && placed > KP_BAKE_BUDGET);
+        if (two) {
+            double secs = 0.0;
+            const int rc = kp::build_two_level(c, bo, &secs);
+            if (rc == KP_OK) c->build_seconds = secs;
+            return rc;
+        }
+        if ((uint64_t)n + placed >= (1ull << 28))
+            return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: baked instances exceed 2^28 triangles");
+        kp::bake_placed(c, baked);
+        soup = baked.data();
+        n_all = (uint32_t)(baked.size() / 9);
     }
     kp::Bvh8Host h;
     std::string err;
     auto t0 = std::chrono::steady_clock::now();
-    if (!kp::build_bvh8(c->soup.data(), n, bo, h, err)) return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: " + err);
+    if (!kp::build_bvh8(soup, n_all, bo, h, err)) return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: " + err);
     auto t1 = std::chrono::steady_clock::now();
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     kp::free_bvh(c);
@@ -501,14 +540,14 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     const uint32_t grid = c->grid_closest > c->grid_occluded ? c->grid_closest : c->grid_occluded;
     c->spill_lanes = (size_t)grid * KP_TRACE_BLOCK;
     KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
-    c->n_tris_built = n;
-    c->n_tris_stored = n;
+    c->n_tris_built = n_all;
+    c->n_tris_stored = n_all;
     c->n_world_prims = n;
     c->n_nodes = (uint32_t)h.nodes.size();
     c->n_bvh2 = h.n_bvh2_nodes;
     c->max_depth = h.max_depth;
     // A traversal visits every node and leaf at most once.
-    c->max_steps = 2u * (uint32_t)h.nodes.size() + n + 64u;
+    c->max_steps = 2u * (uint32_t)h.nodes.size() + n_all + 64u;
     c->build_seconds = std::chrono::duration<double>(t1 - t0).count();
     c->built = true;
     kp::render_scene_dirty(c);
@@ -522,13 +561,14 @@ int kp_get_stats(kp_ctx* c, kp_stats* s) {
     s->n_nodes = c->n_nodes;
     s->n_bvh2_nodes = c->n_bvh2;
     s->bvh_bytes = (uint64_t)c->n_nodes * KP_NODE_BYTES + (uint64_t)c->n_tris_stored * KP_TRI_BYTES +
-                   (c->two_level ? (uint64_t)(c->inst_blas.size() + 1) * 68u : 0u);
+                   (c->two_level ? (uint64_t)c->inst_blas.size() * 68u : 0u);
     s->max_depth = c->max_depth;
     s->spill_entries = c->spill_cap;
     s->grid_closest = c->grid_closest;
     s->grid_occluded = c->grid_occluded;
     s->build_seconds = c->build_seconds;
     s->device_cus = (uint32_t)c->cus;
+    s->two_level = c->two_level ? 1u : 0u;
     return KP_OK;
 }
 

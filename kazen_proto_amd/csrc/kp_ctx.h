@@ -27,11 +27,13 @@ struct kp_ctx {
     std::vector<uint32_t> blas_first, blas_count, blas_material;
     std::vector<float> inst_o2w, inst_w2o;   // 12 per instance, row-major 3x4
     std::vector<uint32_t> inst_blas;
+    std::vector<uint32_t> inst_material;     // KP_INVALID: the prototype's material
 
     // BVH on the device
     bool built = false;
     bool two_level = false;          // built with instances (top-level BVH + prototypes)
     uint32_t n_tlas_nodes = 0;       // nodes [0, n_tlas_nodes) form the top level
+    uint32_t world_root = 0xFFFFFFFFu; // root of the kp_mesh_add meshes' BVH in two-level mode
     uint32_t n_world_prims = 0;      // primitives of the kp_mesh_add meshes
     float4* d_insts = nullptr;       // instance records (4 float4 each)
     uint32_t* d_tlas_inst = nullptr; // top-level leaf position -> instance
@@ -74,6 +76,11 @@ int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, b
 int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out);
 // Traversal arguments of the context's BVH (one- or two-level).
 TraceArgs trace_base(kp_ctx* c);
+// World-space triangles (9 floats each) of every primitive id: the
+// kp_mesh_add meshes, then each instance's prototype under its
+// object-to-world matrix, each coordinate ((m0 x + m1 y) + m2 z) + m3 in FP64
+// rounded to FP32.
+void bake_placed(const kp_ctx* c, std::vector<float>& out);
 void render_state_free(kp_ctx* c);
 void guide_state_free(kp_ctx* c);
 // Marks the integrator's device copy of the scene stale (meshes changed).

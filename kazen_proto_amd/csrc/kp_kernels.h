@@ -27,10 +27,13 @@ struct TraceArgs {
     const uint32_t* roots;  // if set, root node of each ray (batched scenes); else node 0
     // two-level traversal (instances): nodes [0, n_tlas) are the top level,
     // whose leaf children name instances through tlas_inst; insts holds per
-    // instance the world-to-object rows and {root node, primitive base}
+    // instance the world-to-object rows and {root node, primitive base, flags}
+    // world_root is the root of the kp_mesh_add meshes' BVH (KP_INVALID if
+    // none), traversed before the top level.
     const float4* insts;
     const uint32_t* tlas_inst;
     uint32_t n_tlas;
+    uint32_t world_root;
 };
 
 hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStream_t s);

@@ -785,36 +785,25 @@ static int prepare_scene(kp_ctx* c) {
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_scene(r);
     const uint32_t n = c->n_tris_built;
-    if (c->two_level && n >= (1u << 28))
+    const bool has_inst = !c->inst_blas.empty();
+    if (has_inst && n >= (1u << 28))
         return fail(c, KP_ERR_INVALID, "render: instanced scene exceeds 2^28 placed triangles");
-    // World-space triangles of every primitive id: the kp_mesh_add meshes as
-    // stored, then each instance's prototype under its object-to-world matrix
-    // (shading records are per placed triangle; only traversal is instanced).
+    // World-space triangles of every primitive id (instances baked the same
+    // way whether traversal is baked or two-level: shading records are per
+    // placed triangle; only traversal can be shared).
     std::vector<float> placed;
-    if (c->two_level) {
-        placed.resize(9 * (size_t)n);
-        std::memcpy(placed.data(), c->soup.data(), c->soup.size() * sizeof(float));
-        size_t o = c->soup.size();
-        for (size_t k = 0; k < c->inst_blas.size(); ++k) {
-            const uint32_t b = c->inst_blas[k];
-            const float* m = &c->inst_o2w[12 * k];
-            const float* src = &c->blas_soup[9 * (size_t)c->blas_first[b]];
-            for (size_t v = 0; v < 3 * (size_t)c->blas_count[b]; ++v, o += 3)
-                for (int a = 0; a < 3; ++a)
-                    placed[o + a] = (float)((double)m[4 * a] * src[3 * v] + (double)m[4 * a + 1] * src[3 * v + 1] +
-                                            (double)m[4 * a + 2] * src[3 * v + 2] + (double)m[4 * a + 3]);
-        }
-    }
-    const float* world = c->two_level ? placed.data() : c->soup.data();
+    if (has_inst) bake_placed(c, placed);
+    const float* world = has_inst ? placed.data() : c->soup.data();
+    if (has_inst && placed.size() != 9 * (size_t)n) return fail(c, KP_ERR_STATE, "render: BVH out of date");
     // meshes as (first primitive, count, material): kp_mesh_add meshes, then instances
     std::vector<uint32_t> g_first(c->mesh_first), g_count(c->mesh_count), g_mat(c->mesh_material);
-    if (c->two_level) {
-        uint32_t next = c->n_world_prims;
+    if (has_inst) {
+        uint32_t next = (uint32_t)(c->soup.size() / 9);
         for (size_t k = 0; k < c->inst_blas.size(); ++k) {
             const uint32_t b = c->inst_blas[k];
             g_first.push_back(next);
             g_count.push_back(c->blas_count[b]);
-            g_mat.push_back(c->blas_material[b]);
+            g_mat.push_back(c->inst_material[k] != KP_INVALID ? c->inst_material[k] : c->blas_material[b]);
             next += c->blas_count[b];
         }
     }
@@ -1671,6 +1660,14 @@ int kp_mesh_set_material(kp_ctx* c, uint32_t mesh_id, uint32_t material_id) {
     if (!c) return KP_ERR_INVALID;
     if (mesh_id >= c->mesh_material.size()) return kp::fail(c, KP_ERR_INVALID, "bad mesh id");
     c->mesh_material[mesh_id] = material_id;
+    kp::render_scene_dirty(c);
+    return KP_OK;
+}
+
+int kp_instance_set_material(kp_ctx* c, uint32_t instance_id, uint32_t material_id) {
+    if (!c) return KP_ERR_INVALID;
+    if (instance_id >= c->inst_material.size()) return kp::fail(c, KP_ERR_INVALID, "bad instance id");
+    c->inst_material[instance_id] = material_id;
     kp::render_scene_dirty(c);
     return KP_OK;
 }

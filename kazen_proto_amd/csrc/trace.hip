@@ -273,8 +273,16 @@ __device__ __forceinline__ void ray_to_object(const float4* __restrict__ inst, c
 // Special stack entries of the two-level traversal (low byte of y zero; a
 // group entry always has remaining-child bits there):
 //   instance entry  x = instance record, y = bound of its box entry distance
-//   return marker   x = KP_INVALID, y = -inf: back to the world-space ray
+//   return marker   x = kReturnMarker, y = -inf: back to the world-space ray
+//   same-space end  x = kReturnSame, y = -inf: an identity placement ends
+//   top level       x = kEnterTop, y = -inf: the meshes are done, go to the
+//                   top level's root (same space and primitive base)
 constexpr uint32_t kReturnMarker = 0xFFFFFFFFu;
+constexpr uint32_t kReturnSame = 0xFFFFFFFEu;
+constexpr uint32_t kEnterTop = 0xFFFFFFFDu;
+// Instance record flag (bits of the third word of its fourth float4): the
+// world-to-object matrix is exactly the identity, so the ray is used as is.
+constexpr uint32_t kInstIdentity = 1u;
 
 template <int S>
 struct Stack {
@@ -404,16 +412,22 @@ __device__ __forceinline__ void test_leaves(const float4* __restrict__ tris, con
 // or a lane has nothing else to do, so triangle tests run with many lanes
 // active instead of one or two per iteration.
 //
-// Two-level variant (kInst): nodes below args.n_tlas form the top level,
-// whose leaf children are instances.  A hit instance is pushed as an instance
+// Two-level variant (kInst): a ray first traverses the BVH of the meshes
+// (world_root, one-level, in world space) with a top-level marker below it on
+// the stack; nodes below args.n_tlas form the top level, whose leaf children
+// are instances.  Each lane keeps its world-space ray in LDS, read back on
+// entering and leaving an instance.  A hit instance is pushed as an instance
 // entry (culled later like a group by its entry distance); taking it pushes a
 // return marker, moves the ray to the instance's object space and continues
 // at the instance's root; reaching the marker restores the world-space ray
-// (refetched from the ray buffer).  Leaf children are tested at once here (a
-// postponed set would mix ray spaces).
+// (refetched from the ray buffer).  An identity placement keeps the ray and
+// only switches the primitive base.  Postponed leaf children are tested
+// before any of these switches, so a set is always tested in the space and
+// with the primitive base it was found in.
 template <bool kAny, bool kCount, bool kInst>
 __device__ __forceinline__ void traverse(const TraceArgs& args) {
     __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
+    __shared__ float4 wray[kInst ? 2 : 1][kInst ? KP_TRACE_BLOCK : 1];   // world ray per lane (two-level)
     const int tid = threadIdx.x;
     const uint32_t gl = blockIdx.x * KP_TRACE_BLOCK + tid;
     Stack<kStack> st;
@@ -453,6 +467,14 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
             st.sp = 0;
             steps = 0;
             feed.pending = false;
+            if (kInst) {
+                wray[0][tid] = feed.p0;
+                wray[kInst ? 1 : 0][tid] = feed.p1;
+                if (cur != KP_INVALID && args.world_root != KP_INVALID) {
+                    st.push(kEnterTop, 0xFF800000u);   // the top level, after the meshes
+                    cur = args.world_root;
+                }
+            }
         }
         feed.assign();
         if (__ballot(ray_id != KP_INVALID || feed.pending) == 0) break;
@@ -483,9 +505,6 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                                 }
                             }
                         }
-                    } else if (kInst && h.leaves) {
-                        test_leaves<kAny, kCount>(tris, r, h.leaves, h.w1.y, h.w1.z, h.w1.w, best_t, best_p, bu, bv,
-                                                  occl, cnt_tris, inst_pbase);
                     } else if (h.leaves) {
                         if (pend == 0u) {
                             pend = h.leaves;
@@ -494,7 +513,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                             pm1 = h.w1.w;
                         } else {   // this lane already holds a set: test these now
                             test_leaves<kAny, kCount>(tris, r, h.leaves, h.w1.y, h.w1.z, h.w1.w, best_t, best_p, bu,
-                                                      bv, occl, cnt_tris);
+                                                      bv, occl, cnt_tris, inst_pbase);
                         }
                     }
                     if (h.inner && !fault && !(kAny && occl)) {
@@ -521,26 +540,47 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                             continue;
                         }
                         if (kInst && (e.y & 0xFFu) == 0u) {
+                            if (e.x == kEnterTop) {   // same space and base: leaves may stay postponed
+                                st.drop();
+                                cur = 0u;
+                                break;
+                            }
+                            // postponed leaves belong to the current space and
+                            // base: the lane waits (no node) until the wave tests them
+                            if (pend != 0u) break;
                             st.drop();
-                            const float4 wa = args.rays[2 * (size_t)ray_id], wb = args.rays[2 * (size_t)ray_id + 1];
+                            if (e.x == kReturnSame) {
+                                inst_pbase = 0u;
+                                continue;
+                            }
                             if (e.x == kReturnMarker) {   // leave the instance: world-space ray again
+                                const float4 wa = wray[0][tid], wb = wray[kInst ? 1 : 0][tid];
                                 setup_ray(wa, wb, r);
                                 nt = node_test_setup(r);
                                 inst_pbase = 0u;
                                 continue;
                             }
                             const float4* inst = args.insts + 4 * (size_t)e.x;
-                            float4 oa, ob;
-                            ray_to_object(inst, wa, wb, oa, ob);
-                            if (!ray_valid(oa, ob)) continue;   // degenerate in object space: misses
-                            if (!st.push(kReturnMarker, 0xFF800000u)) {
-                                atomicOr(args.err, 1u);
-                                fault = true;
-                                break;
-                            }
                             const float4 rec = inst[3];
-                            setup_ray(oa, ob, r);
-                            nt = node_test_setup(r);
+                            if (__float_as_uint(rec.z) & kInstIdentity) {
+                                if (!st.push(kReturnSame, 0xFF800000u)) {
+                                    atomicOr(args.err, 1u);
+                                    fault = true;
+                                    break;
+                                }
+                            } else {
+                                const float4 wa = wray[0][tid], wb = wray[kInst ? 1 : 0][tid];
+                                float4 oa, ob;
+                                ray_to_object(inst, wa, wb, oa, ob);
+                                if (!ray_valid(oa, ob)) continue;   // degenerate in object space: misses
+                                if (!st.push(kReturnMarker, 0xFF800000u)) {
+                                    atomicOr(args.err, 1u);
+                                    fault = true;
+                                    break;
+                                }
+                                setup_ray(oa, ob, r);
+                                nt = node_test_setup(r);
+                            }
                             inst_pbase = __float_as_uint(rec.y);
                             cur = __float_as_uint(rec.x);
                             break;
@@ -557,14 +597,14 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                 // postponed leaves: tested together once enough lanes hold some,
                 // or when a lane has no node left to visit
                 const bool stuck = pend != 0u && (cur == KP_INVALID || fault);
-                if (!kInst && (__ballot(stuck) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch)) {
+                if (__ballot(stuck) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch) {
                     if (pend != 0u && !(kAny && occl)) {
                         test_leaves<kAny, kCount>(tris, r, pend, pbase, pm0, pm1, best_t, best_p, bu, bv, occl,
-                                                  cnt_tris);
+                                                  cnt_tris, inst_pbase);
                     }
                     pend = 0u;
                 }
-                bool done = fault || (kAny && occl) || (cur == KP_INVALID && pend == 0u);
+                bool done = fault || (kAny && occl) || (cur == KP_INVALID && pend == 0u && st.sp == 0);
                 if (!done && ++steps > args.max_steps) {
                     atomicOr(args.err, 2u);
                     done = true;

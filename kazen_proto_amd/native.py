@@ -33,15 +33,19 @@ class KpError(RuntimeError):
         self.code = code
 
 
+INSTANCES_AUTO, INSTANCES_TWO_LEVEL, INSTANCES_BAKE = 0, 1, 2
+BAKE_BUDGET = 1 << 24
+
+
 class BuildOpts(ctypes.Structure):
-    _fields_ = [("leaf_size", _u32), ("sah_bins", _u32), ("reserved", _u32 * 6)]
+    _fields_ = [("leaf_size", _u32), ("sah_bins", _u32), ("instances", _u32), ("reserved", _u32 * 5)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("n_tris", _u64), ("n_nodes", _u64), ("n_bvh2_nodes", _u64),
                 ("bvh_bytes", _u64), ("max_depth", _u32), ("spill_entries", _u32),
                 ("grid_closest", _u32), ("grid_occluded", _u32), ("build_seconds", ctypes.c_double),
-                ("device_cus", _u32), ("reserved", _u32 * 7)]
+                ("device_cus", _u32), ("two_level", _u32), ("reserved", _u32 * 6)]
 
 
 class Material(ctypes.Structure):
@@ -127,6 +131,7 @@ SIGNATURES = {
     "kp_instance_add": (_i32, [_p, _u32, _p, ctypes.POINTER(_u32)]),
     "kp_instance_get": (_i32, [_p, _u32, ctypes.POINTER(_u32), _p, _p]),
     "kp_geometry_clear": (_i32, [_p]),
+    "kp_instance_set_material": (_i32, [_p, _u32, _u32]),
     "kp_get_stats": (_i32, [_p, _p]),
     "kp_intersect_closest": (_i32, [_p, _p, _u32, _p, _i32]),
     "kp_occluded": (_i32, [_p, _p, _u32, _p, _i32]),
@@ -334,6 +339,9 @@ class Context:
         self._check(lib().kp_instance_add(self._h, blas_id, _np_ptr(xf), ctypes.byref(iid)))
         return iid.value
 
+    def instance_set_material(self, instance_id, material_id):
+        self._check(lib().kp_instance_set_material(self._h, instance_id, material_id))
+
     def instance_get(self, instance_id):
         """(prototype id, object-to-world 3x4, world-to-object 3x4 as traversed)."""
         b = _u32()
@@ -346,10 +354,11 @@ class Context:
         """Drops every mesh, prototype, instance and the BVH (materials stay)."""
         self._check(lib().kp_geometry_clear(self._h))
 
-    def bvh_build(self, leaf_size=4, sah_bins=32):
+    def bvh_build(self, leaf_size=4, sah_bins=32, instances=INSTANCES_AUTO):
         o = BuildOpts()
         o.leaf_size = leaf_size
         o.sah_bins = sah_bins
+        o.instances = instances
         self._check(lib().kp_bvh_build(self._h, ctypes.byref(o)))
 
     def stats(self):
@@ -398,7 +407,7 @@ class Context:
         return out
 
     # ------------------------------------------------------------ render --
-    def load_scene(self, scene, leaf_size=4):
+    def load_scene(self, scene, leaf_size=4, instances=INSTANCES_AUTO):
         """Adds every mesh and material of a scenes.Scene and builds the BVH."""
         mids = []
         for md in scene.materials:
@@ -414,9 +423,12 @@ class Context:
             b = self.blas_add_soup(tris)
             self.blas_set_material(b, mids[mi])
             protos.append(b)
-        for pi, o2w in getattr(scene, "instances", []):
-            self.instance_add(protos[pi], o2w)
-        self.bvh_build(leaf_size=leaf_size)
+        inst_mat = getattr(scene, "instance_material", [])
+        for k, (pi, o2w) in enumerate(getattr(scene, "instances", [])):
+            iid = self.instance_add(protos[pi], o2w)
+            if k < len(inst_mat) and inst_mat[k] is not None:
+                self.instance_set_material(iid, mids[inst_mat[k]])
+        self.bvh_build(leaf_size=leaf_size, instances=instances)
 
     def film_reset(self, width, height):
         self._check(lib().kp_film_reset(self._h, width, height))

@@ -45,6 +45,7 @@ class Scene:
     prototypes: list = field(default_factory=list)
     prototype_material: list = field(default_factory=list)
     instances: list = field(default_factory=list)
+    instance_material: list = field(default_factory=list)   # per instance: material or None (prototype's)
 
     def add_mesh(self, tris9, mat_index):
         self.meshes.append(np.ascontiguousarray(tris9, np.float32).reshape(-1, 9))
@@ -55,18 +56,20 @@ class Scene:
         self.prototype_material.append(int(mat_index))
         return len(self.prototypes) - 1
 
-    def add_instance(self, proto, o2w):
+    def add_instance(self, proto, o2w, mat_index=None):
         self.instances.append((int(proto), np.asarray(o2w, np.float32).reshape(3, 4)))
+        self.instance_material.append(None if mat_index is None else int(mat_index))
         return len(self.instances) - 1
 
     def placed_meshes(self):
         """(world-space [n,9] triangles, material) of every mesh and instance,
         in primitive-id order (instances transformed in float64)."""
         out = list(zip(self.meshes, self.mesh_material))
-        for pi, o2w in self.instances:
+        for k, (pi, o2w) in enumerate(self.instances):
             m = np.asarray(o2w, np.float64)
             v = self.prototypes[pi].reshape(-1, 3).astype(np.float64) @ m[:, :3].T + m[:, 3]
-            out.append((v.reshape(-1, 9).astype(np.float32), self.prototype_material[pi]))
+            mi = self.instance_material[k] if k < len(self.instance_material) else None
+            out.append((v.reshape(-1, 9).astype(np.float32), self.prototype_material[pi] if mi is None else mi))
         return out
 
     @property
@@ -451,14 +454,16 @@ def random_material(rng):
 
 
 def terrain_scene(grid=2048, n_spheres=256, sphere_lon=64, sphere_lat=32, tiles=8, width=1024, height=1024,
-                  spp=64, max_depth=8, rr_depth=3, seed=2, extent=10.0):
+                  spp=64, max_depth=8, rr_depth=3, seed=2, extent=10.0, instanced=False):
     """Full plain path-trace workload: a grid x grid heightfield over
     [-extent, extent]^2 (2 (grid-1)^2 triangles, 8,380,418 at 2048), heights
     1.5 fBm (6 octaves, lacunarity 2, gain 0.5), split into tiles x tiles
     meshes; n_spheres copies of one UV sphere (sphere_lon x sphere_lat, 4,032
     triangles at 64 x 32) at uniform random positions above the ground,
-    radius U[0.15, 0.45], wound outward, flattened into world space; every mesh draws its
-    material with random_material; 4 area-light quads (2 x 2, radiance 30)
+    radius U[0.15, 0.45], wound outward, flattened into world space (or, with
+    instanced, one prototype placed n_spheres times with a material per
+    instance: the same scene, traversed two-level); every mesh and sphere
+    draws its material with random_material; 4 area-light quads (2 x 2, radiance 30)
     at height 7 facing down.  Camera (0, 6, 15) looking at (0, 0, 0), 50
     degree field of view."""
     rng = np.random.default_rng(seed)
@@ -481,6 +486,9 @@ def terrain_scene(grid=2048, n_spheres=256, sphere_lon=64, sphere_lat=32, tiles=
     # uv_sphere winds inward; swap two vertices so winding normals point out
     # (the dielectric's outside is the side its winding normal faces)
     unit = uv_sphere(sphere_lon, sphere_lat).reshape(-1, 3, 3)[:, [0, 2, 1]]
+    proto = None
+    if instanced and n_spheres:
+        proto = s.add_prototype(unit.reshape(-1, 9), 0)
     for _ in range(n_spheres):
         r = rng.uniform(0.15, 0.45)
         cx, cz = rng.uniform(-0.9 * extent, 0.9 * extent, 2)
@@ -489,7 +497,10 @@ def terrain_scene(grid=2048, n_spheres=256, sphere_lon=64, sphere_lat=32, tiles=
         cy = h[iz, ix] + r * rng.uniform(1.0, 2.5)
         m = len(s.materials)
         s.materials.append(random_material(rng))
-        s.add_mesh((unit * r + np.array([cx, cy, cz])).reshape(-1, 9), m)
+        if proto is not None:
+            s.add_instance(proto, [[r, 0, 0, cx], [0, r, 0, cy], [0, 0, r, cz]], m)
+        else:
+            s.add_mesh((unit * r + np.array([cx, cy, cz])).reshape(-1, 9), m)
     light = len(s.materials)
     s.materials.append(material(LAMBERT, albedo=(0, 0, 0), Le=(30, 30, 30)))
     for lx, lz in ((-5, -5), (5, -5), (-5, 5), (5, 5)):

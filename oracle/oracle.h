@@ -58,7 +58,8 @@ uint32_t or_count_near_ties(const float* tris, uint32_t n_tris, const float* ray
  * with the row-major 3x4 world-to-object matrix inst_w2o[12k..]; the ray goes
  * to object space as o'_a = ((m_a0 ox + m_a1 oy) + m_a2 oz) + m_a3 and
  * d'_a = (m_a0 dx + m_a1 dy) + m_a2 dz in FP32 (t keeps its meaning); the
- * hit's prim = inst_base[k] + index in the mesh; ties by the lower prim. */
+ * hit's prim = inst_base[k] + index in the mesh; ties by the lower prim.
+ * An exactly-identity matrix uses the ray as given (no arithmetic). */
 void or_instanced_closest(const float* mesh_tris, const uint32_t* mesh_first,
                           const uint32_t* mesh_count, const float* inst_w2o,
                           const uint32_t* inst_mesh, const uint32_t* inst_base,

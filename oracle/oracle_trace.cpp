@@ -498,14 +498,19 @@ void or_instanced_closest(const float* mesh_tris, const uint32_t* mesh_first, co
             uint32_t best_p = 0xFFFFFFFFu;
             for (uint32_t k = 0; k < n_inst; ++k) {
                 const float* m = inst_w2o + 12 * (size_t)k;
+                static const float eye[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
                 float orr[8];
-                for (int a = 0; a < 3; ++a) {
-                    const float* row = m + 4 * a;
-                    orr[a] = ((row[0] * wr[0] + row[1] * wr[1]) + row[2] * wr[2]) + row[3];
-                    orr[4 + a] = (row[0] * wr[4] + row[1] * wr[5]) + row[2] * wr[6];
+                if (std::memcmp(m, eye, sizeof(eye)) == 0) {
+                    std::memcpy(orr, wr, sizeof(orr));   // identity placement: the ray as given
+                } else {
+                    for (int a = 0; a < 3; ++a) {
+                        const float* row = m + 4 * a;
+                        orr[a] = ((row[0] * wr[0] + row[1] * wr[1]) + row[2] * wr[2]) + row[3];
+                        orr[4 + a] = (row[0] * wr[4] + row[1] * wr[5]) + row[2] * wr[6];
+                    }
+                    orr[3] = wr[3];
+                    orr[7] = wr[7];
                 }
-                orr[3] = wr[3];
-                orr[7] = wr[7];
                 if (!ray_valid(orr)) continue;
                 WRay r = make_wray(orr);
                 const uint32_t mesh = inst_mesh[k];

@@ -1,9 +1,12 @@
-"""Two-level (instanced) traversal on the GPU against the oracle.
+"""Instancing on the GPU against the oracle.
 
-Closest hits must equal oracle.instanced_closest bit for bit, with the
-world-to-object matrices the library traverses with (kp_instance_get);
-occlusion must equal "the oracle finds a hit"; an instanced render must
-match the render of the same scene with the instances baked into meshes."""
+Two-level traversal: closest hits must equal oracle.instanced_closest bit for
+bit, with the world-to-object matrices the library traverses with
+(kp_instance_get); occlusion must equal "the oracle finds a hit".  Baked
+instances: hits must equal the brute force over the placements' triangles
+transformed as the library documents (FP64 ((m0 x + m1 y) + m2 z) + m3,
+rounded to FP32).  Renders with instances (either way) must match the render
+of the same scene with the instances added as meshes."""
 import numpy as np
 import pytest
 
@@ -33,9 +36,11 @@ def random_o2w(rng, spread=6.0, mirror=False):
     return np.hstack([rot @ np.diag(s), rng.uniform(-spread, spread, (3, 1))]).astype(np.float32)
 
 
-def random_rays(rng, n, box=8.0, finite_frac=0.3):
+def random_rays(rng, n, box=8.0, finite_frac=0.3, aim=None):
+    """Origins uniform in the box; directions uniform, or toward points
+    uniform in [-aim, aim]^3 when aim is given."""
     o = rng.uniform(-box, box, (n, 3))
-    d = rng.normal(size=(n, 3))
+    d = rng.normal(size=(n, 3)) if aim is None else rng.uniform(-aim, aim, (n, 3)) - o
     r = np.zeros((n, 8), np.float32)
     r[:, :3], r[:, 4:7] = o, d
     r[:, 3] = 0.0
@@ -46,15 +51,29 @@ def random_rays(rng, n, box=8.0, finite_frac=0.3):
     return r
 
 
-def build(world, protos, placements, leaf=4):
+def build(world, protos, placements, leaf=4, mode=native.INSTANCES_TWO_LEVEL):
     ctx = native.Context(0)
     if world is not None:
         ctx.mesh_add_soup(world)
     ids = [ctx.blas_add_soup(p) for p in protos]
     for pi, o2w in placements:
         ctx.instance_add(ids[pi], o2w)
-    ctx.bvh_build(leaf_size=leaf)
+    ctx.bvh_build(leaf_size=leaf, instances=mode)
     return ctx
+
+
+def baked_soup(world, protos, placements):
+    """Placements transformed as kp_bvh_build bakes them: per coordinate
+    ((m0 x + m1 y) + m2 z) + m3 in float64, rounded to float32."""
+    out = [] if world is None else [np.asarray(world, np.float32)]
+    for pi, o2w in placements:
+        m = np.asarray(o2w, np.float32).astype(np.float64)
+        v = np.asarray(protos[pi], np.float32).reshape(-1, 3).astype(np.float64)
+        w = np.empty_like(v)
+        for a in range(3):
+            w[:, a] = ((m[a, 0] * v[:, 0] + m[a, 1] * v[:, 1]) + m[a, 2] * v[:, 2]) + m[a, 3]
+        out.append(w.astype(np.float32).reshape(-1, 9))
+    return np.concatenate(out)
 
 
 def oracle_hits(ctx, world, protos, placements, rays):
@@ -86,9 +105,10 @@ def test_closest_and_occluded_match_oracle(with_world, leaf):
     rng, world, protos, placements = mixed_scene(11 + leaf, with_world=with_world)
     ctx = build(world, protos, placements, leaf)
     st = ctx.stats()
+    assert st["two_level"] == 1
     n_world = 0 if world is None else world.shape[0]
     assert st["n_tris"] == n_world + sum(protos[p].shape[0] for p, _ in placements)
-    rays = random_rays(rng, 20000)
+    rays = random_rays(rng, 20000, aim=6.0)
     ref = oracle_hits(ctx, world, protos, placements, rays)
     got = ctx.intersect_closest(rays)
     assert mismatches(got, ref) == 0
@@ -101,12 +121,27 @@ def test_closest_and_occluded_match_oracle(with_world, leaf):
     ctx.close()
 
 
+@pytest.mark.parametrize("mode", [native.INSTANCES_AUTO, native.INSTANCES_BAKE])
+def test_baked_instances_match_brute_force(mode):
+    rng, world, protos, placements = mixed_scene(21)
+    ctx = build(world, protos, placements, mode=mode)
+    st = ctx.stats()
+    assert st["two_level"] == 0
+    soup = baked_soup(world, protos, placements)
+    assert st["n_tris"] == soup.shape[0]
+    rays = random_rays(rng, 20000, aim=6.0)
+    ref = oracle.closest_bruteforce(soup, rays)
+    assert mismatches(ctx.intersect_closest(rays), ref) == 0
+    assert np.array_equal(ctx.occluded(rays).astype(bool), ref[:, 1].view(np.uint32) != MISS)
+    ctx.close()
+
+
 def test_device_pointer_path_and_coincident_instances():
     rng = np.random.default_rng(5)
     proto = random_tris(rng, 80, spread=1.0)
     placements = [(0, EYE), (0, EYE), (0, random_o2w(rng, spread=2.0))]
     ctx = build(None, [proto], placements)
-    rays = random_rays(rng, 8000, box=3.0, finite_frac=0.0)
+    rays = random_rays(rng, 8000, box=3.0, finite_frac=0.0, aim=0.8)
     ref = oracle_hits(ctx, None, [proto], placements, rays)
     db = native.DeviceBuffer.from_array(ctx, rays)
     got = ctx.intersect_closest(db).numpy()
@@ -126,7 +161,7 @@ def test_many_instances_deep_top_level():
         m = np.eye(3) * rng.uniform(0.5, 1.5)
         placements.append((0, np.hstack([m, rng.uniform(-8, 8, (3, 1))]).astype(np.float32)))
     ctx = build(None, [proto], placements)
-    rays = random_rays(rng, 4000, box=9.0)
+    rays = random_rays(rng, 4000, box=9.0, aim=8.0)
     ref = oracle_hits(ctx, None, [proto], placements, rays)
     got = ctx.intersect_closest(rays)
     assert mismatches(got, ref) == 0
@@ -183,9 +218,10 @@ def flattened(s):
     return f
 
 
-def render(s, spp):
+def render(s, spp, mode=native.INSTANCES_AUTO):
     ctx = native.Context(0)
-    ctx.load_scene(s)
+    ctx.load_scene(s, instances=mode)
+    assert ctx.stats()["two_level"] == (1 if mode == native.INSTANCES_TWO_LEVEL and s.instances else 0)
     ctx.film_reset(s.width, s.height)
     ctx.render_accumulate(native.make_render_desc(s, spp=spp))
     img = ctx.film_read(s.width, s.height)
@@ -193,9 +229,10 @@ def render(s, spp):
     return img
 
 
-def test_instanced_render_matches_flattened():
+@pytest.mark.parametrize("mode", [native.INSTANCES_TWO_LEVEL, native.INSTANCES_AUTO])
+def test_instanced_render_matches_flattened(mode):
     s = instanced_room()
-    a = render(s, 64)
+    a = render(s, 64, mode)
     b = render(flattened(s), 64)
     assert np.isfinite(a).all() and a.mean() > 0.01
     # same sampling protocol and nearly identical hits: the images agree
@@ -207,7 +244,7 @@ def test_instanced_render_matches_flattened():
 
 def test_instanced_render_matches_oracle_statistically():
     s = instanced_room(spp=16, size=24)
-    img = render(s, 256)
+    img = render(s, 256, native.INSTANCES_TWO_LEVEL)
     f = flattened(s)
     mats = [oracle.material(m["type"], m["dist"], m["albedo"], m["F0"], m["alpha"], m["eta"], m["Le"])
             for m in f.materials]
@@ -221,3 +258,16 @@ def test_instanced_render_matches_oracle_statistically():
     assert abs(img.mean() - ref.mean()) < 0.05 * ref.mean()
     rel = float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
     assert rel < 0.1
+
+
+def test_instanced_terrain_matches_flattened():
+    """The terrain workload with its spheres as one prototype placed with a
+    material per instance renders the same image as with the spheres baked."""
+    kw = dict(grid=129, n_spheres=24, sphere_lon=16, sphere_lat=8, tiles=2, width=40, height=40, spp=16,
+              max_depth=6)
+    a = render(scenes.terrain_scene(instanced=True, **kw), 16, native.INSTANCES_TWO_LEVEL)
+    b = render(scenes.terrain_scene(**kw), 16)
+    assert np.isfinite(a).all() and a.mean() > 0.01
+    close = np.abs(a - b) <= 1e-3 * np.maximum(np.abs(b), 1e-2)
+    assert close.mean() > 0.9
+    assert abs(a.mean() - b.mean()) < 0.02 * b.mean()
