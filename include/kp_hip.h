@@ -313,6 +313,12 @@ typedef struct {
     uint32_t pass_paths;   /* >= 1; 262144 */
     uint32_t train_passes; /* passes that train (then the network is frozen); 0 = every pass */
     float bmin[3], bmax[3];
+    float weight_clamp;    /* training weights capped at this multiple of the batch's mean
+                              (fewer rare-path outliers steering a step); 0 = no cap */
+    uint32_t relative_weights; /* 1: a training sample counts by its path's contribution
+                                  relative to its pixel's current estimate R, divided by
+                                  sqrt(R^2 + 0.01) (the per-pixel relative error the image
+                                  is judged by); 0: by its incident radiance */
 } kp_guiding_desc;
 /* Needs kp_guiding_init first; a later kp_guiding_init keeps the settings. */
 int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);

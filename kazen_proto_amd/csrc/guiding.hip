@@ -363,11 +363,25 @@ __global__ __launch_bounds__(256) void k_guide_reduce_seg(const float* __restric
 
 // One block: sums weights[0..n) in a fixed order (FP64), then writes the
 // batch count, the loss scale 1 / sum and the number of positive weights.
-__global__ __launch_bounds__(1024) void k_guide_normalise(const float* __restrict__ weights, uint32_t n,
+// Sum and count of the positive record weights (one block); with clamp > 0,
+// weights above clamp x (mean positive weight) are first capped in place.
+__global__ __launch_bounds__(1024) void k_guide_normalise(float* __restrict__ weights, uint32_t n, float clamp,
                                                           uint32_t* count, float* scale, uint32_t* valid) {
     __shared__ double ssum[1024];
     __shared__ uint32_t scnt[1024];
     const int t = threadIdx.x;
+    auto reduce = [&](double acc, uint32_t nv) {
+        ssum[t] = acc;
+        scnt[t] = nv;
+        __syncthreads();
+        for (int h = 512; h > 0; h >>= 1) {
+            if (t < h) {
+                ssum[t] += ssum[t + h];
+                scnt[t] += scnt[t + h];
+            }
+            __syncthreads();
+        }
+    };
     double acc = 0.0;
     uint32_t nv = 0;
     for (uint32_t i = t; i < n; i += 1024) {
@@ -375,20 +389,26 @@ __global__ __launch_bounds__(1024) void k_guide_normalise(const float* __restric
         acc += (double)w;
         nv += w > 0.0f ? 1u : 0u;
     }
-    ssum[t] = acc;
-    scnt[t] = nv;
+    reduce(acc, nv);
+    const double total = ssum[0];
+    const uint32_t nvalid = scnt[0];
     __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-        if (t < h) {
-            ssum[t] += ssum[t + h];
-            scnt[t] += scnt[t + h];
+    double kept = total;
+    if (clamp > 0.0f && nvalid > 0u) {
+        const float cap = (float)(clamp * total / (double)nvalid);
+        acc = 0.0;
+        for (uint32_t i = t; i < n; i += 1024) {
+            const float w = fminf(weights[i], cap);
+            weights[i] = w;
+            acc += (double)w;
         }
-        __syncthreads();
+        reduce(acc, 0u);
+        kept = ssum[0];
     }
     if (t == 0) {
         *count = n;
-        *scale = ssum[0] > 0.0 ? (float)(1.0 / ssum[0]) : 0.0f;
-        *valid = scnt[0];
+        *scale = kept > 0.0 ? (float)(1.0 / kept) : 0.0f;
+        *valid = nvalid;
     }
 }
 
@@ -506,8 +526,8 @@ int guide_train_device(kp_ctx* c, float lr, bool apply) {
 
 int guide_normalise_records(kp_ctx* c, uint32_t n) {
     GuideState* g = c->guide;
-    hipLaunchKernelGGL(k_guide_normalise, dim3(1), dim3(1024), 0, c->stream, g->weights, n, g->count, g->scale,
-                       g->rec_valid);
+    hipLaunchKernelGGL(k_guide_normalise, dim3(1), dim3(1024), 0, c->stream, g->weights, n, g->weight_clamp,
+                       g->count, g->scale, g->rec_valid);
     KP_HIP_TRY(c, hipGetLastError());
     return KP_OK;
 }
@@ -656,8 +676,8 @@ int kp_guiding_configure(kp_ctx* c, const kp_guiding_desc* d) {
     if (rc) return rc;
     if (!(d->bsdf_fraction > 0.0f && d->bsdf_fraction <= 1.0f))
         return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: bsdf_fraction must be in (0, 1]");
-    if (!(d->min_alpha >= 0.0f) || !(d->lr >= 0.0f))
-        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: min_alpha and lr must be >= 0");
+    if (!(d->min_alpha >= 0.0f) || !(d->lr >= 0.0f) || !(d->weight_clamp >= 0.0f))
+        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: min_alpha, lr and weight_clamp must be >= 0");
     if (d->max_records == 0 || d->max_records > c->guide->max_batch || d->pass_paths == 0)
         return kp::fail(c, KP_ERR_INVALID, "kp_guiding_configure: max_records must be 1..262144, pass_paths >= 1");
     for (int i = 0; i < 3; ++i)
@@ -666,6 +686,8 @@ int kp_guiding_configure(kp_ctx* c, const kp_guiding_desc* d) {
     GuideState* g = c->guide;
     g->bsdf_fraction = d->bsdf_fraction;
     g->min_alpha = d->min_alpha;
+    g->weight_clamp = d->weight_clamp;
+    g->relative_weights = d->relative_weights != 0;
     g->lr = d->lr;
     g->train_steps = d->train_steps;
     g->guide_after = d->guide_after;

@@ -34,6 +34,8 @@ struct GuideState {
     float3 bmin{0, 0, 0}, binv{1, 1, 1};
     // guided rendering (kp_guiding_configure)
     float bsdf_fraction = 0.5f, min_alpha = 0.5f, lr = 1e-3f;
+    float weight_clamp = 0.0f;   // cap on training weights, multiple of the batch mean (0: none)
+    bool relative_weights = false;   // weights relative to the pixel estimate
     uint32_t train_steps = 1, guide_after = 1, max_records = 1u << 18, pass_paths = 1u << 18;
     uint32_t max_train_passes = 0;   // 0 = every pass trains
     bool auto_box = true;

@@ -644,9 +644,14 @@ __global__ __launch_bounds__(256) void k_connect(RenderArgs a) {
 // (i / max_depth) * rec_stride + rec_phase and vertex i % max_depth + 1; its
 // weight is the luminance of (radiance at the end of the path - radiance
 // before the vertex's sampled direction) / throughput before the vertex, an
-// estimate of f cos L_i / p for the sampled direction.  Records that were not
-// completed get weight 0 and finite inputs.
-__global__ __launch_bounds__(256) void k_guide_records(RenderArgs a, uint32_t n_rec, float* weights) {
+// estimate of f cos L_i / p for the sampled direction.  With film != nullptr
+// the weight is instead the luminance of the radiance gathered after the
+// vertex (throughput included) over sqrt(R^2 + 0.01), R the luminance of the
+// pixel's current estimate (film sum x inv_count): the sample's share of the
+// per-pixel relative error.  Records that were not completed get weight 0 and
+// finite inputs.
+__global__ __launch_bounds__(256) void k_guide_records(RenderArgs a, uint32_t n_rec, float* weights,
+                                                       const float4* film, float inv_count) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n_rec) return;
     const uint32_t slot = (i / a.max_depth) * a.rec_stride + a.rec_phase;
@@ -655,9 +660,16 @@ __global__ __launch_bounds__(256) void k_guide_records(RenderArgs a, uint32_t n_
     const bool made = x0.w == 1.0f && x1.w == 1.0f && slot < a.n_paths;
     if (made) {
         const float4 le = a.ps.s1[slot];
-        if (x1.x > 0.0f) w += 0.2126f * (le.x - x0.x) / x1.x;
-        if (x1.y > 0.0f) w += 0.7152f * (le.y - x0.y) / x1.y;
-        if (x1.z > 0.0f) w += 0.0722f * (le.z - x0.z) / x1.z;
+        if (film) {
+            const float4 f = film[slot / a.k];
+            const float R = (0.2126f * f.x + 0.7152f * f.y + 0.0722f * f.z) * inv_count;
+            w = (0.2126f * (le.x - x0.x) + 0.7152f * (le.y - x0.y) + 0.0722f * (le.z - x0.z)) /
+                sqrtf(R * R + 0.01f);
+        } else {
+            if (x1.x > 0.0f) w += 0.2126f * (le.x - x0.x) / x1.x;
+            if (x1.y > 0.0f) w += 0.7152f * (le.y - x0.y) / x1.y;
+            if (x1.z > 0.0f) w += 0.0722f * (le.z - x0.z) / x1.z;
+        }
         if (!(w > 0.0f) || !isfinite(w)) w = 0.0f;
     } else {
         float* in6 = a.rec_in6 + 6 * (size_t)i;
@@ -1211,8 +1223,9 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
                            r->film_comp);
         KP_HIP_TRY(c, hipGetLastError());
         if (train) {
+            const float inv_count = 1.0f / (float)(r->film_count + (uint64_t)(pass + 1) * k);
             hipLaunchKernelGGL(k_guide_records, dim3((n_rec + 255) / 256), dim3(256), 0, c->stream, a, n_rec,
-                               g->weights);
+                               g->weights, g->relative_weights ? (const float4*)r->film_sum : nullptr, inv_count);
             KP_HIP_TRY(c, hipGetLastError());
             if ((rc = guide_normalise_records(c, n_rec))) return rc;
             for (uint32_t sidx = 0; sidx < g->train_steps; ++sidx)

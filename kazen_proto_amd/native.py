@@ -84,7 +84,8 @@ class GuidingDesc(ctypes.Structure):
     _fields_ = [("bsdf_fraction", ctypes.c_float), ("min_alpha", ctypes.c_float),
                 ("lr", ctypes.c_float), ("train_steps", _u32), ("guide_after", _u32),
                 ("max_records", _u32), ("pass_paths", _u32), ("train_passes", _u32),
-                ("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3)]
+                ("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3), ("weight_clamp", ctypes.c_float),
+                ("relative_weights", _u32)]
 
 
 def make_material(d):
@@ -507,7 +508,8 @@ class Context:
         return g
 
     def guiding_configure(self, bsdf_fraction=0.5, min_alpha=0.5, lr=1e-3, train_steps=1, guide_after=1,
-                          max_records=1 << 18, pass_paths=1 << 18, box=None, train_passes=0):
+                          max_records=1 << 18, pass_paths=1 << 18, box=None, train_passes=0, weight_clamp=0.0,
+                          relative_weights=False):
         """Guided-render settings (kp_guiding_configure); box = (lo, hi) or None for the scene's bounds."""
         d = GuidingDesc()
         d.bsdf_fraction = bsdf_fraction
@@ -518,6 +520,8 @@ class Context:
         d.max_records = max_records
         d.pass_paths = pass_paths
         d.train_passes = train_passes
+        d.weight_clamp = weight_clamp
+        d.relative_weights = 1 if relative_weights else 0
         if box is not None:
             d.bmin[:] = [float(x) for x in box[0]]
             d.bmax[:] = [float(x) for x in box[1]]
