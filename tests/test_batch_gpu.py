@@ -113,3 +113,25 @@ def test_batch_rejects_guided_and_bad_input(setup):
         b.set_tuning(pass_paths=16)
     with pytest.raises(native.KpError):
         b.film_read(len(ss), 4, 4)
+
+
+def test_batch_with_baked_instances_and_two_level_rejected():
+    """A scene whose instances were baked into its one-level BVH batches like
+    any other (films bit-identical to its render alone); a two-level scene is
+    rejected by kp_batch_create."""
+    inst = scenes.terrain_scene(grid=64, n_spheres=6, sphere_lon=12, sphere_lat=6, width=32, height=24, spp=4,
+                                max_depth=5, seed=23, instanced=True)
+    ss = [inst, scenes.cornell_box(width=24, height=24, spp=4, sphere_lon=16, sphere_lat=8)]
+    alone = [_alone(s) for s in ss]
+    owner = native.Context(0)
+    b = _batch(owner, ss)
+    b.render([native.make_render_desc(s) for s in ss])
+    _check(b, ss, alone, "baked instances")
+    b.close()
+    two = native.Context(0)
+    two.load_scene(inst, instances=native.INSTANCES_TWO_LEVEL)
+    assert two.stats()["two_level"] == 1
+    with pytest.raises(native.KpError):
+        native.Batch(owner, [two])
+    two.close()
+    owner.close()
