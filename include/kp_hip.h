@@ -141,6 +141,12 @@ int kp_instance_add(kp_ctx* ctx, uint32_t blas_id, const float* xform12, uint32_
 /* Material of one instance, overriding its prototype's
  * (kp_blas_set_material) for the integrator. */
 int kp_instance_set_material(kp_ctx* ctx, uint32_t instance_id, uint32_t material_id);
+/* Where a hit's primitive id comes from: *kind = 0 for a kp_mesh_add mesh
+ * (*index = mesh id), 1 for an instance (*index = instance id); *local = the
+ * triangle's index within that mesh or prototype. Valid after kp_bvh_build.
+ * Replaces the (geomID, instID, primID) of an Embree hit
+ * (src/core/accel.cpp:374-399). Any output may be NULL. */
+int kp_prim_locate(kp_ctx* ctx, uint32_t prim, uint32_t* kind, uint32_t* index, uint32_t* local);
 /* Removes every mesh, prototype and instance and the BVH (materials, film
  * and guiding state stay), so a scene can be rebuilt from scratch.
  * (The reference rebuilds by constructing a new Embree scene; no counterpart

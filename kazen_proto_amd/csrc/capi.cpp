@@ -460,6 +460,29 @@ int kp_instance_add(kp_ctx* c, uint32_t blas_id, const float* xform12, uint32_t*
     return KP_OK;
 }
 
+int kp_prim_locate(kp_ctx* c, uint32_t prim, uint32_t* kind, uint32_t* index, uint32_t* local) {
+    if (!c) return KP_ERR_INVALID;
+    if (!c->built) return kp::fail(c, KP_ERR_STATE, "kp_prim_locate before kp_bvh_build");
+    if (prim >= c->n_tris_built) return kp::fail(c, KP_ERR_INVALID, "kp_prim_locate: primitive id out of range");
+    uint32_t k, i, l;
+    if (prim < c->n_world_prims) {
+        // meshes are contiguous in insertion order
+        const auto it = std::upper_bound(c->mesh_first.begin(), c->mesh_first.end(), prim);
+        i = (uint32_t)(it - c->mesh_first.begin()) - 1u;
+        k = 0;
+        l = prim - c->mesh_first[i];
+    } else {
+        const auto it = std::upper_bound(c->inst_first.begin(), c->inst_first.end(), prim);
+        i = (uint32_t)(it - c->inst_first.begin()) - 1u;
+        k = 1;
+        l = prim - c->inst_first[i];
+    }
+    if (kind) *kind = k;
+    if (index) *index = i;
+    if (local) *local = l;
+    return KP_OK;
+}
+
 int kp_geometry_clear(kp_ctx* c) {
     if (!c) return KP_ERR_INVALID;
     KP_HIP_TRY(c, hipSetDevice(c->device));
@@ -477,6 +500,7 @@ int kp_geometry_clear(kp_ctx* c) {
     c->inst_w2o.clear();
     c->inst_blas.clear();
     c->inst_material.clear();
+    c->inst_first.clear();
     c->n_tris_built = c->n_tris_stored = c->n_world_prims = 0;
     c->n_nodes = c->n_bvh2 = c->max_depth = 0;
     kp::render_scene_dirty(c);
@@ -500,6 +524,15 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     if (o) {
         if (o->leaf_size) bo.leaf_size = o->leaf_size;
         if (o->sah_bins) bo.sah_bins = o->sah_bins;
+    }
+    // first primitive id of every instance (the same in both instance modes)
+    c->inst_first.resize(c->inst_blas.size());
+    {
+        uint64_t next = n;
+        for (size_t k = 0; k < c->inst_blas.size(); ++k) {
+            c->inst_first[k] = (uint32_t)std::min<uint64_t>(next, 0xFFFFFFFFu);
+            next += c->blas_count[c->inst_blas[k]];
+        }
     }
     std::vector<float> baked;
     const float* soup = c->soup.data();

@@ -28,6 +28,7 @@ struct kp_ctx {
     std::vector<float> inst_o2w, inst_w2o;   // 12 per instance, row-major 3x4
     std::vector<uint32_t> inst_blas;
     std::vector<uint32_t> inst_material;     // KP_INVALID: the prototype's material
+    std::vector<uint32_t> inst_first;        // first primitive id of each instance (set by kp_bvh_build)
 
     // BVH on the device
     bool built = false;

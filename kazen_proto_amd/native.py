@@ -133,6 +133,7 @@ SIGNATURES = {
     "kp_instance_get": (_i32, [_p, _u32, ctypes.POINTER(_u32), _p, _p]),
     "kp_geometry_clear": (_i32, [_p]),
     "kp_instance_set_material": (_i32, [_p, _u32, _u32]),
+    "kp_prim_locate": (_i32, [_p, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "kp_get_stats": (_i32, [_p, _p]),
     "kp_intersect_closest": (_i32, [_p, _p, _u32, _p, _i32]),
     "kp_occluded": (_i32, [_p, _p, _u32, _p, _i32]),
@@ -342,6 +343,12 @@ class Context:
 
     def instance_set_material(self, instance_id, material_id):
         self._check(lib().kp_instance_set_material(self._h, instance_id, material_id))
+
+    def prim_locate(self, prim):
+        """(kind, index, local): kind 0 = kp_mesh_add mesh, 1 = instance."""
+        k, i, l = _u32(), _u32(), _u32()
+        self._check(lib().kp_prim_locate(self._h, prim, ctypes.byref(k), ctypes.byref(i), ctypes.byref(l)))
+        return k.value, i.value, l.value
 
     def instance_get(self, instance_id):
         """(prototype id, object-to-world 3x4, world-to-object 3x4 as traversed)."""

@@ -118,6 +118,17 @@ def test_closest_and_occluded_match_oracle(with_world, leaf):
     assert np.array_equal(occ.astype(bool), hit)
     got2, nodes, tris = ctx.intersect_closest(rays, counted=True)
     assert mismatches(got2, ref) == 0 and nodes > rays.shape[0] and tris > 0
+    # every hit primitive maps back to its mesh or instance and local triangle
+    firsts = np.cumsum([0] + [protos[p].shape[0] for p, _ in placements])[:-1] + n_world
+    for prim in np.unique(got[hit, 1].view(np.uint32))[::7]:
+        kind, index, local = ctx.prim_locate(int(prim))
+        if prim < n_world:
+            assert (kind, index, local) == (0, 0, prim)
+        else:
+            k = int(np.searchsorted(firsts, prim, side="right") - 1)
+            assert (kind, index, local) == (1, k, prim - firsts[k])
+    with pytest.raises(native.KpError):
+        ctx.prim_locate(st["n_tris"])
     ctx.close()
 
 
