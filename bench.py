@@ -104,6 +104,8 @@ def guiding_equal_error(args):
                 break
             if spp >= 64:
                 step_spp = 16
+            if spp >= 1024:
+                step_spp = 64
         t_hit = None
         if curve[-1][2] <= args.guiding_target:
             if len(curve) == 1:
@@ -256,10 +258,10 @@ def main():
     ap.add_argument("--guiding-target", type=float, default=0.01)
     ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
     ap.add_argument("--guiding-lr", type=float, default=2e-3)
-    ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
-    ap.add_argument("--guiding-pass-paths", type=int, default=1 << 21,
+    ap.add_argument("--guiding-steps", type=int, default=16, help="Adam steps per pass")
+    ap.add_argument("--guiding-pass-paths", type=int, default=1 << 24,
                     help="paths per guided pass (each pass trains once)")
-    ap.add_argument("--guiding-train-passes", type=int, default=256,
+    ap.add_argument("--guiding-train-passes", type=int, default=64,
                     help="passes that train the network before it is frozen (0 = all)")
     ap.add_argument("--guiding-after", type=int, default=2, help="training passes before guided sampling")
     ap.add_argument("--seed-guide", type=int, default=17)

@@ -189,6 +189,13 @@ void kp_bvh8_blob_free(kp_bvh8_blob* b);
 /* Tuning: refill a wave once fewer than `lanes` (1..64) of its lanes are
  * busy (default 48). */
 int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
+/* Tuning: at most `max_paths` paths (2^16 .. 2^28) in flight per pass of
+ * kp_render / kp_render_accumulate and per chunk of kp_integrate_rays
+ * (default 2^26: 13.4 GB of path state, 200 B per path). Larger passes keep
+ * every bounce's launches full (a pass's later bounces hold only the paths
+ * still alive); results do not depend on it. Guided renders use
+ * kp_guiding_desc.pass_paths instead. */
+int kp_set_pass_paths(kp_ctx* ctx, uint32_t max_paths);
 
 /* Integrator -------------------------------------------------------------- */
 #define KP_MAT_LAMBERT 0      /* Diffuse (src/shading/bsdfs.cpp:27-41) */
@@ -345,7 +352,7 @@ int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);
  * node width is 8 in every configuration. */
 typedef struct kp_batch kp_batch;
 typedef struct {
-    uint32_t pass_paths;           /* paths in flight per pass (queue size), 256 .. 2^24 */
+    uint32_t pass_paths;           /* paths in flight per pass (queue size), 256 .. 2^26 */
     uint32_t shade_blocks_per_cu;  /* 256-thread shade / connect blocks per CU */
     uint32_t trace_blocks_per_cu;  /* persistent traversal blocks per CU (occupancy) */
     uint32_t refill_below;         /* traversal wave refill threshold, lanes 1..64 */

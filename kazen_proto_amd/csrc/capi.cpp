@@ -770,4 +770,12 @@ int kp_set_trace_tuning(kp_ctx* c, uint32_t lanes) {
     return KP_OK;
 }
 
+int kp_set_pass_paths(kp_ctx* c, uint32_t max_paths) {
+    if (!c) return KP_ERR_INVALID;
+    if (max_paths < (1u << 16) || max_paths > (1u << 28))
+        return kp::fail(c, KP_ERR_INVALID, "kp_set_pass_paths: max_paths must be 2^16 .. 2^28");
+    c->pass_paths = max_paths;
+    return KP_OK;
+}
+
 }  // extern "C"

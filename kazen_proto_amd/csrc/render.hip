@@ -1091,7 +1091,7 @@ static int integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_re
     int rc = prepare_scene(c);
     if (rc) return rc;
     RenderState* r = rstate(c);
-    const uint32_t chunk_cap = 1u << 21;
+    const uint32_t chunk_cap = c->pass_paths;
     const uint32_t cap = n < chunk_cap ? n : chunk_cap;
     if ((rc = ensure_paths(c, cap))) return rc;
     float4* d_in = nullptr;
@@ -1161,7 +1161,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     int rc = prepare_scene(c);
     if (rc) return rc;
     const uint32_t npix = d->width * d->height;
-    const uint32_t k = samples_per_pass(npix, d->spp, guided ? g->pass_paths : (1u << 21));
+    const uint32_t k = samples_per_pass(npix, d->spp, guided ? g->pass_paths : c->pass_paths);
     const uint32_t n_paths = npix * k;
     if ((rc = ensure_paths(c, n_paths))) return rc;
     RenderArgs a{};
@@ -1406,7 +1406,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
     e = hipMalloc((void**)&b->spill, (size_t)grid * KP_TRACE_BLOCK * b->spill_cap * sizeof(uint2));
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return bail(e, "kp_batch_create: spill / rebase");
-    b->tune.pass_paths = 1u << 21;
+    b->tune.pass_paths = 1u << 24;
     b->tune.shade_blocks_per_cu = 8;
     b->tune.trace_blocks_per_cu = c->grid_closest / (uint32_t)c->cus;
     b->tune.refill_below = c->refill_below;
@@ -1637,7 +1637,7 @@ static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t p
         }
         return KP_OK;
     };
-    if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 19, 1u << 20, 1u << 21, 1u << 22}))) return rc;
+    if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25}))) return rc;
     if ((rc = try_values(&kp_batch_tuning::trace_blocks_per_cu, {tb_max, std::max(1u, tb_max / 2), std::max(1u, tb_max - 1)})))
         return rc;
     if ((rc = try_values(&kp_batch_tuning::refill_below, {16u, 32u, 48u, 56u}))) return rc;
@@ -1773,7 +1773,7 @@ int kp_batch_destroy(kp_batch* h) {
 int kp_batch_set_tuning(kp_batch* h, const kp_batch_tuning* t) {
     if (!h || !t) return KP_ERR_INVALID;
     kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
-    if (t->pass_paths < 256 || t->pass_paths > (1u << 24) || t->shade_blocks_per_cu == 0 ||
+    if (t->pass_paths < 256 || t->pass_paths > (1u << 26) || t->shade_blocks_per_cu == 0 ||
         t->shade_blocks_per_cu > 64 || t->trace_blocks_per_cu == 0 || t->refill_below == 0 || t->refill_below > 64)
         return kp::fail(b->c, KP_ERR_INVALID, "kp_batch_set_tuning: value out of range");
     b->tune = *t;
