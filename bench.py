@@ -199,6 +199,7 @@ def batch_leg(args):
         ss.append(scenes.terrain_scene(grid=grid, n_spheres=16, sphere_lon=16, sphere_lat=8, width=res,
                                        height=res, spp=spp, max_depth=8, seed=100 + i))
     gen = time.time() - t0
+    print("[bench] batch: %d scenes generated (%.1fs)" % (len(ss), gen), file=sys.stderr, flush=True)
     owner = native.Context(0)
     ctxs = []
     for s in ss:
@@ -206,15 +207,22 @@ def batch_leg(args):
         c.load_scene(s)
         ctxs.append(c)
     descs = [native.make_render_desc(s) for s in ss]
-    # one scene after another (each render is its own sequence of launches)
+    print("[bench] batch: scenes loaded (%.1fs)" % (time.time() - t0), file=sys.stderr, flush=True)
+    # one scene after another (each render is its own sequence of launches);
+    # each context frees its path state afterwards, so at most one is held
     seq_sec = 0.0
     for c, d in zip(ctxs, descs):
         c.render(d)
         seq_sec += c.render_stats()["seconds"]
+        c.release_buffers()
+    print("[bench] batch: one by one %.3fs (%.1fs)" % (seq_sec, time.time() - t0), file=sys.stderr, flush=True)
     b = native.Batch(owner, ctxs)
+    print("[bench] batch: created (%.1fs)" % (time.time() - t0), file=sys.stderr, flush=True)
     for c in ctxs:
         c.close()
+    print("[bench] batch: scene contexts closed (%.1fs)" % (time.time() - t0), file=sys.stderr, flush=True)
     tuned = b.autotune(descs, passes=2).as_dict()
+    print("[bench] batch: autotuned %s (%.1fs)" % (tuned, time.time() - t0), file=sys.stderr, flush=True)
     b.render(descs)                 # warm-up with the tuned settings
     b.render(descs)
     st = b.stats()

@@ -196,6 +196,10 @@ int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
  * still alive); results do not depend on it. Guided renders use
  * kp_guiding_desc.pass_paths instead. */
 int kp_set_pass_paths(kp_ctx* ctx, uint32_t max_paths);
+/* Frees the context's cached work buffers (integrator path state, host-call
+ * staging); the next call that needs them allocates them again. Scene, BVH,
+ * film and guiding state stay. For callers that keep many contexts. */
+int kp_release_buffers(kp_ctx* ctx);
 
 /* Integrator -------------------------------------------------------------- */
 #define KP_MAT_LAMBERT 0      /* Diffuse (src/shading/bsdfs.cpp:27-41) */
@@ -352,7 +356,7 @@ int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);
  * node width is 8 in every configuration. */
 typedef struct kp_batch kp_batch;
 typedef struct {
-    uint32_t pass_paths;           /* paths in flight per pass (queue size), 256 .. 2^26 */
+    uint32_t pass_paths;           /* paths in flight per pass (queue size), 256 .. 2^22 */
     uint32_t shade_blocks_per_cu;  /* 256-thread shade / connect blocks per CU */
     uint32_t trace_blocks_per_cu;  /* persistent traversal blocks per CU (occupancy) */
     uint32_t refill_below;         /* traversal wave refill threshold, lanes 1..64 */

@@ -29,6 +29,9 @@
 #include <cstring>
 #include <vector>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include "../../include/kp_hip.h"
 #include "kp_closures.h"
 #include "kp_ctx.h"
@@ -1309,6 +1312,13 @@ static void batch_free(BatchState* b) {
     delete b;
 }
 
+// Progress lines on stderr when the environment sets KP_VERBOSE (batch set-up
+// and autotuning can take seconds with nothing else to show).
+static bool verbose() {
+    static const bool v = std::getenv("KP_VERBOSE") != nullptr;
+    return v;
+}
+
 static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState** out) {
     uint64_t nn = 0, nt = 0, nm = 0, nl = 0, nlt = 0, ncdf = 0;
     uint32_t depth = 0;
@@ -1406,11 +1416,12 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
     e = hipMalloc((void**)&b->spill, (size_t)grid * KP_TRACE_BLOCK * b->spill_cap * sizeof(uint2));
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return bail(e, "kp_batch_create: spill / rebase");
-    b->tune.pass_paths = 1u << 24;
+    b->tune.pass_paths = 1u << 21;
     b->tune.shade_blocks_per_cu = 8;
     b->tune.trace_blocks_per_cu = c->grid_closest / (uint32_t)c->cus;
     b->tune.refill_below = c->refill_below;
     b->tune.use_graph = 1;
+    if (verbose()) std::fprintf(stderr, "[kp] batch: %u scenes merged\n", n);
     *out = b;
     return KP_OK;
 }
@@ -1559,6 +1570,9 @@ static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max
     tc.roots = true;
     uint64_t passes = (work + chunk - 1) / chunk;
     if (max_passes && passes > max_passes) passes = max_passes;
+    if (verbose())
+        std::fprintf(stderr, "[kp] batch render: %llu passes of %u paths, graph %u\n", (unsigned long long)passes,
+                     chunk, b->tune.use_graph);
     auto t0 = std::chrono::steady_clock::now();
     if (b->tune.use_graph) {
         const std::vector<uint64_t> key = {chunk, n, max_depth, tc.grid_closest, tc.grid_occluded, tc.shade_blocks,
@@ -1612,10 +1626,14 @@ static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t p
     auto cost = [&](const kp_batch_tuning& t, double* sec_per_path) -> int {
         b->tune = t;
         uint32_t run = 0;
+        if (verbose())
+            std::fprintf(stderr, "[kp] autotune: pass_paths %u trace %u shade %u refill %u\n", t.pass_paths,
+                         t.trace_blocks_per_cu, t.shade_blocks_per_cu, t.refill_below);
         int rc = batch_render(b, descs, 1, &run);   // warm-up: allocation, graph capture
         if (rc) return rc;
         if ((rc = batch_render(b, descs, passes, &run))) return rc;
         *sec_per_path = b->stats.seconds / (double)std::max<uint64_t>(1, b->stats.paths);
+        if (verbose()) std::fprintf(stderr, "[kp] autotune:   %.3f ns per path\n", *sec_per_path * 1e9);
         return KP_OK;
     };
     kp_batch_tuning best = b->tune;
@@ -1637,7 +1655,7 @@ static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t p
         }
         return KP_OK;
     };
-    if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25}))) return rc;
+    if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 19, 1u << 20, 1u << 21, 1u << 22}))) return rc;
     if ((rc = try_values(&kp_batch_tuning::trace_blocks_per_cu, {tb_max, std::max(1u, tb_max / 2), std::max(1u, tb_max - 1)})))
         return rc;
     if ((rc = try_values(&kp_batch_tuning::refill_below, {16u, 32u, 48u, 56u}))) return rc;
@@ -1682,6 +1700,23 @@ int kp_instance_set_material(kp_ctx* c, uint32_t instance_id, uint32_t material_
     if (instance_id >= c->inst_material.size()) return kp::fail(c, KP_ERR_INVALID, "bad instance id");
     c->inst_material[instance_id] = material_id;
     kp::render_scene_dirty(c);
+    return KP_OK;
+}
+
+int kp_release_buffers(kp_ctx* c) {
+    if (!c) return KP_ERR_INVALID;
+    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->render && c->render->path_mem) {
+        KP_HIP_TRY(c, hipFree(c->render->path_mem));
+        c->render->path_mem = nullptr;
+        c->render->cap_paths = 0;
+        c->render->ps = kp::PathState{};
+    }
+    if (c->d_stage_in) KP_HIP_TRY(c, hipFree(c->d_stage_in));
+    if (c->d_stage_out) KP_HIP_TRY(c, hipFree(c->d_stage_out));
+    c->d_stage_in = c->d_stage_out = nullptr;
+    c->stage_in_bytes = c->stage_out_bytes = 0;
     return KP_OK;
 }
 
@@ -1773,7 +1808,7 @@ int kp_batch_destroy(kp_batch* h) {
 int kp_batch_set_tuning(kp_batch* h, const kp_batch_tuning* t) {
     if (!h || !t) return KP_ERR_INVALID;
     kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
-    if (t->pass_paths < 256 || t->pass_paths > (1u << 26) || t->shade_blocks_per_cu == 0 ||
+    if (t->pass_paths < 256 || t->pass_paths > (1u << 22) || t->shade_blocks_per_cu == 0 ||
         t->shade_blocks_per_cu > 64 || t->trace_blocks_per_cu == 0 || t->refill_below == 0 || t->refill_below > 64)
         return kp::fail(b->c, KP_ERR_INVALID, "kp_batch_set_tuning: value out of range");
     b->tune = *t;

@@ -147,6 +147,7 @@ SIGNATURES = {
     "kp_bvh8_blob_free": (None, [_p]),
     "kp_set_trace_tuning": (_i32, [_p, _u32]),
     "kp_set_pass_paths": (_i32, [_p, _u32]),
+    "kp_release_buffers": (_i32, [_p]),
     "kp_material_add": (_i32, [_p, _p, ctypes.POINTER(_u32)]),
     "kp_mesh_set_material": (_i32, [_p, _u32, _u32]),
     "kp_film_reset": (_i32, [_p, _u32, _u32]),
@@ -381,6 +382,10 @@ class Context:
     def set_pass_paths(self, max_paths):
         """Paths in flight per integrator pass (default 2^26)."""
         self._check(lib().kp_set_pass_paths(self._h, max_paths))
+
+    def release_buffers(self):
+        """Frees cached path state and staging buffers (reallocated on demand)."""
+        self._check(lib().kp_release_buffers(self._h))
 
     def intersect_closest(self, rays, out=None, counted=False):
         """rays: [n,8] float32 numpy array or DeviceBuffer.  Returns [n,4]
