@@ -178,3 +178,24 @@ def test_render_state_errors():
     with pytest.raises(native.KpError):
         c.render(gdesc(s))          # no materials
     c.close()
+
+
+def test_pass_size_and_buffer_release_do_not_change_the_film():
+    """Paths are independent and every pixel folds its samples in sample order,
+    so the film is bit-identical whatever the pass size (here 2^16 paths, i.e.
+    several passes, against one pass of all 81,920 paths), also after the
+    cached path state has been released and reallocated."""
+    s = scenes.cornell_box(width=64, height=40, spp=32)
+    c = gpu_ctx(s)
+    one = c.render(gdesc(s))
+    c.release_buffers()
+    c.set_pass_paths(1 << 16)
+    many = c.render(gdesc(s))
+    assert c.render_stats()["passes"] > 1
+    c.release_buffers()
+    again = c.render(gdesc(s))
+    with pytest.raises(native.KpError):
+        c.set_pass_paths(100)
+    c.close()
+    assert np.array_equal(one.view(np.uint32), many.view(np.uint32))
+    assert np.array_equal(one.view(np.uint32), again.view(np.uint32))
