@@ -14,6 +14,7 @@ SAH BVH traversal on host cores) timed on the same ray set.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -240,6 +241,25 @@ def batch_leg(args):
     return out
 
 
+def batch_leg_child(args):
+    """The batched-scenes leg in a child process with a time limit, so that a
+    stall in it (two full runs once stopped making progress in batch set-up
+    with larger batch passes) cannot cost the whole benchmark line: on a
+    timeout or failure the leg is reported with its error and no numbers."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--batch-only", "--batch-scenes", str(args.batch_scenes)]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=args.batch_timeout, check=False)
+    except subprocess.TimeoutExpired:
+        print("[bench] batch leg exceeded %.0f s and was stopped" % args.batch_timeout, file=sys.stderr, flush=True)
+        return {"error": "timed out after %.0f s" % args.batch_timeout}
+    if r.returncode != 0:
+        return {"error": "exit status %d" % r.returncode}
+    try:
+        return json.loads(r.stdout.decode().strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": "no result line"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +279,10 @@ def main():
     ap.add_argument("--pt-res", type=int, default=1024)
     ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched-scenes leg and print its JSON (the main run starts this)")
+    ap.add_argument("--batch-timeout", type=float, default=150.0,
+                    help="seconds the batched-scenes leg may take before it is reported as failed")
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain equal-error leg")
     ap.add_argument("--guiding-res", type=int, default=512)
@@ -278,6 +302,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
+    if args.batch_only:
+        print(json.dumps(batch_leg(args)), flush=True)
+        return 0
 
     from kazen_proto_amd import native, scenes
 
@@ -380,7 +407,7 @@ def main():
         path_trace = path_trace_leg(args)
     batch = None
     if not args.no_batch:
-        batch = batch_leg(args)
+        batch = batch_leg_child(args)
     guiding = None
     if not args.no_guiding:
         guiding = guiding_equal_error(args)
