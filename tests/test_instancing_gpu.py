@@ -109,9 +109,15 @@ def test_closest_and_occluded_match_oracle(with_world, leaf):
     n_world = 0 if world is None else world.shape[0]
     assert st["n_tris"] == n_world + sum(protos[p].shape[0] for p, _ in placements)
     rays = random_rays(rng, 20000, aim=6.0)
+    # degenerate rays miss: NaN origin, zero direction, empty interval, infinite direction
+    rays[0, 0] = np.nan
+    rays[1, 4:7] = 0.0
+    rays[2, 3], rays[2, 7] = 5.0, 1.0
+    rays[3, 4] = np.inf
     ref = oracle_hits(ctx, world, protos, placements, rays)
     got = ctx.intersect_closest(rays)
     assert mismatches(got, ref) == 0
+    assert np.all(got[:4, 1].view(np.uint32) == MISS)
     hit = ref[:, 1].view(np.uint32) != MISS
     assert hit.mean() > 0.2
     occ = ctx.occluded(rays)
