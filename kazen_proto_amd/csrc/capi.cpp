@@ -344,6 +344,10 @@ int kp_create(kp_ctx** out, int device) {
     if ((e = kp::trace_occupancy(&bc, &bo)) != hipSuccess) return bail(e, "occupancy");
     c->grid_closest = (uint32_t)(c->cus * (bc > 0 ? bc : 1));
     c->grid_occluded = (uint32_t)(c->cus * (bo > 0 ? bo : 1));
+    int sp = 0, sg = 0;
+    if ((e = kp::shade_occupancy(&sp, &sg)) != hipSuccess) return bail(e, "occupancy");
+    c->shade_blocks = (uint32_t)(c->cus * (sp > 0 ? sp : 1));
+    c->shade_blocks_guided = (uint32_t)(c->cus * (sg > 0 ? sg : 1));
     *out = c;
     return KP_OK;
 }

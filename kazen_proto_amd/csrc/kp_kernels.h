@@ -39,6 +39,7 @@ struct TraceArgs {
 hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStream_t s);
 hipError_t launch_occluded(const TraceArgs& a, uint32_t grid, hipStream_t s);
 hipError_t trace_occupancy(int* blocks_per_cu_closest, int* blocks_per_cu_occluded);
+hipError_t shade_occupancy(int* blocks_per_cu_plain, int* blocks_per_cu_guided);
 hipError_t launch_reset_u32(uint32_t* p, uint32_t n, hipStream_t s);
 hipError_t launch_copy16(void* dst, const void* src, uint64_t n16, int cus, hipStream_t s);
 

@@ -10,7 +10,9 @@
 //   extend  : closest hit of the compacted ray queue (trace.hip)
 //   shade   : emission with MIS, next-event estimation (writes a shadow ray),
 //             Russian roulette, BSDF sampling (writes the next ray); both
-//             queues are compacted with one ballot + one atomic per wave
+//             queues are compacted with a ballot per wave and one atomic per
+//             block and queue (same-address device atomics are the limit:
+//             one per wave made shading 3.4x slower)
 //   connect : occlusion of the shadow queue, then the unoccluded
 //             contributions are added to their paths
 // After the last bounce every pixel folds its k path radiances into the film
@@ -59,13 +61,16 @@ struct PathState {
     float4* srays;        // shadow queue rays
     float4* scontrib;     // shadow contribution.xyz, slot bits in w
     uint32_t* socc;       // occlusion result per shadow ray
-    uint32_t* cnt;        // [0],[1] ray queue sizes, [2] shadow queue size
+    uint32_t* cnt;        // queue sizes qc(0), qc(1) rays, qc(2) shadow rays
     unsigned long long* totals;   // [0] extension rays, [1] shadow rays
     // batched scenes only
     uint32_t* slot_scene; // per slot: scene of the path
     uint32_t* slot_pixel; // per slot: pixel of the path in its scene
     uint32_t* qroot[2];   // per ray-queue entry: BVH root of the ray's scene
     uint32_t* sroot;      // per shadow-queue entry: BVH root
+    // Each queue counter has a 256-byte line of its own: the two counters
+    // one shade appends to are then served by different memory channels.
+    __host__ __device__ uint32_t& qc(uint32_t i) const { return cnt[i * 64u]; }
 };
 
 // One scene of a batch (the camera, film and sampling settings of its
@@ -165,9 +170,9 @@ __global__ __launch_bounds__(256) void k_generate(RenderArgs a) {
 #pragma clang fp contract(off)
     const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
     if (slot == 0) {
-        a.ps.cnt[0] = a.n_paths;
-        a.ps.cnt[1] = 0;
-        a.ps.cnt[2] = 0;
+        a.ps.qc(0) = a.n_paths;
+        a.ps.qc(1) = 0;
+        a.ps.qc(2) = 0;
     }
     if (slot >= a.n_paths) return;
     const uint32_t pixel = slot / a.k;
@@ -196,9 +201,9 @@ __global__ __launch_bounds__(256) void k_generate(RenderArgs a) {
 __global__ __launch_bounds__(256) void k_generate_rays(RenderArgs a, const float4* rays) {
     const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
     if (slot == 0) {
-        a.ps.cnt[0] = a.n_paths;
-        a.ps.cnt[1] = 0;
-        a.ps.cnt[2] = 0;
+        a.ps.qc(0) = a.n_paths;
+        a.ps.qc(1) = 0;
+        a.ps.qc(2) = 0;
     }
     if (slot >= a.n_paths) return;
     const uint64_t initstate = ((uint64_t)a.s0 << 32) ^ (uint64_t)a.seed ^ 0x9E3779B97F4A7C15ull;
@@ -226,9 +231,9 @@ __global__ __launch_bounds__(256) void k_generate_batch(RenderArgs a) {
     const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = batch_pass_paths(a);
     if (slot == 0) {
-        a.ps.cnt[0] = n;
-        a.ps.cnt[1] = 0;
-        a.ps.cnt[2] = 0;
+        a.ps.qc(0) = n;
+        a.ps.qc(1) = 0;
+        a.ps.qc(2) = 0;
     }
     if (slot >= n) return;
     const unsigned long long w = a.pass[0] + slot;
@@ -342,26 +347,45 @@ __global__ void k_write_radiance(RenderArgs a, float* out) {
 // Resets the output queue counters of the coming shade, accumulating totals.
 __global__ void k_bookkeep(RenderArgs a, int cur) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        a.ps.totals[0] += a.ps.cnt[cur];
-        a.ps.cnt[cur ^ 1] = 0;
-        a.ps.cnt[2] = 0;
+        a.ps.totals[0] += a.ps.qc(cur);
+        a.ps.qc(cur ^ 1) = 0;
+        a.ps.qc(2) = 0;
     }
 }
 
 __global__ void k_count_shadow(RenderArgs a) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) a.ps.totals[1] += a.ps.cnt[2];
+    if (threadIdx.x == 0 && blockIdx.x == 0) a.ps.totals[1] += a.ps.qc(2);
 }
 
-// Appends lanes with `want` to a queue; returns the lane's position.
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
-    const uint64_t m = __ballot(want);
-    if (!m) return 0u;
+// Appends the lanes with `want0` to queue c0 and those with `want1` to queue
+// c1: one ballot per wave and queue, one atomic per block and queue; returns
+// the lane's position in c0 (and in c1 through o1).  Every thread of the
+// block must call it (it synchronises the block); lds holds 10 words.
+__device__ __forceinline__ uint32_t block_append2(bool want0, bool want1, uint32_t* c0, uint32_t* c1,
+                                                  uint32_t* lds, uint32_t& o1) {
+    const uint64_t m0 = __ballot(want0), m1 = __ballot(want1);
     const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const int w = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[w] = (uint32_t)__popcll(m0);
+        lds[4 + w] = (uint32_t)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t0 = lds[0] + lds[1] + lds[2] + lds[3], t1 = lds[4] + lds[5] + lds[6] + lds[7];
+        lds[8] = t0 ? atomicAdd(c0, t0) : 0u;
+        lds[9] = t1 ? atomicAdd(c1, t1) : 0u;
+    }
+    __syncthreads();
+    uint32_t b0 = lds[8], b1 = lds[9];
+    for (int i = 0; i < w; ++i) {
+        b0 += lds[i];
+        b1 += lds[4 + i];
+    }
+    __syncthreads();   // lds is reused by the next call
+    const uint64_t below = (1ull << lane) - 1ull;
+    o1 = b1 + (uint32_t)__popcll(m1 & below);
+    return b0 + (uint32_t)__popcll(m0 & below);
 }
 
 // Shades the current queue.  kGuided adds the guiding network: 12 draws per
@@ -371,7 +395,8 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
 template <bool kGuided>
 __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t depth) {
     __shared__ uint32_t glds[kGuided ? 4 * 96 * KP_G_ROW : 1];
-    const uint32_t n = a.ps.cnt[cur];
+    __shared__ uint32_t app_lds[10];
+    const uint32_t n = a.ps.qc(cur);
     const uint32_t total = gridDim.x * 256;
     const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
     const uint32_t iters = (n + total - 1) / total;
@@ -611,14 +636,14 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
             a.ps.s1[slot] = make_float4(L.x, L.y, L.z, 0.0f);
             a.ps.rng[slot] = rng.state;
         }
-        const uint32_t qo = wave_append(cont, &a.ps.cnt[cur ^ 1]);
+        uint32_t so;
+        const uint32_t qo = block_append2(cont, shad, &a.ps.qc(cur ^ 1), &a.ps.qc(2), app_lds, so);
         if (cont) {
             rout[2 * (size_t)qo] = nr0;
             rout[2 * (size_t)qo + 1] = nr1;
             qout[qo] = slot;
             if (a.scenes) a.ps.qroot[cur ^ 1][qo] = root;
         }
-        const uint32_t so = wave_append(shad, &a.ps.cnt[2]);
         if (shad) {
             a.ps.srays[2 * (size_t)so] = sr0;
             a.ps.srays[2 * (size_t)so + 1] = sr1;
@@ -629,7 +654,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
 }
 
 __global__ __launch_bounds__(256) void k_connect(RenderArgs a) {
-    const uint32_t n = a.ps.cnt[2];
+    const uint32_t n = a.ps.qc(2);
     const uint32_t total = gridDim.x * 256;
     for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < n; s += total) {
         if (a.ps.socc[s]) continue;
@@ -716,6 +741,12 @@ __global__ void k_film_resolve(const float4* sum, const float4* comp, uint32_t n
 __global__ void k_zero_f4(float4* p, uint32_t n) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+hipError_t shade_occupancy(int* blocks_per_cu_plain, int* blocks_per_cu_guided) {
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu_plain, k_shade<false>, 256, 0);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu_guided, k_shade<true>, 256, 0);
 }
 
 }  // namespace kp
@@ -910,7 +941,7 @@ static int alloc_paths(kp_ctx* c, PathState& ps, void*& mem, uint32_t& cap, uint
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
     const size_t o_r0 = take(N * 32), o_r1 = take(N * 32), o_q0 = take(N * 4), o_q1 = take(N * 4);
     const size_t o_h = take(N * 16), o_s0 = take(N * 16), o_s1 = take(N * 16), o_rng = take(N * 8);
-    const size_t o_sr = take(N * 32), o_sc = take(N * 16), o_so = take(N * 4), o_cnt = take(64),
+    const size_t o_sr = take(N * 32), o_sc = take(N * 16), o_so = take(N * 4), o_cnt = take(3 * 256),
                  o_tot = take(64);
     const size_t o_ss = take(N * 4), o_sp = take(N * 4), o_qr0 = take(N * 4), o_qr1 = take(N * 4),
                  o_sr2 = take(N * 4);
@@ -934,7 +965,7 @@ static int alloc_paths(kp_ctx* c, PathState& ps, void*& mem, uint32_t& cap, uint
     ps.qroot[0] = (uint32_t*)(b + o_qr0);
     ps.qroot[1] = (uint32_t*)(b + o_qr1);
     ps.sroot = (uint32_t*)(b + o_sr2);
-    KP_HIP_TRY(c, hipMemset(ps.cnt, 0, 64));
+    KP_HIP_TRY(c, hipMemset(ps.cnt, 0, 3 * 256));
     KP_HIP_TRY(c, hipMemset(ps.totals, 0, 64));
     cap = n;
     return KP_OK;
@@ -1002,7 +1033,7 @@ static uint32_t samples_per_pass(uint32_t npix, uint32_t spp, uint64_t target) {
 // Traversal and launch-shape settings of the integrator's launches.
 struct TraceCfg {
     TraceArgs base;            // BVH, work counter, error word, spill, tuning
-    uint32_t grid_closest, grid_occluded, shade_blocks;
+    uint32_t grid_closest, grid_occluded, shade_blocks, shade_blocks_guided;
     bool roots;                // per-ray roots from the path state (batched scenes)
 };
 
@@ -1012,7 +1043,10 @@ static TraceCfg ctx_trace_cfg(kp_ctx* c) {
     t.base = trace_base(c);
     t.grid_closest = c->grid_closest;
     t.grid_occluded = c->grid_occluded;
-    t.shade_blocks = (uint32_t)c->cus * 8u;
+    // as many blocks as stay resident: every wave strides over the queue in
+    // one round, no second partial round of blocks
+    t.shade_blocks = c->shade_blocks;
+    t.shade_blocks_guided = c->shade_blocks_guided;
     t.roots = false;
     return t;
 }
@@ -1027,12 +1061,12 @@ static int run_bounces(kp_ctx* c, const RenderArgs& a, bool guided, const TraceC
         ta.rays = a.ps.rays[cur];
         ta.hits = a.ps.hits;
         ta.n = a.n_paths;
-        ta.n_dev = &a.ps.cnt[cur];
+        ta.n_dev = &a.ps.qc(cur);
         ta.roots = tc.roots ? a.ps.qroot[cur] : nullptr;
         KP_HIP_TRY(c, launch_closest(ta, tc.grid_closest, false, c->stream));
         hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, c->stream, a, cur);
         if (guided)
-            hipLaunchKernelGGL(k_shade<true>, dim3(tc.shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
+            hipLaunchKernelGGL(k_shade<true>, dim3(tc.shade_blocks_guided), dim3(256), 0, c->stream, a, cur, depth);
         else
             hipLaunchKernelGGL(k_shade<false>, dim3(tc.shade_blocks), dim3(256), 0, c->stream, a, cur, depth);
         KP_HIP_TRY(c, hipGetLastError());
@@ -1040,7 +1074,7 @@ static int run_bounces(kp_ctx* c, const RenderArgs& a, bool guided, const TraceC
             ta.rays = a.ps.srays;
             ta.hits = nullptr;
             ta.occl = a.ps.socc;
-            ta.n_dev = &a.ps.cnt[2];
+            ta.n_dev = &a.ps.qc(2);
             ta.roots = tc.roots ? a.ps.sroot : nullptr;
             KP_HIP_TRY(c, launch_occluded(ta, tc.grid_occluded, c->stream));
             hipLaunchKernelGGL(k_count_shadow, dim3(1), dim3(64), 0, c->stream, a);
@@ -1567,6 +1601,7 @@ static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max
     tc.grid_closest = std::min((uint32_t)c->cus * tb, c->grid_closest);
     tc.grid_occluded = std::min((uint32_t)c->cus * tb, c->grid_occluded);
     tc.shade_blocks = (uint32_t)c->cus * std::max(b->tune.shade_blocks_per_cu, 1u);
+    tc.shade_blocks_guided = c->shade_blocks_guided;
     tc.roots = true;
     uint64_t passes = (work + chunk - 1) / chunk;
     if (max_passes && passes > max_passes) passes = max_passes;
