@@ -64,6 +64,7 @@ __device__ __forceinline__ void g_write_features(uint32_t* row, const float* in6
 __global__ __launch_bounds__(256) void k_guide_eval(GuideNet net, const float* __restrict__ in6, uint32_t n,
                                                     float* out32, const float* dirs, const float* u3,
                                                     float* dirs_out, float* pdf_out, int mode) {
+    // mode 0: 64 feature rows + two 16-row activation buffers per wave
     __shared__ uint32_t lds[4][96 * KP_G_ROW];
     const int wave = threadIdx.x >> 6, lane = __lane_id();
     uint32_t* feat = lds[wave];

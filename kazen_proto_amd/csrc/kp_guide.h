@@ -52,15 +52,13 @@ struct GuideNet {
     float3 bmin, binv;     // scene box: position -> [0,1]^3
 };
 
-__device__ __forceinline__ uint32_t f2bf_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return u >> 16;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
-}
+// FP32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pack_bf2(float lo, float hi) {
-    return f2bf_bits(lo) | (f2bf_bits(hi) << 16);
+    const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
 }
+__device__ __forceinline__ uint32_t f2bf_bits(float f) { return pack_bf2(f, 0.0f) & 0xffffu; }
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
@@ -165,6 +163,69 @@ __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin,
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         src = dst;
+    }
+}
+
+// Rows of LDS (KP_G_ROW words each) one wave needs for g_wave_guide.
+#define KP_G_WAVE_ROWS 64
+
+// Forward pass of a wave's 64 samples in place, layer-major: each layer's
+// weight fragments and biases are read from memory once and applied to the
+// four 16-sample chunks (the per-chunk order reads them four times and waits
+// on the memory system 20 times instead of 5).  x holds the 64 bf16 input
+// rows (KP_G_ROW words each); a chunk's rows are read into registers before
+// its outputs overwrite them.  On return row s holds sample s's 32 outputs
+// as f32 (words 0..31).  Every lane of the wave must call it.
+__device__ inline void g_wave_forward(const GuideNet& net, uint32_t* x) {
+    const int lane = __lane_id();
+    const int g = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int l = 0; l < KP_G_LAYERS; ++l) {
+        const int ntiles = g_dim_out(l) / 16;
+        const float* bias = net.params + g_layer_offset(l) + g_dim_in(l) * g_dim_out(l);
+        // launder the fragment pointer per layer: loads of later layers are
+        // not hoisted here (they would pin 32 registers per lane and layer)
+        const u32x4* wf = net.wf;
+        asm volatile("" : "+s"(wf));
+        u32x4 w[8];
+#pragma unroll
+        for (int i = 0; i < 2 * ntiles; ++i) w[i] = wf[(g_wf_base(l) + i) * 64 + lane];
+        float bv[16];
+#pragma unroll
+        for (int t = 0; t < ntiles; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[4 * t + j] = bias[16 * t + 4 * g + j];
+#pragma unroll 1
+        for (int c = 0; c < 4; ++c) {
+            const int row = 16 * c + r;
+            const u32x4 b0 = *(const u32x4*)(x + row * KP_G_ROW + 4 * g);
+            const u32x4 b1 = *(const u32x4*)(x + row * KP_G_ROW + 16 + 4 * g);
+#pragma unroll
+            for (int t = 0; t < ntiles; ++t) {
+                // the bias is the accumulator's initial value
+                f32x4 acc = {bv[4 * t], bv[4 * t + 1], bv[4 * t + 2], bv[4 * t + 3]};
+                acc = mfma16(w[2 * t], b0, acc);
+                acc = mfma16(w[2 * t + 1], b1, acc);
+                const int n0 = 16 * t + 4 * g;
+                float v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = acc[j];
+                if (l < KP_G_LAYERS - 1) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = fmaxf(v[j], 0.0f);
+                    x[row * KP_G_ROW + n0 / 2] = pack_bf2(v[0], v[1]);
+                    x[row * KP_G_ROW + n0 / 2 + 1] = pack_bf2(v[2], v[3]);
+                } else {
+                    float* o = reinterpret_cast<float*>(x) + row * KP_G_ROW + n0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) o[j] = v[j];
+                }
+            }
+        }
+        // the next layer reads rows written by other lanes of the wave
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -306,16 +367,44 @@ struct GuideAnswer {
     float dx, dy, dz;               // the sampled direction
 };
 
-// Density contribution of lobe (raw outputs o4) for the unit direction w.
-__device__ __forceinline__ float g_lobe_term(const float* o4, float pi, float wx, float wy, float wz) {
-    const float k = fminf(g_softplus(o4[1]) + 0.01f, 1.0e4f);
-    const float norm = k / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * k)));
-    const float z = tanhf(o4[2]);
+// Hardware transcendentals for the guided shade's mixture queries
+// (v_exp_f32, v_log_f32, v_rcp_f32, v_rsq_f32, v_sin_f32 / v_cos_f32 on
+// revolutions): about 1e-6 relative, far inside the 2e-3 at which mixture
+// densities are compared with the oracle.  Training keeps the libm forms.
+__device__ __forceinline__ float g_fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float g_flog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+__device__ __forceinline__ float g_frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float g_ftanh(float x) {
+    const float t = g_fexp(-2.0f * fabsf(x));
+    return copysignf((1.0f - t) * g_frcp(1.0f + t), x);
+}
+
+// One lobe (raw outputs o4, mixture weight pi) set up for density queries:
+// concentration, pi x vMF normalisation, mean direction.
+struct LobeQ {
+    float k, pn, mx, my, mz;
+};
+__device__ __forceinline__ LobeQ g_lobe_q(const float* o4, float pi) {
+    LobeQ L;
+    const float sp_in = o4[1];
+    const float soft = sp_in > 20.0f ? sp_in : g_flog(1.0f + g_fexp(sp_in));
+    const float k = fminf(soft + 0.01f, 1.0e4f);
+    const float norm = k * g_frcp(2.0f * 3.14159265358979f * (1.0f - g_fexp(-2.0f * k)));
+    const float z = g_ftanh(o4[2]);
     const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    float sp, cp;
-    sincosf(3.14159265358979f * tanhf(o4[3]), &sp, &cp);
-    const float c = rr * cp * wx + z * wy + rr * sp * wz;
-    return pi * norm * expf(k * (c - 1.0f));
+    // phi = pi tanh(o4[3]): half a revolution per unit
+    const float rev = 0.5f * g_ftanh(o4[3]);
+    L.k = k;
+    L.pn = pi * norm;
+    L.mx = rr * __builtin_amdgcn_cosf(rev);
+    L.my = z;
+    L.mz = rr * __builtin_amdgcn_sinf(rev);
+    return L;
+}
+// Density contribution of the lobe for the unit direction w.
+__device__ __forceinline__ float g_lobe_eval(const LobeQ& L, float wx, float wy, float wz) {
+    const float c = L.mx * wx + L.my * wy + L.mz * wz;
+    return L.pn * g_fexp(L.k * (c - 1.0f));
 }
 
 __device__ __forceinline__ float g_sum4(float v) {
@@ -324,28 +413,34 @@ __device__ __forceinline__ float g_sum4(float v) {
     return v;
 }
 
-// Answers every lane's query.  Each 16-lane group's samples go through the
-// network together (4 chunks); after the last layer lane (g, r) of chunk q
-// holds lobes g and g+4 of the query of lane 16q + r, and the mixture sums
+// Answers every lane's query.  The wave's 64 samples go through the network
+// together (g_wave_forward); then, for each 16-sample chunk c, lane (g, r)
+// takes lobes g and g+4 of the query of lane 16c + r, and the mixture sums
 // are reduced over the four lanes of a sample with xor-shuffles.  `lds` is
-// this wave's 96 rows (KP_G_ROW words each).  Every lane must call it.
+// this wave's KP_G_WAVE_ROWS rows (KP_G_ROW words each).  Every lane must
+// call it.
 __device__ inline void g_wave_guide(const GuideNet& net, uint32_t* lds, const GuideQuery& q, GuideAnswer& ans) {
     const int lane = __lane_id();
     const int g = lane >> 4, r = lane & 15;
     uint32_t* feat = lds;
-    uint32_t* ping = lds + 64 * KP_G_ROW;
-    uint32_t* pong = ping + 16 * KP_G_ROW;
     g_encode_row(feat + lane * KP_G_ROW, q.px, q.py, q.pz, q.nx, q.ny, q.nz);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    uint32_t* act[4] = {ping, pong, ping, pong};
+    g_wave_forward(net, feat);
+    const float* outs = reinterpret_cast<const float*>(feat);
     ans.pdf_a = ans.pdf_b = 0.0f;
     ans.dx = ans.dy = 0.0f;
     ans.dz = 1.0f;
     for (int c = 0; c < 4; ++c) {
+        // outputs 16t + 4g + j of the sample: lobe g (t = 0) and g + 4 (t = 1)
         float o[8];
-        g_forward_chunk(net, feat + 16 * c * KP_G_ROW, act, nullptr, 0, o);
+        const float* os = outs + (16 * c + r) * KP_G_ROW + 4 * g;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            o[j] = os[j];
+            o[4 + j] = os[16 + j];
+        }
         // the query of sample r of this chunk lives in lane 16c + r
         const int owner = 16 * c + r;
         const float ax = __shfl(q.ax, owner), ay = __shfl(q.ay, owner), az = __shfl(q.az, owner);
@@ -361,7 +456,8 @@ __device__ inline void g_wave_guide(const GuideNet& net, uint32_t* lds, const Gu
         sum += __shfl_xor(sum, 16);
         sum += __shfl_xor(sum, 32);
         const float pi0 = (float)(e0 / sum), pi1 = (float)(e1 / sum);
-        const float pa = g_sum4(g_lobe_term(o, pi0, ax, ay, az) + g_lobe_term(o + 4, pi1, ax, ay, az));
+        const LobeQ l0 = g_lobe_q(o, pi0), l1 = g_lobe_q(o + 4, pi1);
+        const float pa = g_sum4(g_lobe_eval(l0, ax, ay, az) + g_lobe_eval(l1, ax, ay, az));
         float sx = bx, sy = by, sz = bz;
         if (smp) {
             // lobe i lives in lane 16*(i%4) + r, slot i/4: gather all weights
@@ -380,38 +476,34 @@ __device__ inline void g_wave_guide(const GuideNet& net, uint32_t* lds, const Gu
             }
             const int src = 16 * (sel & 3) + r;
             const bool hi = sel >= 4;
-            float ob[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) ob[j] = __shfl(hi ? o[4 + j] : o[j], src);
-            const float k = fminf(g_softplus(ob[1]) + 0.01f, 1.0e4f);
-            const float z = tanhf(ob[2]);
-            const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-            float sp, cp;
-            sincosf(3.14159265358979f * tanhf(ob[3]), &sp, &cp);
-            const float mxx = rr * cp, myy = z, mzz = rr * sp;
-            const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
+            // the selected lobe's set-up, from the lane that holds it (the
+            // four lanes of a sample agree on sel, so on hi)
+            const float k = __shfl(hi ? l1.k : l0.k, src);
+            const float mxx = __shfl(hi ? l1.mx : l0.mx, src), myy = __shfl(hi ? l1.my : l0.my, src),
+                        mzz = __shfl(hi ? l1.mz : l0.mz, src);
+            // cosine to the mean (W. Jakob 2012), clamped to -1 where exp(-2k)
+            // underflows and u1 = 0
+            const float w = fmaxf(1.0f + g_flog(u1 + (1.0f - u1) * g_fexp(-2.0f * k)) * g_frcp(k), -1.0f);
             const float srr = sqrtf(fmaxf(0.0f, 1.0f - w * w));
-            float s2, c2;
-            sincosf(2.0f * 3.14159265358979f * u2, &s2, &c2);
-            const float lx = srr * c2, ly = srr * s2;
+            const float lx = srr * __builtin_amdgcn_cosf(u2), ly = srr * __builtin_amdgcn_sinf(u2);
             float tx, ty, tz;
             if (fabsf(mxx) > fabsf(myy)) {
-                const float inv = 1.0f / sqrtf(mxx * mxx + mzz * mzz);
+                const float inv = __builtin_amdgcn_rsqf(mxx * mxx + mzz * mzz);
                 tx = mzz * inv; ty = 0.0f; tz = -mxx * inv;
             } else {
-                const float inv = 1.0f / sqrtf(myy * myy + mzz * mzz);
+                const float inv = __builtin_amdgcn_rsqf(myy * myy + mzz * mzz);
                 tx = 0.0f; ty = mzz * inv; tz = -myy * inv;
             }
             const float bxx = ty * mzz - tz * myy, byy = tz * mxx - tx * mzz, bzz = tx * myy - ty * mxx;
             const float dx = lx * tx + ly * bxx + w * mxx;
             const float dy = lx * ty + ly * byy + w * myy;
             const float dz = lx * tz + ly * bzz + w * mzz;
-            const float len = sqrtf(dx * dx + dy * dy + dz * dz);
-            sx = dx / len;
-            sy = dy / len;
-            sz = dz / len;
+            const float il = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+            sx = dx * il;
+            sy = dy * il;
+            sz = dz * il;
         }
-        const float pb = g_sum4(g_lobe_term(o, pi0, sx, sy, sz) + g_lobe_term(o + 4, pi1, sx, sy, sz));
+        const float pb = g_sum4(g_lobe_eval(l0, sx, sy, sz) + g_lobe_eval(l1, sx, sy, sz));
         if (g == c) {   // this lane owns sample r of chunk c
             ans.pdf_a = pa;
             ans.pdf_b = pb;

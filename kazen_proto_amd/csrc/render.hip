@@ -394,7 +394,7 @@ __device__ __forceinline__ uint32_t block_append2(bool want0, bool want1, uint32
 // network query runs once per wave after every lane has prepared its vertex.
 template <bool kGuided>
 __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t depth) {
-    __shared__ uint32_t glds[kGuided ? 4 * 96 * KP_G_ROW : 1];
+    __shared__ uint32_t glds[kGuided ? 4 * KP_G_WAVE_ROWS * KP_G_ROW : 1];
     __shared__ uint32_t app_lds[10];
     const uint32_t n = a.ps.qc(cur);
     const uint32_t total = gridDim.x * 256;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                 gq.u0 = gq.sample ? u[9] : 0.5f;
                 gq.u1 = gq.sample ? u[10] : 0.5f;
                 gq.u2 = gq.sample ? u[11] : 0.5f;
-                g_wave_guide(a.net, glds + (threadIdx.x >> 6) * 96 * KP_G_ROW, gq, ga);
+                g_wave_guide(a.net, glds + (threadIdx.x >> 6) * KP_G_WAVE_ROWS * KP_G_ROW, gq, ga);
             }
         }
         // ---- per-lane finish ---------------------------------------------

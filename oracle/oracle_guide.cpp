@@ -206,7 +206,8 @@ void or_vmf_sample(const double* out32, const float* u3, float* dir, double* pdf
     }
     const double k = m.kappa[lobe];
     const double u1 = u3[1], u2 = u3[2];
-    const double wz = 1.0 + std::log(u1 + (1.0 - u1) * std::exp(-2.0 * k)) / k;
+    // clamped to -1 where exp(-2k) underflows and u1 = 0 (log of 0)
+    const double wz = std::max(1.0 + std::log(u1 + (1.0 - u1) * std::exp(-2.0 * k)) / k, -1.0);
     const double sr = std::sqrt(std::max(0.0, 1.0 - wz * wz));
     const double lx = sr * std::cos(2.0 * kPiD * u2), ly = sr * std::sin(2.0 * kPiD * u2);
     // frame around mu (any orthonormal completion)
