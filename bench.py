@@ -290,7 +290,7 @@ def main():
     ap.add_argument("--guiding-target", type=float, default=0.01)
     ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
     ap.add_argument("--guiding-lr", type=float, default=2e-3)
-    ap.add_argument("--guiding-steps", type=int, default=16, help="Adam steps per pass")
+    ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
     ap.add_argument("--guiding-pass-paths", type=int, default=1 << 24,
                     help="paths per guided pass (each pass trains once)")
     ap.add_argument("--guiding-train-passes", type=int, default=64,

@@ -213,12 +213,10 @@ __device__ inline void g_wave_forward(const GuideNet& net, uint32_t* x) {
                 if (l < KP_G_LAYERS - 1) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) v[j] = fmaxf(v[j], 0.0f);
-                    x[row * KP_G_ROW + n0 / 2] = pack_bf2(v[0], v[1]);
-                    x[row * KP_G_ROW + n0 / 2 + 1] = pack_bf2(v[2], v[3]);
+                    *reinterpret_cast<uint2*>(x + row * KP_G_ROW + n0 / 2) =
+                        make_uint2(pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]));
                 } else {
-                    float* o = reinterpret_cast<float*>(x) + row * KP_G_ROW + n0;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) o[j] = v[j];
+                    *reinterpret_cast<float4*>(x + row * KP_G_ROW + n0) = make_float4(v[0], v[1], v[2], v[3]);
                 }
             }
         }
@@ -332,6 +330,7 @@ __device__ __forceinline__ void g_encode_row(uint32_t* row, float px, float py, 
                                              float nz) {
     const float pc[3] = {px, py, pz};
     const float nc[3] = {nx, ny, nz};
+    uint32_t wd[32];
     int w = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -339,7 +338,7 @@ __device__ __forceinline__ void g_encode_row(uint32_t* row, float px, float py, 
         for (int k = 0; k < 6; ++k) {
             float s, co;
             sincospif(pc[c] * (float)(1 << k), &s, &co);
-            row[w++] = pack_bf2(s, co);
+            wd[w++] = pack_bf2(s, co);
         }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -347,10 +346,14 @@ __device__ __forceinline__ void g_encode_row(uint32_t* row, float px, float py, 
         for (int k = 0; k < 2; ++k) {
             float s, co;
             sincospif(nc[c] * (float)(1 << k), &s, &co);
-            row[w++] = pack_bf2(s, co);
+            wd[w++] = pack_bf2(s, co);
         }
 #pragma unroll
-    for (; w < 32; ++w) row[w] = 0u;
+    for (; w < 32; ++w) wd[w] = 0u;
+    // 16-byte stores: a lane's row is 8 stores, not 32
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        reinterpret_cast<u32x4*>(row)[i] = u32x4{wd[4 * i], wd[4 * i + 1], wd[4 * i + 2], wd[4 * i + 3]};
 }
 
 // Per-lane query of the guiding distribution at one shading point.
@@ -391,7 +394,7 @@ __device__ __forceinline__ LobeQ g_lobe_q(const float* o4, float pi) {
     const float k = fminf(soft + 0.01f, 1.0e4f);
     const float norm = k * g_frcp(2.0f * 3.14159265358979f * (1.0f - g_fexp(-2.0f * k)));
     const float z = g_ftanh(o4[2]);
-    const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float rr = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - z * z));
     // phi = pi tanh(o4[3]): half a revolution per unit
     const float rev = 0.5f * g_ftanh(o4[3]);
     L.k = k;
@@ -407,111 +410,94 @@ __device__ __forceinline__ float g_lobe_eval(const LobeQ& L, float wx, float wy,
     return L.pn * g_fexp(L.k * (c - 1.0f));
 }
 
-__device__ __forceinline__ float g_sum4(float v) {
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
-}
-
 // Answers every lane's query.  The wave's 64 samples go through the network
-// together (g_wave_forward); then, for each 16-sample chunk c, lane (g, r)
-// takes lobes g and g+4 of the query of lane 16c + r, and the mixture sums
-// are reduced over the four lanes of a sample with xor-shuffles.  `lds` is
-// this wave's KP_G_WAVE_ROWS rows (KP_G_ROW words each).  Every lane must
-// call it.
+// together (g_wave_forward), which leaves sample s's 32 outputs in row s;
+// then each lane evaluates its own sample's mixture: mixture weights (FP32
+// exponentials, FP64 normalisation), the sampled direction if asked (lobe by
+// inverse CDF of the weights, then the vMF sample), and the densities at A
+// and at B or the sample, one lobe at a time.  `lds` is this wave's
+// KP_G_WAVE_ROWS rows (KP_G_ROW words each).  Every lane must call it.
 __device__ inline void g_wave_guide(const GuideNet& net, uint32_t* lds, const GuideQuery& q, GuideAnswer& ans) {
     const int lane = __lane_id();
-    const int g = lane >> 4, r = lane & 15;
-    uint32_t* feat = lds;
-    g_encode_row(feat + lane * KP_G_ROW, q.px, q.py, q.pz, q.nx, q.ny, q.nz);
+    uint32_t* row = lds + lane * KP_G_ROW;
+    g_encode_row(row, q.px, q.py, q.pz, q.nx, q.ny, q.nz);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    g_wave_forward(net, feat);
-    const float* outs = reinterpret_cast<const float*>(feat);
-    ans.pdf_a = ans.pdf_b = 0.0f;
-    ans.dx = ans.dy = 0.0f;
-    ans.dz = 1.0f;
-    for (int c = 0; c < 4; ++c) {
-        // outputs 16t + 4g + j of the sample: lobe g (t = 0) and g + 4 (t = 1)
-        float o[8];
-        const float* os = outs + (16 * c + r) * KP_G_ROW + 4 * g;
+    g_wave_forward(net, lds);
+    float o[KP_G_OUT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            o[j] = os[j];
-            o[4 + j] = os[16 + j];
-        }
-        // the query of sample r of this chunk lives in lane 16c + r
-        const int owner = 16 * c + r;
-        const float ax = __shfl(q.ax, owner), ay = __shfl(q.ay, owner), az = __shfl(q.az, owner);
-        const float bx = __shfl(q.bx, owner), by = __shfl(q.by, owner), bz = __shfl(q.bz, owner);
-        const int smp = __shfl((int)q.sample, owner);
-        const float u0 = __shfl(q.u0, owner), u1 = __shfl(q.u1, owner), u2 = __shfl(q.u2, owner);
-        // softmax over the 8 logits of the sample (sum in FP64)
-        float mx = fmaxf(o[0], o[4]);
-        mx = fmaxf(mx, __shfl_xor(mx, 16));
-        mx = fmaxf(mx, __shfl_xor(mx, 32));
-        const double e0 = exp((double)(o[0] - mx)), e1 = exp((double)(o[4] - mx));
-        double sum = e0 + e1;
-        sum += __shfl_xor(sum, 16);
-        sum += __shfl_xor(sum, 32);
-        const float pi0 = (float)(e0 / sum), pi1 = (float)(e1 / sum);
-        const LobeQ l0 = g_lobe_q(o, pi0), l1 = g_lobe_q(o + 4, pi1);
-        const float pa = g_sum4(g_lobe_eval(l0, ax, ay, az) + g_lobe_eval(l1, ax, ay, az));
-        float sx = bx, sy = by, sz = bz;
-        if (smp) {
-            // lobe i lives in lane 16*(i%4) + r, slot i/4: gather all weights
-            float pis[8];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                pis[i] = __shfl(pi0, 16 * i + r);
-                pis[4 + i] = __shfl(pi1, 16 * i + r);
-            }
-            int sel = KP_G_LOBES - 1;
-            float acc = 0.0f;
-#pragma unroll
-            for (int i = 0; i < KP_G_LOBES - 1; ++i) {
-                acc += pis[i];
-                if (sel == KP_G_LOBES - 1 && u0 < acc) sel = i;
-            }
-            const int src = 16 * (sel & 3) + r;
-            const bool hi = sel >= 4;
-            // the selected lobe's set-up, from the lane that holds it (the
-            // four lanes of a sample agree on sel, so on hi)
-            const float k = __shfl(hi ? l1.k : l0.k, src);
-            const float mxx = __shfl(hi ? l1.mx : l0.mx, src), myy = __shfl(hi ? l1.my : l0.my, src),
-                        mzz = __shfl(hi ? l1.mz : l0.mz, src);
-            // cosine to the mean (W. Jakob 2012), clamped to -1 where exp(-2k)
-            // underflows and u1 = 0
-            const float w = fmaxf(1.0f + g_flog(u1 + (1.0f - u1) * g_fexp(-2.0f * k)) * g_frcp(k), -1.0f);
-            const float srr = sqrtf(fmaxf(0.0f, 1.0f - w * w));
-            const float lx = srr * __builtin_amdgcn_cosf(u2), ly = srr * __builtin_amdgcn_sinf(u2);
-            float tx, ty, tz;
-            if (fabsf(mxx) > fabsf(myy)) {
-                const float inv = __builtin_amdgcn_rsqf(mxx * mxx + mzz * mzz);
-                tx = mzz * inv; ty = 0.0f; tz = -mxx * inv;
-            } else {
-                const float inv = __builtin_amdgcn_rsqf(myy * myy + mzz * mzz);
-                tx = 0.0f; ty = mzz * inv; tz = -myy * inv;
-            }
-            const float bxx = ty * mzz - tz * myy, byy = tz * mxx - tx * mzz, bzz = tx * myy - ty * mxx;
-            const float dx = lx * tx + ly * bxx + w * mxx;
-            const float dy = lx * ty + ly * byy + w * myy;
-            const float dz = lx * tz + ly * bzz + w * mzz;
-            const float il = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
-            sx = dx * il;
-            sy = dy * il;
-            sz = dz * il;
-        }
-        const float pb = g_sum4(g_lobe_eval(l0, sx, sy, sz) + g_lobe_eval(l1, sx, sy, sz));
-        if (g == c) {   // this lane owns sample r of chunk c
-            ans.pdf_a = pa;
-            ans.pdf_b = pb;
-            ans.dx = sx;
-            ans.dy = sy;
-            ans.dz = sz;
-        }
+    for (int i = 0; i < KP_G_OUT / 4; ++i) {
+        const float4 v = reinterpret_cast<const float4*>(row)[i];
+        o[4 * i] = v.x;
+        o[4 * i + 1] = v.y;
+        o[4 * i + 2] = v.z;
+        o[4 * i + 3] = v.w;
     }
+    float mx = o[0];
+#pragma unroll
+    for (int i = 1; i < KP_G_LOBES; ++i) mx = fmaxf(mx, o[4 * i]);
+    float e[KP_G_LOBES];
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < KP_G_LOBES; ++i) {
+        e[i] = g_fexp(o[4 * i] - mx);
+        sum += (double)e[i];
+    }
+    const double inv_sum = 1.0 / sum;
+    float pi[KP_G_LOBES];
+#pragma unroll
+    for (int i = 0; i < KP_G_LOBES; ++i) pi[i] = (float)((double)e[i] * inv_sum);
+    float sx = q.bx, sy = q.by, sz = q.bz;
+    if (q.sample) {
+        int sel = KP_G_LOBES - 1;
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = 0; i < KP_G_LOBES - 1; ++i) {
+            acc += pi[i];
+            if (sel == KP_G_LOBES - 1 && q.u0 < acc) sel = i;
+        }
+        // the selected lobe's raw outputs, from LDS (a dynamic index into o
+        // would put o in scratch memory)
+        const float4 ov = reinterpret_cast<const float4*>(row)[sel];
+        const float o4[4] = {ov.x, ov.y, ov.z, ov.w};
+        const LobeQ L = g_lobe_q(o4, 1.0f);
+        const float k = L.k, mxx = L.mx, myy = L.my, mzz = L.mz;
+        // cosine to the mean (W. Jakob 2012), clamped to -1 where exp(-2k)
+        // underflows and u1 = 0
+        const float u1 = q.u1, u2 = q.u2;
+        const float w = fmaxf(1.0f + g_flog(u1 + (1.0f - u1) * g_fexp(-2.0f * k)) * g_frcp(k), -1.0f);
+        const float srr = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - w * w));
+        const float lx = srr * __builtin_amdgcn_cosf(u2), ly = srr * __builtin_amdgcn_sinf(u2);
+        float tx, ty, tz;
+        if (fabsf(mxx) > fabsf(myy)) {
+            const float inv = __builtin_amdgcn_rsqf(mxx * mxx + mzz * mzz);
+            tx = mzz * inv; ty = 0.0f; tz = -mxx * inv;
+        } else {
+            const float inv = __builtin_amdgcn_rsqf(myy * myy + mzz * mzz);
+            tx = 0.0f; ty = mzz * inv; tz = -myy * inv;
+        }
+        const float bxx = ty * mzz - tz * myy, byy = tz * mxx - tx * mzz, bzz = tx * myy - ty * mxx;
+        const float dx = lx * tx + ly * bxx + w * mxx;
+        const float dy = lx * ty + ly * byy + w * myy;
+        const float dz = lx * tz + ly * bzz + w * mzz;
+        const float il = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+        sx = dx * il;
+        sy = dy * il;
+        sz = dz * il;
+    }
+    float pa = 0.0f, pb = 0.0f;
+#pragma unroll
+    for (int i = 0; i < KP_G_LOBES; ++i) {
+        const LobeQ L = g_lobe_q(o + 4 * i, pi[i]);
+        pa += g_lobe_eval(L, q.ax, q.ay, q.az);
+        pb += g_lobe_eval(L, sx, sy, sz);
+    }
+    ans.pdf_a = pa;
+    ans.pdf_b = pb;
+    ans.dx = sx;
+    ans.dy = sy;
+    ans.dz = sz;
 }
 
 }  // namespace kp
