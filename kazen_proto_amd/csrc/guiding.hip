@@ -115,47 +115,65 @@ __global__ __launch_bounds__(256) void k_guide_eval(GuideNet net, const float* _
 }
 
 // ----------------------------------------------------------- training ----
-// d(-scale * w * log q)/d outputs, same formulas as the oracle.
-__device__ inline void g_output_grad(const float* o, float wx, float wy, float wz, float weight,
-                                     float* gout) {
-    const MixNorm nrm = g_mix_norm(o);
-    float v[KP_G_LOBES];
-    float q = 0.0f;
+// d(-scale * w * log q)/d outputs, same formulas as the oracle, split over
+// the four consecutive lanes of a sample: lane
+// part (0..3) takes lobes part and part + 4 and writes their 8 outputs to
+// g8 (lobe part, then lobe part + 4); the mixture density q is summed over
+// the four lanes.  The mixture weights use FP32 exponentials and an FP64
+// normalisation.  All four lanes must call it.
+__device__ inline void g_output_grad_part(const float* o, int part, float wx, float wy, float wz, float weight,
+                                          float* g8) {
+    float mx = o[0];
 #pragma unroll
-    for (int i = 0; i < KP_G_LOBES; ++i) {
-        const LobeP L = g_lobe(o, nrm, i);
-        const float c = L.mx * wx + L.my * wy + L.mz * wz;
-        v[i] = L.norm * expf(L.kappa * (c - 1.0f));
-        q += L.pi * v[i];
+    for (int i = 1; i < KP_G_LOBES; ++i) mx = fmaxf(mx, o[4 * i]);
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < KP_G_LOBES; ++i) sum += (double)expf(o[4 * i] - mx);
+    const double inv_sum = 1.0 / sum;
+    float v[2], pi[2], k[2], c[2], mxv[2], myv[2], mzv[2];
+    float qp = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float* o4 = o + 4 * (part + 4 * h);
+        pi[h] = (float)((double)expf(o4[0] - mx) * inv_sum);
+        const float kk = fminf(g_softplus(o4[1]) + 0.01f, 1.0e4f);
+        const float norm = kk / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * kk)));
+        const float z = tanhf(o4[2]);
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sp, cp;
+        sincosf(3.14159265358979f * tanhf(o4[3]), &sp, &cp);
+        mxv[h] = rr * cp;
+        myv[h] = z;
+        mzv[h] = rr * sp;
+        k[h] = kk;
+        c[h] = mxv[h] * wx + myv[h] * wy + mzv[h] * wz;
+        v[h] = norm * expf(kk * (c[h] - 1.0f));
+        qp += pi[h] * v[h];
     }
-    q = fmaxf(q, 1e-30f);
+    qp += __shfl_xor(qp, 1);
+    qp += __shfl_xor(qp, 2);
+    const float q = fmaxf(qp, 1e-30f);
 #pragma unroll
-    for (int i = 0; i < KP_G_LOBES; ++i) {
-        const LobeP L = g_lobe(o, nrm, i);
-        const float k = L.kappa;
-        const float c = L.mx * wx + L.my * wy + L.mz * wz;
-        gout[4 * i] = -weight * L.pi * (v[i] - q) / q;
-        const float dlogc = 1.0f / k - 2.0f / expm1f(2.0f * k);
-        const float dv_dk = v[i] * (dlogc + c - 1.0f);
-        const float b = o[4 * i + 1];
+    for (int h = 0; h < 2; ++h) {
+        const float* o4 = o + 4 * (part + 4 * h);
+        float* g = g8 + 4 * h;
+        const float kk = k[h];
+        g[0] = -weight * pi[h] * (v[h] - q) / q;
+        const float dlogc = 1.0f / kk - 2.0f / expm1f(2.0f * kk);
+        const float dv_dk = v[h] * (dlogc + c[h] - 1.0f);
+        const float b = o4[1];
         const bool clamped = g_softplus(b) + 0.01f > 1.0e4f;
         const float sig = 1.0f / (1.0f + expf(-b));
-        gout[4 * i + 1] = clamped ? 0.0f : -weight * L.pi * dv_dk / q * sig;
-        const float s = -weight * L.pi * v[i] * k / q;
-        const float z = L.my;
+        g[1] = clamped ? 0.0f : -weight * pi[h] * dv_dk / q * sig;
+        const float sc = -weight * pi[h] * v[h] * kk / q;
+        const float z = myv[h];
         const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        const float td = tanhf(o[4 * i + 3]);
+        const float td = tanhf(o4[3]);
         const float dphi = 3.14159265358979f * (1.0f - td * td);
-        // cos/sin phi from mu
-        const float cp = rr > 0.0f ? L.mx / rr : 1.0f, sp = rr > 0.0f ? L.mz / rr : 0.0f;
-        gout[4 * i + 2] = s * (-z * rr * cp * wx + (1.0f - z * z) * wy - z * rr * sp * wz);
-        gout[4 * i + 3] = s * (-rr * sp * dphi * wx + rr * cp * dphi * wz);
+        const float cp = rr > 0.0f ? mxv[h] / rr : 1.0f, sp = rr > 0.0f ? mzv[h] / rr : 0.0f;
+        g[2] = sc * (-z * rr * cp * wx + (1.0f - z * z) * wy - z * rr * sp * wz);
+        g[3] = sc * (-rr * sp * dphi * wx + rr * cp * dphi * wz);
     }
-}
-
-__device__ __forceinline__ uint32_t lds_bf16(const uint32_t* buf, int row, int col) {
-    const uint32_t w = buf[row * KP_G_ROW + (col >> 1)];
-    return (col & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
 // Weight-gradient tiles (16 x 16) of the network in a fixed enumeration:
@@ -288,16 +306,42 @@ __global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const floa
 
     for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {
         const uint32_t base = grp * 64;
-        if (tid < 64) {
-            const bool valid = base + tid < n;
-            uint32_t* row = act[0] + tid * KP_G_ROW;
-            g_write_features(row, in6 + 6 * (size_t)(valid ? base + tid : 0), valid);
+        {
+            // four lanes per sample: lanes 0-2 encode position x / y / z (6
+            // frequencies each), lane 3 the normal (2 frequencies per axis);
+            // the same features as g_write_features
+            const int smp = tid >> 2, part = tid & 3;
+            const bool valid = base + smp < n;
+            const float* src = in6 + 6 * (size_t)(valid ? base + smp : 0);
+            uint32_t wd[6];
+            if (part < 3) {
+                const float x = src[part];
 #pragma unroll
-            for (int w = 0; w < 24; ++w) {
-                const uint32_t x = row[w];
-                g_store_t(actT[0], 2 * w, tid, x & 0xffffu);
-                g_store_t(actT[0], 2 * w + 1, tid, x >> 16);
+                for (int k = 0; k < 6; ++k) {
+                    float sn, cs;
+                    sincospif(x * (float)(1 << k), &sn, &cs);
+                    wd[k] = valid ? pack_bf2(sn, cs) : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        float sn, cs;
+                        sincospif(src[3 + c] * (float)(1 << k), &sn, &cs);
+                        wd[2 * c + k] = valid ? pack_bf2(sn, cs) : 0u;
+                    }
             }
+            uint32_t* row = act[0] + smp * KP_G_ROW;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const int w = 6 * part + j;
+                row[w] = wd[j];
+                g_store_t(actT[0], 2 * w, smp, wd[j] & 0xffffu);
+                g_store_t(actT[0], 2 * w + 1, smp, wd[j] >> 16);
+            }
+            row[24 + 2 * part] = 0u;
+            row[25 + 2 * part] = 0u;
         }
         __syncthreads();
         {
@@ -308,24 +352,26 @@ __global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const floa
                             nullptr, at, 16 * wave);
         }
         __syncthreads();
-        if (tid < 64) {
-            const uint32_t s = base + tid;
-            float gout[KP_G_OUT];
-            if (s < n) {
-                float o[KP_G_OUT];
-                for (int i = 0; i < KP_G_OUT; ++i) o[i] = outs[tid * KP_G_ROW + i];
-                g_output_grad(o, dirs[3 * (size_t)s], dirs[3 * (size_t)s + 1], dirs[3 * (size_t)s + 2],
-                              weights[s] * scale, gout);
-            } else {
-                for (int i = 0; i < KP_G_OUT; ++i) gout[i] = 0.0f;
+        {
+            const int smp = tid >> 2, part = tid & 3;
+            const uint32_t sg = base + smp;
+            const bool valid = sg < n;
+            const size_t si = valid ? sg : 0;
+            float g8[8];
+            g_output_grad_part(outs + smp * KP_G_ROW, part, dirs[3 * si], dirs[3 * si + 1], dirs[3 * si + 2],
+                               valid ? weights[si] * scale : 0.0f, g8);
+            uint32_t* row = dyb[0] + smp * KP_G_ROW;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int lobe = part + 4 * h;
+                const float* g = g8 + 4 * h;
+                row[2 * lobe] = valid ? pack_bf2(g[0], g[1]) : 0u;
+                row[2 * lobe + 1] = valid ? pack_bf2(g[2], g[3]) : 0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) g_store_t(dyT[0], 4 * lobe + j, smp, valid ? f2bf_bits(g[j]) : 0u);
             }
-            uint32_t* row = dyb[0] + tid * KP_G_ROW;
-            for (int w = 0; w < 16; ++w) {
-                row[w] = pack_bf2(gout[2 * w], gout[2 * w + 1]);
-                g_store_t(dyT[0], 2 * w, tid, f2bf_bits(gout[2 * w]));
-                g_store_t(dyT[0], 2 * w + 1, tid, f2bf_bits(gout[2 * w + 1]));
-            }
-            for (int w = 16; w < 32; ++w) row[w] = 0u;
+#pragma unroll
+            for (int w = 16 + 4 * part; w < 20 + 4 * part; ++w) row[w] = 0u;
         }
         __syncthreads();
         g_backward_layer<4>(net, act, actT, dyb[0], dyT[0], dyb[1], dyT[1], tid, wave, lane, acc, bacc);

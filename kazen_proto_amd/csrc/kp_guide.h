@@ -228,100 +228,13 @@ __device__ inline void g_wave_forward(const GuideNet& net, uint32_t* x) {
 }
 
 // ------------------------------------------------------------ mixture ----
-// The mixture is evaluated straight from the 32 network outputs `o` (which
-// may live in LDS), one lobe at a time, so no per-lane lobe array is needed.
+// The mixture is evaluated straight from the 32 network outputs `o`, one
+// lobe at a time, so no per-lane lobe array is needed.
 // Lobe i: pi_i = softmax(o[4i]) (normalised in FP64), kappa_i =
 // min(softplus(o[4i+1]) + 0.01, 1e4), mu_i = (r cos phi, z, r sin phi) with
 // z = tanh(o[4i+2]), phi = pi tanh(o[4i+3]).
 
 __device__ __forceinline__ float g_softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
-
-struct MixNorm {
-    float mx;        // max logit
-    double inv_sum;  // 1 / sum exp(logit - mx)
-};
-
-__device__ __forceinline__ MixNorm g_mix_norm(const float* o) {
-    MixNorm n;
-    n.mx = o[0];
-#pragma unroll
-    for (int i = 1; i < KP_G_LOBES; ++i) n.mx = fmaxf(n.mx, o[4 * i]);
-    double sum = 0.0;
-#pragma unroll
-    for (int i = 0; i < KP_G_LOBES; ++i) sum += exp((double)(o[4 * i] - n.mx));
-    n.inv_sum = 1.0 / sum;
-    return n;
-}
-
-struct LobeP {
-    float pi, kappa, norm, mx, my, mz;
-};
-
-__device__ __forceinline__ LobeP g_lobe(const float* o, const MixNorm& n, int i) {
-    LobeP L;
-    L.pi = (float)(exp((double)(o[4 * i] - n.mx)) * n.inv_sum);
-    const float k = fminf(g_softplus(o[4 * i + 1]) + 0.01f, 1.0e4f);
-    L.kappa = k;
-    L.norm = k / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * k)));
-    const float z = tanhf(o[4 * i + 2]);
-    const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    float sp, cp;
-    sincosf(3.14159265358979f * tanhf(o[4 * i + 3]), &sp, &cp);
-    L.mx = rr * cp;
-    L.my = z;
-    L.mz = rr * sp;
-    return L;
-}
-
-// Mixture density of the unit direction w.
-__device__ inline float g_mix_pdf(const float* o, float wx, float wy, float wz) {
-    const MixNorm n = g_mix_norm(o);
-    float q = 0.0f;
-    for (int i = 0; i < KP_G_LOBES; ++i) {
-        const LobeP L = g_lobe(o, n, i);
-        const float c = L.mx * wx + L.my * wy + L.mz * wz;
-        q += L.pi * L.norm * expf(L.kappa * (c - 1.0f));
-    }
-    return q;
-}
-
-// Lobe by inverse CDF of pi (u0), then the vMF sample (W. Jakob 2012).
-__device__ inline void g_mix_sample(const float* o, float u0, float u1, float u2, float& wx, float& wy,
-                                    float& wz) {
-    const MixNorm n = g_mix_norm(o);
-    int lobe = KP_G_LOBES - 1;
-    double acc = 0.0;
-    for (int i = 0; i < KP_G_LOBES - 1; ++i) {
-        acc += exp((double)(o[4 * i] - n.mx)) * n.inv_sum;
-        if ((double)u0 < acc) {
-            lobe = i;
-            break;
-        }
-    }
-    const LobeP l = g_lobe(o, n, lobe);
-    const float k = l.kappa;
-    const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
-    const float sr = sqrtf(fmaxf(0.0f, 1.0f - w * w));
-    float sp, cp;
-    sincosf(2.0f * 3.14159265358979f * u2, &sp, &cp);
-    const float lx = sr * cp, ly = sr * sp;
-    float tx, ty, tz;
-    if (fabsf(l.mx) > fabsf(l.my)) {
-        const float inv = 1.0f / sqrtf(l.mx * l.mx + l.mz * l.mz);
-        tx = l.mz * inv; ty = 0.0f; tz = -l.mx * inv;
-    } else {
-        const float inv = 1.0f / sqrtf(l.my * l.my + l.mz * l.mz);
-        tx = 0.0f; ty = l.mz * inv; tz = -l.my * inv;
-    }
-    const float bx = ty * l.mz - tz * l.my, by = tz * l.mx - tx * l.mz, bz = tx * l.my - ty * l.mx;
-    const float dx = lx * tx + ly * bx + w * l.mx;
-    const float dy = lx * ty + ly * by + w * l.my;
-    const float dz = lx * tz + ly * bz + w * l.mz;
-    const float len = sqrtf(dx * dx + dy * dy + dz * dz);
-    wx = dx / len;
-    wy = dy / len;
-    wz = dz / len;
-}
 
 // ---------------------------------------------- wave-level guide queries --
 // Features of one sample written straight into an LDS row as bf16 pairs
