@@ -41,6 +41,12 @@ int ensure_buffer(kp_ctx* c, void** buf, size_t* cap, size_t bytes) {
     return KP_OK;
 }
 
+// Largest persistent grid of any traversal launch: the spill area has one
+// slice per lane of it.
+uint32_t max_trace_grid(const kp_ctx* c) {
+    return std::max(std::max(c->grid_closest, c->grid_occluded), std::max(c->grid_closest5, c->grid_occluded5));
+}
+
 TraceArgs trace_base(kp_ctx* c) {
     TraceArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -50,6 +56,7 @@ TraceArgs trace_base(kp_ctx* c) {
     a.err = c->d_words + 1;
     a.spill = c->d_spill;
     a.spill_cap = c->spill_cap;
+    a.spill_blocks = (uint32_t)(c->spill_lanes / KP_TRACE_BLOCK);
     a.refill_below = c->refill_below;
     a.counters = c->d_counters;
     a.max_steps = c->max_steps;
@@ -289,8 +296,7 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     // return marker, 7 per prototype level.
     // Postponed leaves need no stack entries.
     c->spill_cap = 8u * top.max_depth + 7u * max_unit_depth + 16u;
-    const uint32_t grid = c->grid_closest > c->grid_occluded ? c->grid_closest : c->grid_occluded;
-    c->spill_lanes = (size_t)grid * KP_TRACE_BLOCK;
+    c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
     KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
     c->two_level = true;
     c->world_root = has_world ? node_off[0] : KP_INVALID;
@@ -337,13 +343,18 @@ int kp_create(kp_ctx** out, int device) {
         return bail(e, "stream");
     c->stream = c->own_stream;
     if ((e = hipMalloc((void**)&c->d_words, 64)) != hipSuccess) return bail(e, "alloc");
-    if ((e = hipMemset(c->d_words, 0, 64)) != hipSuccess) return bail(e, "memset");
+    if ((e = hipMemsetAsync(c->d_words, 0, 64, c->stream)) != hipSuccess) return bail(e, "memset");
     if ((e = hipMalloc((void**)&c->d_counters, 64)) != hipSuccess) return bail(e, "alloc");
-    if ((e = hipMemset(c->d_counters, 0, 64)) != hipSuccess) return bail(e, "memset");
+    if ((e = hipMemsetAsync(c->d_counters, 0, 64, c->stream)) != hipSuccess) return bail(e, "memset");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(e, "memset");
     int bc = 0, bo = 0;
     if ((e = kp::trace_occupancy(&bc, &bo)) != hipSuccess) return bail(e, "occupancy");
     c->grid_closest = (uint32_t)(c->cus * (bc > 0 ? bc : 1));
     c->grid_occluded = (uint32_t)(c->cus * (bo > 0 ? bo : 1));
+    int b5c = 0, b5o = 0;
+    if ((e = kp::trace_occupancy_five(&b5c, &b5o)) != hipSuccess) return bail(e, "occupancy");
+    c->grid_closest5 = (uint32_t)(c->cus * (b5c > 0 ? b5c : 1));
+    c->grid_occluded5 = (uint32_t)(c->cus * (b5o > 0 ? b5o : 1));
     int sp = 0, sg = 0;
     if ((e = kp::shade_occupancy(&sp, &sg)) != hipSuccess) return bail(e, "occupancy");
     c->shade_blocks = (uint32_t)(c->cus * (sp > 0 ? sp : 1));
@@ -386,7 +397,8 @@ int kp_synchronize(kp_ctx* c) {
     uint32_t flags = 0;
     KP_HIP_TRY(c, hipMemcpy(&flags, c->d_words + 1, 4, hipMemcpyDeviceToHost));
     if (flags) {
-        KP_HIP_TRY(c, hipMemset(c->d_words + 1, 0, 4));
+        KP_HIP_TRY(c, hipMemsetAsync(c->d_words + 1, 0, 4, c->stream));
+        KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
         return kp::fail(c, KP_ERR_KERNEL,
                         (flags & 1u) ? "traversal stack exceeded its spill capacity"
                                      : "traversal exceeded its step limit");
@@ -574,8 +586,7 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     // Worst-case stack: 7 pushes per level below the root, plus slack; the
     // top KP_TRACE_STACK entries live in LDS.
     c->spill_cap = 7u * h.max_depth + 8u;
-    const uint32_t grid = c->grid_closest > c->grid_occluded ? c->grid_closest : c->grid_occluded;
-    c->spill_lanes = (size_t)grid * KP_TRACE_BLOCK;
+    c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
     KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
     c->n_tris_built = n_all;
     c->n_tris_stored = n_all;

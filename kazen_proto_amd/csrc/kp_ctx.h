@@ -49,6 +49,7 @@ struct kp_ctx {
     size_t spill_lanes = 0;
     uint32_t grid_closest = 0, grid_occluded = 0;
     uint32_t shade_blocks = 0, shade_blocks_guided = 0;   // resident blocks of k_shade, whole device
+    uint32_t grid_closest5 = 0, grid_occluded5 = 0;       // the integrator's 5-wave traversal kernels
     uint32_t refill_below = 48;   // measured best for incoherent rays (leaf 4)
     uint32_t pass_paths = 1u << 26;   // paths per pass of the integrator (measured: larger is faster)
     uint32_t max_steps = 64;
@@ -79,6 +80,7 @@ int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, b
 int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out);
 // Traversal arguments of the context's BVH (one- or two-level).
 TraceArgs trace_base(kp_ctx* c);
+uint32_t max_trace_grid(const kp_ctx* c);
 // World-space triangles (9 floats each) of every primitive id: the
 // kp_mesh_add meshes, then each instance's prototype under its
 // object-to-world matrix, each coordinate ((m0 x + m1 y) + m2 z) + m3 in FP64

@@ -20,6 +20,7 @@ struct TraceArgs {
     uint32_t* counter;      // work counter (reset by the launcher)
     uint2* spill;           // per-lane stack spill area
     uint32_t spill_cap;     // entries per lane
+    uint32_t spill_blocks;  // blocks of KP_TRACE_BLOCK lanes the spill area covers (launch grid limit)
     uint32_t refill_below;  // refill a wave once fewer lanes than this are busy
     uint32_t* err;          // bit0 stack overflow, bit1 step limit
     uint32_t max_steps;     // per-ray step limit (> any legitimate traversal)
@@ -34,11 +35,15 @@ struct TraceArgs {
     const uint32_t* tlas_inst;
     uint32_t n_tlas;
     uint32_t world_root;
+    // one-level closest / occluded launches of the integrator: the kernels
+    // compiled for 5 waves per SIMD (96 registers) instead of 4
+    uint32_t five_waves;
 };
 
 hipError_t launch_closest(const TraceArgs& a, uint32_t grid, bool count, hipStream_t s);
 hipError_t launch_occluded(const TraceArgs& a, uint32_t grid, hipStream_t s);
 hipError_t trace_occupancy(int* blocks_per_cu_closest, int* blocks_per_cu_occluded);
+hipError_t trace_occupancy_five(int* blocks_per_cu_closest, int* blocks_per_cu_occluded);
 hipError_t shade_occupancy(int* blocks_per_cu_plain, int* blocks_per_cu_guided);
 hipError_t launch_reset_u32(uint32_t* p, uint32_t n, hipStream_t s);
 hipError_t launch_copy16(void* dst, const void* src, uint64_t n16, int cus, hipStream_t s);

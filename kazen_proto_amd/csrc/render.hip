@@ -396,7 +396,12 @@ template <bool kGuided>
 __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t depth) {
     __shared__ uint32_t glds[kGuided ? 4 * KP_G_WAVE_ROWS * KP_G_ROW : 1];
     __shared__ uint32_t app_lds[10];
-    const uint32_t n = a.ps.qc(cur);
+    __shared__ uint32_t n_blk;
+    // one read of the queue size per block: the loop below synchronises the
+    // block every iteration, so all its threads must agree on the trip count
+    if (threadIdx.x == 0) n_blk = a.ps.qc(cur);
+    __syncthreads();
+    const uint32_t n = n_blk;
     const uint32_t total = gridDim.x * 256;
     const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
     const uint32_t iters = (n + total - 1) / total;
@@ -965,8 +970,11 @@ static int alloc_paths(kp_ctx* c, PathState& ps, void*& mem, uint32_t& cap, uint
     ps.qroot[0] = (uint32_t*)(b + o_qr0);
     ps.qroot[1] = (uint32_t*)(b + o_qr1);
     ps.sroot = (uint32_t*)(b + o_sr2);
-    KP_HIP_TRY(c, hipMemset(ps.cnt, 0, 3 * 256));
-    KP_HIP_TRY(c, hipMemset(ps.totals, 0, 64));
+    // on the context's stream (created non-blocking: null-stream work would
+    // not be ordered with its kernels), never inside a captured graph
+    KP_HIP_TRY(c, hipMemsetAsync(ps.cnt, 0, 3 * 256, c->stream));
+    KP_HIP_TRY(c, hipMemsetAsync(ps.totals, 0, 64, c->stream));
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     cap = n;
     return KP_OK;
 }
@@ -1043,6 +1051,11 @@ static TraceCfg ctx_trace_cfg(kp_ctx* c) {
     t.base = trace_base(c);
     t.grid_closest = c->grid_closest;
     t.grid_occluded = c->grid_occluded;
+    if (!c->two_level) {
+        t.base.five_waves = 1;
+        t.grid_closest = c->grid_closest5;
+        t.grid_occluded = c->grid_occluded5;
+    }
     // as many blocks as stay resident: every wave strides over the queue in
     // one round, no second partial round of blocks
     t.shade_blocks = c->shade_blocks;
@@ -1398,26 +1411,27 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
     alloc((void**)&b->d_pass, 64);
     if (e != hipSuccess) return bail(e, "kp_batch_create: allocation");
     uint32_t on = 0, ot = 0, om = 0, ol = 0, olt = 0, ocdf = 0;
+    uint64_t steps = 0;   // per-ray step limit: a ray stays in its own scene's BVH
     for (uint32_t i = 0; i < n; ++i) {
         kp_ctx* s = scenes[i];
         const RenderState* r = s->render;
         const uint32_t sn = s->n_nodes, st = s->n_tris_built, sm = (uint32_t)r->mats.size();
-        e = hipMemcpy(b->nodes + 8 * (size_t)on, s->d_nodes, (size_t)sn * KP_NODE_BYTES, hipMemcpyDeviceToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(b->tris + 3 * (size_t)ot, s->d_tris, (size_t)st * KP_TRI_BYTES, hipMemcpyDeviceToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(b->stris + 3 * (size_t)ot, r->stris, (size_t)st * 3 * sizeof(float4),
-                          hipMemcpyDeviceToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(b->mats + om, r->d_mats, (size_t)sm * sizeof(Material), hipMemcpyDeviceToDevice);
-        if (e == hipSuccess && r->n_lights) {
-            e = hipMemcpy(b->lights + ol, r->d_lights, (size_t)r->n_lights * sizeof(LightRec),
-                          hipMemcpyDeviceToDevice);
-            if (e == hipSuccess)
-                e = hipMemcpy(b->light_tris + olt, r->d_light_tris, (size_t)r->n_light_tris * 4,
-                              hipMemcpyDeviceToDevice);
-            if (e == hipSuccess)
-                e = hipMemcpy(b->light_cdf + ocdf, r->d_light_cdf, (size_t)r->n_cdf * 4, hipMemcpyDeviceToDevice);
+        // The copies go on the batch's stream, in order with the rebase kernels
+        // below; the scene's own stream is drained first (its buffers are the
+        // sources).  The final synchronisation of the batch's stream then also
+        // covers the copies, so the caller may release the scene contexts.
+        e = hipStreamSynchronize(s->stream);
+        auto copy = [&](void* dst, const void* src, size_t bytes) {
+            if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream);
+        };
+        copy(b->nodes + 8 * (size_t)on, s->d_nodes, (size_t)sn * KP_NODE_BYTES);
+        copy(b->tris + 3 * (size_t)ot, s->d_tris, (size_t)st * KP_TRI_BYTES);
+        copy(b->stris + 3 * (size_t)ot, r->stris, (size_t)st * 3 * sizeof(float4));
+        copy(b->mats + om, r->d_mats, (size_t)sm * sizeof(Material));
+        if (r->n_lights) {
+            copy(b->lights + ol, r->d_lights, (size_t)r->n_lights * sizeof(LightRec));
+            copy(b->light_tris + olt, r->d_light_tris, (size_t)r->n_light_tris * 4);
+            copy(b->light_cdf + ocdf, r->d_light_cdf, (size_t)r->n_cdf * 4);
         }
         if (e != hipSuccess) return bail(e, "kp_batch_create: copy");
         hipLaunchKernelGGL(k_rebase_nodes, dim3((sn + 255) / 256), dim3(256), 0, c->stream, b->nodes + 8 * (size_t)on,
@@ -1431,6 +1445,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
             hipLaunchKernelGGL(k_rebase_lights, dim3((nmax + 255) / 256), dim3(256), 0, c->stream, b->lights + ol,
                                r->n_lights, olt, ocdf, b->light_tris + olt, r->n_light_tris, ot);
         if ((e = hipGetLastError()) != hipSuccess) return bail(e, "kp_batch_create: rebase");
+        steps = std::max<uint64_t>(steps, 2ull * sn + st + 64u);
         b->root.push_back(on);
         b->light_first.push_back(ol);
         b->n_lights.push_back(r->n_lights);
@@ -1445,7 +1460,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
     b->n_tris = ot;
     b->bvh_depth = depth;
     b->spill_cap = 7u * depth + 8u;
-    b->max_steps = 2u * on + ot + 64u;
+    b->max_steps = (uint32_t)std::min<uint64_t>(steps, 0xFFFFFFF0ull);
     const uint32_t grid = std::max(c->grid_closest, c->grid_occluded);
     e = hipMalloc((void**)&b->spill, (size_t)grid * KP_TRACE_BLOCK * b->spill_cap * sizeof(uint2));
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1592,6 +1607,7 @@ static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max
     tc.base.counter = c->d_words;
     tc.base.err = c->d_words + 1;
     tc.base.spill = b->spill;
+    tc.base.spill_blocks = std::max(c->grid_closest, c->grid_occluded);
     tc.base.spill_cap = b->spill_cap;
     tc.base.refill_below = b->tune.refill_below;
     tc.base.max_steps = b->max_steps;
