@@ -212,9 +212,13 @@ def batch_leg(args):
     # one scene after another (each render is its own sequence of launches);
     # each context frees its path state afterwards, so at most one is held
     seq_sec = 0.0
+    seq_rays = 0
+    films = []
     for c, d in zip(ctxs, descs):
-        c.render(d)
-        seq_sec += c.render_stats()["seconds"]
+        films.append(c.render(d))
+        st1 = c.render_stats()
+        seq_sec += st1["seconds"]
+        seq_rays += st1["extension_rays"] + st1["shadow_rays"]
         c.release_buffers()
     print("[bench] batch: one by one %.3fs (%.1fs)" % (seq_sec, time.time() - t0), file=sys.stderr, flush=True)
     b = native.Batch(owner, ctxs)
@@ -227,6 +231,10 @@ def batch_leg(args):
     b.render(descs)                 # warm-up with the tuned settings
     b.render(descs)
     st = b.stats()
+    # each batch film must equal the scene's own render bit for bit (same
+    # paths, same per-pixel summation order)
+    films_equal = all(np.array_equal(b.film_read(i, d.width, d.height).view(np.uint32), f.view(np.uint32))
+                      for i, (d, f) in enumerate(zip(descs, films)))
     b.close()
     owner.close()
     rays = st["extension_rays"] + st["shadow_rays"]
@@ -236,6 +244,7 @@ def batch_leg(args):
            "seconds": round(st["seconds"], 4), "samples_per_s": round(st["paths"] / st["seconds"], 1),
            "mrays_per_s": round(rays / st["seconds"] / 1e6, 2), "passes": int(st["passes"]),
            "one_by_one_seconds": round(seq_sec, 4), "speedup_vs_one_by_one": round(seq_sec / st["seconds"], 3),
+           "rays": int(rays), "one_by_one_rays": int(seq_rays), "films_equal_one_by_one": bool(films_equal),
            "tuning": tuned, "setup_seconds": round(time.time() - t0, 1), "generate_seconds": round(gen, 1)}
     print("[bench] batch: %s" % out, file=sys.stderr, flush=True)
     return out

@@ -356,7 +356,7 @@ int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);
  * node width is 8 in every configuration. */
 typedef struct kp_batch kp_batch;
 typedef struct {
-    uint32_t pass_paths;           /* paths in flight per pass (queue size), 256 .. 2^22 */
+    uint32_t pass_paths;           /* paths in flight per pass (queue size), 256 .. 2^25 */
     uint32_t shade_blocks_per_cu;  /* 256-thread shade / connect blocks per CU */
     uint32_t trace_blocks_per_cu;  /* persistent traversal blocks per CU (occupancy) */
     uint32_t refill_below;         /* traversal wave refill threshold, lanes 1..64 */

@@ -1461,13 +1461,13 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
     b->bvh_depth = depth;
     b->spill_cap = 7u * depth + 8u;
     b->max_steps = (uint32_t)std::min<uint64_t>(steps, 0xFFFFFFF0ull);
-    const uint32_t grid = std::max(c->grid_closest, c->grid_occluded);
+    const uint32_t grid = max_trace_grid(c);
     e = hipMalloc((void**)&b->spill, (size_t)grid * KP_TRACE_BLOCK * b->spill_cap * sizeof(uint2));
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return bail(e, "kp_batch_create: spill / rebase");
-    b->tune.pass_paths = 1u << 21;
-    b->tune.shade_blocks_per_cu = 8;
-    b->tune.trace_blocks_per_cu = c->grid_closest / (uint32_t)c->cus;
+    b->tune.pass_paths = 1u << 24;
+    b->tune.shade_blocks_per_cu = c->shade_blocks / (uint32_t)c->cus;
+    b->tune.trace_blocks_per_cu = c->grid_closest5 / (uint32_t)c->cus;
     b->tune.refill_below = c->refill_below;
     b->tune.use_graph = 1;
     if (verbose()) std::fprintf(stderr, "[kp] batch: %u scenes merged\n", n);
@@ -1577,7 +1577,9 @@ static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max
     KP_HIP_TRY(c, hipMemcpy(b->d_scenes, b->h_scenes.data(), n * sizeof(SceneRec), hipMemcpyHostToDevice));
     const unsigned long long pass_words[2] = {0ull, work};
     KP_HIP_TRY(c, hipMemcpy(b->d_pass, pass_words, sizeof(pass_words), hipMemcpyHostToDevice));
-    const uint32_t chunk = b->tune.pass_paths;
+    // a pass holds at most the batch's whole work (small batches need no
+    // path state of the full pass size), in multiples of 256 paths
+    const uint32_t chunk = (uint32_t)std::min<unsigned long long>(b->tune.pass_paths, (work + 255ull) & ~255ull);
     const uint32_t old_cap = b->cap_paths;
     void* old_mem = b->path_mem;
     int rc = alloc_paths(c, b->ps, b->path_mem, b->cap_paths, chunk);
@@ -1607,15 +1609,16 @@ static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max
     tc.base.counter = c->d_words;
     tc.base.err = c->d_words + 1;
     tc.base.spill = b->spill;
-    tc.base.spill_blocks = std::max(c->grid_closest, c->grid_occluded);
+    tc.base.spill_blocks = max_trace_grid(c);
+    tc.base.five_waves = 1;   // the integrator's 5-wave traversal kernels
     tc.base.spill_cap = b->spill_cap;
     tc.base.refill_below = b->tune.refill_below;
     tc.base.max_steps = b->max_steps;
     tc.base.counters = c->d_counters;
-    const uint32_t tb_max = std::max(c->grid_closest, c->grid_occluded) / (uint32_t)c->cus;
+    const uint32_t tb_max = std::max(c->grid_closest5, c->grid_occluded5) / (uint32_t)c->cus;
     const uint32_t tb = std::min(std::max(b->tune.trace_blocks_per_cu, 1u), tb_max);
-    tc.grid_closest = std::min((uint32_t)c->cus * tb, c->grid_closest);
-    tc.grid_occluded = std::min((uint32_t)c->cus * tb, c->grid_occluded);
+    tc.grid_closest = std::min((uint32_t)c->cus * tb, c->grid_closest5);
+    tc.grid_occluded = std::min((uint32_t)c->cus * tb, c->grid_occluded5);
     tc.shade_blocks = (uint32_t)c->cus * std::max(b->tune.shade_blocks_per_cu, 1u);
     tc.shade_blocks_guided = c->shade_blocks_guided;
     tc.roots = true;
@@ -1673,7 +1676,7 @@ static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max
 // path, the first pass of each candidate is a warm-up).
 static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t passes, kp_batch_tuning* out) {
     kp_ctx* c = b->c;
-    const uint32_t tb_max = std::max(c->grid_closest, c->grid_occluded) / (uint32_t)c->cus;
+    const uint32_t tb_max = std::max(c->grid_closest5, c->grid_occluded5) / (uint32_t)c->cus;
     auto cost = [&](const kp_batch_tuning& t, double* sec_per_path) -> int {
         b->tune = t;
         uint32_t run = 0;
@@ -1706,7 +1709,7 @@ static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t p
         }
         return KP_OK;
     };
-    if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 19, 1u << 20, 1u << 21, 1u << 22}))) return rc;
+    if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25}))) return rc;
     if ((rc = try_values(&kp_batch_tuning::trace_blocks_per_cu, {tb_max, std::max(1u, tb_max / 2), std::max(1u, tb_max - 1)})))
         return rc;
     if ((rc = try_values(&kp_batch_tuning::refill_below, {16u, 32u, 48u, 56u}))) return rc;
@@ -1859,7 +1862,7 @@ int kp_batch_destroy(kp_batch* h) {
 int kp_batch_set_tuning(kp_batch* h, const kp_batch_tuning* t) {
     if (!h || !t) return KP_ERR_INVALID;
     kp::BatchState* b = reinterpret_cast<kp::BatchState*>(h);
-    if (t->pass_paths < 256 || t->pass_paths > (1u << 22) || t->shade_blocks_per_cu == 0 ||
+    if (t->pass_paths < 256 || t->pass_paths > (1u << 25) || t->shade_blocks_per_cu == 0 ||
         t->shade_blocks_per_cu > 64 || t->trace_blocks_per_cu == 0 || t->refill_below == 0 || t->refill_below > 64)
         return kp::fail(b->c, KP_ERR_INVALID, "kp_batch_set_tuning: value out of range");
     b->tune = *t;
