@@ -252,9 +252,10 @@ def batch_leg(args):
 
 def batch_leg_child(args):
     """The batched-scenes leg in a child process with a time limit, so that a
-    stall in it (two full runs once stopped making progress in batch set-up
-    with larger batch passes) cannot cost the whole benchmark line: on a
-    timeout or failure the leg is reported with its error and no numbers."""
+    stall in it cannot cost the whole benchmark line (runs once stalled in
+    batch set-up while the merged BVH copies were not ordered with the kernels
+    that rebase them): on a timeout or failure the leg is reported with its
+    error and no numbers."""
     cmd = [sys.executable, os.path.abspath(__file__), "--batch-only", "--batch-scenes", str(args.batch_scenes)]
     try:
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=args.batch_timeout, check=False)
