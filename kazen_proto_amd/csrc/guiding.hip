@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_guide_eval(GuideNet net, const float* _
                                                     float* out32, const float* dirs, const float* u3,
                                                     float* dirs_out, float* pdf_out, int mode) {
     // mode 0: 64 feature rows + two 16-row activation buffers per wave
-    __shared__ uint32_t lds[4][96 * KP_G_ROW];
+    __shared__ __align__(16) uint32_t lds[4][96 * KP_G_ROW];   // 16-byte row fragments
     const int wave = threadIdx.x >> 6, lane = __lane_id();
     uint32_t* feat = lds[wave];
     uint32_t* ping = feat + 64 * KP_G_ROW;
@@ -290,11 +290,12 @@ __global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const floa
                                                         const float* __restrict__ weights,
                                                         const uint32_t* n_dev, const float* scale_dev,
                                                         float* __restrict__ partial) {
-    __shared__ uint32_t act[KP_G_LAYERS][64 * KP_G_ROW];    // layer inputs, sample-major (bf16)
-    __shared__ uint32_t actT[KP_G_LAYERS][64 * KP_G_TROW];  // the same, feature-major
-    __shared__ uint32_t dyb[2][64 * KP_G_ROW];              // gradients, sample-major
-    __shared__ uint32_t dyT[2][64 * KP_G_TROW];             // gradients, feature-major
-    __shared__ float outs[64 * KP_G_ROW];                   // network outputs (f32)
+    // all read and written as 16-byte fragments
+    __shared__ __align__(16) uint32_t act[KP_G_LAYERS][64 * KP_G_ROW];    // layer inputs, sample-major (bf16)
+    __shared__ __align__(16) uint32_t actT[KP_G_LAYERS][64 * KP_G_TROW];  // the same, feature-major
+    __shared__ __align__(16) uint32_t dyb[2][64 * KP_G_ROW];              // gradients, sample-major
+    __shared__ __align__(16) uint32_t dyT[2][64 * KP_G_TROW];             // gradients, feature-major
+    __shared__ __align__(16) float outs[64 * KP_G_ROW];                   // network outputs (f32)
     const uint32_t n = *n_dev;
     const uint32_t groups = (n + 63) / 64;
     const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();

@@ -394,7 +394,7 @@ __device__ __forceinline__ uint32_t block_append2(bool want0, bool want1, uint32
 // network query runs once per wave after every lane has prepared its vertex.
 template <bool kGuided>
 __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t depth) {
-    __shared__ uint32_t glds[kGuided ? 4 * KP_G_WAVE_ROWS * KP_G_ROW : 1];
+    __shared__ __align__(16) uint32_t glds[kGuided ? 4 * KP_G_WAVE_ROWS * KP_G_ROW : 1];   // 16-byte row fragments
     __shared__ uint32_t app_lds[10];
     __shared__ uint32_t n_blk;
     // one read of the queue size per block: the loop below synchronises the
