@@ -274,20 +274,41 @@ __global__ __launch_bounds__(256) void k_generate_batch(RenderArgs a) {
 // render's bit for bit whatever the pass size.
 __global__ __launch_bounds__(256) void k_film_batch(RenderArgs a, float4* sum, float4* comp) {
 #pragma clang fp contract(off)
-    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    // The block's 256 slots are staged in LDS with coalesced loads; the
+    // first slot of every pixel run then folds the run from LDS, continuing
+    // from global memory only where a run crosses the block's end.  Same
+    // additions in the same order as a fold from global memory.
+    __shared__ float4 xs[256];
+    __shared__ uint32_t ss[257], sp[257];   // [0]: the slot before the block
+    const uint32_t base = blockIdx.x * 256;
+    const uint32_t t = threadIdx.x, slot = base + t;
     const uint32_t n = batch_pass_paths(a);
+    if (slot < n) {
+        xs[t] = a.ps.s1[slot];
+        ss[t + 1] = a.ps.slot_scene[slot];
+        sp[t + 1] = a.ps.slot_pixel[slot];
+    }
+    if (t == 0) {
+        ss[0] = base > 0 ? a.ps.slot_scene[base - 1] : 0xFFFFFFFFu;
+        sp[0] = base > 0 ? a.ps.slot_pixel[base - 1] : 0xFFFFFFFFu;
+    }
+    __syncthreads();
     if (slot >= n) return;
-    const uint32_t sid = a.ps.slot_scene[slot], pix = a.ps.slot_pixel[slot];
-    if (slot > 0 && a.ps.slot_scene[slot - 1] == sid && a.ps.slot_pixel[slot - 1] == pix) return;
+    const uint32_t sid = ss[t + 1], pix = sp[t + 1];
+    if (ss[t] == sid && sp[t] == pix) return;   // not the first slot of its run
     const uint32_t f = a.scenes[sid].film_first + pix;
     float4 S = sum[f], C = comp[f];
-    for (uint32_t j = slot; j < n && a.ps.slot_scene[j] == sid && a.ps.slot_pixel[j] == pix; ++j) {
-        const float4 x = a.ps.s1[j];
-        float y, t;
-        y = x.x - C.x; t = S.x + y; C.x = (t - S.x) - y; S.x = t;
-        y = x.y - C.y; t = S.y + y; C.y = (t - S.y) - y; S.y = t;
-        y = x.z - C.z; t = S.z + y; C.z = (t - S.z) - y; S.z = t;
-    }
+    auto add = [&](const float4& x) {
+        float y, tt;
+        y = x.x - C.x; tt = S.x + y; C.x = (tt - S.x) - y; S.x = tt;
+        y = x.y - C.y; tt = S.y + y; C.y = (tt - S.y) - y; S.y = tt;
+        y = x.z - C.z; tt = S.z + y; C.z = (tt - S.z) - y; S.z = tt;
+    };
+    uint32_t j = t;
+    for (; j < 256 && base + j < n && ss[j + 1] == sid && sp[j + 1] == pix; ++j) add(xs[j]);
+    if (j == 256)
+        for (uint32_t g = base + 256; g < n && a.ps.slot_scene[g] == sid && a.ps.slot_pixel[g] == pix; ++g)
+            add(a.ps.s1[g]);
     sum[f] = S;
     comp[f] = C;
 }
