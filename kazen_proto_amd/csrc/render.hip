@@ -740,24 +740,15 @@ __global__ __launch_bounds__(256) void k_guide_records(RenderArgs a, uint32_t n_
 // with Kahan compensation.
 __global__ __launch_bounds__(256) void k_film(RenderArgs a, float4* sum, float4* comp) {
 #pragma clang fp contract(off)
-    // One thread per path slot: the block's 256 slots are staged in LDS with
-    // coalesced loads, then the first slot of every pixel (slot = pixel * k)
-    // folds the pixel's k samples in sample order, from LDS and, past the
-    // block's end, from global memory.
-    __shared__ float4 xs[256];
-    const uint32_t t = threadIdx.x, slot = blockIdx.x * 256 + t;
-    const uint32_t n = a.npix * a.k;
-    if (slot < n) xs[t] = a.ps.s1[slot];
-    __syncthreads();
-    if (slot >= n || slot % a.k != 0) return;
-    const uint32_t p = slot / a.k;
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.npix) return;
     float4 s = sum[p], c = comp[p];
     for (uint32_t j = 0; j < a.k; ++j) {
-        const float4 x = t + j < 256 ? xs[t + j] : a.ps.s1[(size_t)slot + j];
-        float y, tt;
-        y = x.x - c.x; tt = s.x + y; c.x = (tt - s.x) - y; s.x = tt;
-        y = x.y - c.y; tt = s.y + y; c.y = (tt - s.y) - y; s.y = tt;
-        y = x.z - c.z; tt = s.z + y; c.z = (tt - s.z) - y; s.z = tt;
+        const float4 x = a.ps.s1[(size_t)p * a.k + j];
+        float y, t;
+        y = x.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
+        y = x.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
+        y = x.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
     }
     sum[p] = s;
     comp[p] = c;
@@ -1299,7 +1290,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
         hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, a);
         KP_HIP_TRY(c, hipGetLastError());
         if ((rc = run_bounces(c, a, guided, ctx_trace_cfg(c), a.max_depth))) return rc;
-        hipLaunchKernelGGL(k_film, dim3((n_paths + 255) / 256), dim3(256), 0, c->stream, a, r->film_sum,
+        hipLaunchKernelGGL(k_film, dim3((npix + 255) / 256), dim3(256), 0, c->stream, a, r->film_sum,
                            r->film_comp);
         KP_HIP_TRY(c, hipGetLastError());
         if (train) {
