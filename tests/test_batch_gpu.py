@@ -112,6 +112,8 @@ def test_batch_rejects_guided_and_bad_input(setup):
     with pytest.raises(native.KpError):
         b.set_tuning(pass_paths=16)
     with pytest.raises(native.KpError):
+        b.set_tuning(pass_paths=1 << 26)   # at most 2^25 paths per batch pass
+    with pytest.raises(native.KpError):
         b.film_read(len(ss), 4, 4)
 
 
