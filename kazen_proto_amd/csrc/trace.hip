@@ -312,14 +312,16 @@ struct Stack {
 constexpr int kStack = KP_TRACE_STACK;
 
 // Work distribution of the persistent kernels.  A wave takes chunks of
-// kChunk consecutive rays from the global counter (one atomic per chunk) and
-// hands them to its lanes from a wave-uniform cursor; every lane keeps its
+// kChunk consecutive rays, its first chunk by its index and the following
+// ones from the global counter (one atomic per chunk), and hands them to its
+// lanes from a wave-uniform cursor; every lane keeps its
 // next ray prefetched in registers, so replacing a finished ray costs no
 // memory round trip.  All members except the prefetch are wave-uniform.
 constexpr uint32_t kChunk = 256;
 
 struct RayFeeder {
     uint32_t next, end, n;
+    uint32_t dyn_base;  // first ray handed out by the counter (after the static chunks)
     bool drained;
     uint32_t* counter;
     const float4* rays;
@@ -334,10 +336,23 @@ struct RayFeeder {
         roots = rts;
         proot = 0u;
         n = count;
-        next = end = 0;
-        drained = false;
         pending = false;
         pid = KP_INVALID;
+        // Wave w's first chunk is rays [w kChunk, (w+1) kChunk) without an
+        // atomic (every wave of the grid would otherwise hit the one counter
+        // address at start); the counter hands out the rays after all first
+        // chunks, and a wave whose first chunk lies beyond the rays has none.
+        const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+        const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        dyn_base = waves * kChunk;
+        next = wave * kChunk;
+        if (next < n) {
+            end = next + kChunk < n ? next + kChunk : n;
+            drained = false;
+        } else {
+            next = end = 0;
+            drained = true;
+        }
         assign();
     }
     // Gives a prefetched ray to every lane without one, as far as rays last.
@@ -345,9 +360,13 @@ struct RayFeeder {
         const uint64_t m = __ballot(!pending);
         if (m == 0 || drained) return;
         if (next >= end) {
+            if (dyn_base >= n) {   // the first chunks covered every ray
+                drained = true;
+                return;
+            }
             uint32_t base = 0;
             if (__lane_id() == 0) base = atomicAdd(counter, kChunk);
-            base = __builtin_amdgcn_readfirstlane(base);
+            base = dyn_base + __builtin_amdgcn_readfirstlane(base);
             if (base >= n) {
                 drained = true;
                 return;
