@@ -312,8 +312,9 @@ struct Stack {
 constexpr int kStack = KP_TRACE_STACK;
 
 // Work distribution of the persistent kernels.  A wave takes chunks of
-// kChunk consecutive rays, its first chunk by its index and the following
-// ones from the global counter (one atomic per chunk), and hands them to its
+// consecutive rays: a first chunk by its index (about half the rays split
+// evenly over the waves), then chunks of kChunk from the global counter (one
+// atomic per chunk), and hands them to its
 // lanes from a wave-uniform cursor; every lane keeps its
 // next ray prefetched in registers, so replacing a finished ray costs no
 // memory round trip.  All members except the prefetch are wave-uniform.
@@ -338,16 +339,20 @@ struct RayFeeder {
         n = count;
         pending = false;
         pid = KP_INVALID;
-        // Wave w's first chunk is rays [w kChunk, (w+1) kChunk) without an
-        // atomic (every wave of the grid would otherwise hit the one counter
-        // address at start); the counter hands out the rays after all first
-        // chunks, and a wave whose first chunk lies beyond the rays has none.
+        // Wave w's first chunk is rays [w F, (w+1) F) without an atomic
+        // (every wave of the grid would otherwise hit the one counter address
+        // at start); the counter hands out the rays after all first chunks,
+        // and a wave whose first chunk lies beyond the rays has none.
         const uint32_t waves = gridDim.x * (blockDim.x >> 6);
         const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        dyn_base = waves * kChunk;
-        next = wave * kChunk;
+        // first chunks hold about half the rays (256 .. 4096 per wave); the
+        // counter's chunks of kChunk balance the rest
+        uint32_t first = (n / (2u * waves)) & ~63u;
+        first = first < kChunk ? kChunk : (first > 4096u ? 4096u : first);
+        dyn_base = waves * first;
+        next = wave * first;
         if (next < n) {
-            end = next + kChunk < n ? next + kChunk : n;
+            end = next + first < n ? next + first : n;
             drained = false;
         } else {
             next = end = 0;
