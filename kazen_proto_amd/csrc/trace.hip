@@ -314,10 +314,10 @@ constexpr int kStack = KP_TRACE_STACK;
 // Work distribution of the persistent kernels.  A wave takes chunks of
 // consecutive rays: a first chunk by its index (about half the rays split
 // evenly over the waves), then chunks of kChunk from the global counter (one
-// atomic per chunk), and hands them to its
-// lanes from a wave-uniform cursor; every lane keeps its
-// next ray prefetched in registers, so replacing a finished ray costs no
-// memory round trip.  All members except the prefetch are wave-uniform.
+// atomic per chunk), and hands them to its lanes from a wave-uniform cursor;
+// every lane keeps its next ray prefetched in registers, so replacing a
+// finished ray costs no memory round trip.  All members except the prefetch
+// are wave-uniform.
 constexpr uint32_t kChunk = 256;
 
 struct RayFeeder {
