@@ -138,3 +138,25 @@ def test_microbench_full_size_parity():
     rays[:, 7] = 0.5
     assert np.array_equal(c.occluded(rays), bvh.occluded(rays, threads=CPU_THREADS).astype(np.uint32))
     c.close()
+
+
+def test_every_ray_traced_once_across_feeder_chunk_boundaries():
+    """The persistent kernels hand each wave a first chunk by its index (about
+    half the rays, at least 256 per wave) and the rest by a counter.  Every
+    ray must be traced exactly once whatever the ray count: a batch of n rays
+    must give the same hits as the first n rays of a larger batch, for counts
+    around one chunk, the end of the first chunks and larger first chunks."""
+    g = golden()
+    c = make_ctx(g["tris"])
+    waves = c.stats()["grid_closest"] * 4
+    rng = np.random.default_rng(77)
+    n_max = 6 * waves * 256 + 5
+    rays = scenes.incoherent_rays(n_max, rng)
+    full = c.intersect_closest(rays)
+    sample = rng.choice(n_max, 2000, replace=False)
+    assert mismatches(full[sample], oracle.closest_bruteforce(g["tris"], rays[sample])) == 0
+    for n in (1, 63, 256, 257, waves * 256 - 1, waves * 256, waves * 256 + 1, 2 * waves * 256 + 7,
+              4 * waves * 256 + 3, n_max - 1):
+        sub = c.intersect_closest(np.ascontiguousarray(rays[:n]))
+        assert mismatches(sub, full[:n]) == 0, "n = %d" % n
+    c.close()
