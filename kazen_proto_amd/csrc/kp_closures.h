@@ -255,6 +255,9 @@ __device__ inline void bsdf_eval(const Material& m, f3 wo, f3 wi, f3& f, float& 
         float denom = dwi + dwo / etap;
         denom = denom * denom;
         float v = D * (1.0f - F) * G * fabsf(dwi * dwo / (denom * ci * co));
+        // radiance BTDF: (n_view / n_light)^2 (Walter 2007 eq. 21 with eta_o on
+        // the camera side); the reference's eval_refraction (bsdfs.h:566-583)
+        // carries the adjoint eta^2 instead
         v /= etap * etap;
         f = mk(v, v, v);
         pdf = pvn * fabsf(dwi) / denom * (1.0f - F);

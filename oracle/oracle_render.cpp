@@ -37,7 +37,15 @@
 //   D18 ray offsetting after Waechter & Binder, Ray Tracing Gems ch. 6, the
 //       method intersection.h:78-91 cites (its implementation has an
 //       operator-precedence defect and is unused);
-//   D19 every closure returns f without the cosine (bsdfs.cpp:31-32 vs 394).
+//   D19 every closure returns f without the cosine (bsdfs.cpp:31-32 vs 394);
+//   D20 the rough-dielectric refraction value is the radiance BTDF: it carries
+//       (n_view / n_light)^2, i.e. Walter 2007 eq. 21 with eta_o on the
+//       camera side.  eval_refraction (bsdfs.h:566-583) multiplies by eta^2
+//       with eta = n_light / n_view, the adjoint (importance) form, which in a
+//       camera path tracer over-weights every refraction into the camera's
+//       medium by eta^2 (2.25 at eta 1.5).  Pinned by the smooth-limit albedo
+//       R + (1 - R)/eta^2 from outside, R + (1 - R) eta^2 from inside, and the
+//       rough albedo written over microfacet normals.
 //   Beckmann Lambda uses the exact erf form rather than the rational fit of
 //   microfacet.h:134-141, so the VNDF pdf equals the sampled density; the
 //   conductor does not clamp D<=4 or f<=1 (bsdfs.cpp:706,716).

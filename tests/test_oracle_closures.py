@@ -2,7 +2,9 @@
 normalisation, the white-furnace tests of the reference
 (tests/render_tests/microfacet_tests.cpp:8-21, src/shading/white_furnace.*),
 chi-square agreement of every sampler with its pdf, reciprocity, energy
-bounds and Fresnel closed forms."""
+bounds, Fresnel closed forms, and the closure values themselves: smooth-limit
+albedos (Schlick for the conductor, R + (1 - R)/eta^2 for the dielectric) and
+rough albedos integrated over microfacet normals."""
 import math
 import zlib
 
@@ -186,3 +188,116 @@ def test_energy_bound(name, theta):
         # throughput from outside is R + T/eta^2 with R + T = 1 up to masking
         # losses, so it lies in [1/eta^2, 1] (less a few percent of masking)
         assert 0.85 / 1.5 ** 2 < est[0] <= 1.0 + 4 * err[0] + 1e-3
+
+
+# ---- closed forms that pin the closure values (not only their sampling) ----
+# Smooth limit: at alpha = 1e-3 masking is negligible, so the directional
+# albedo of a microfacet closure is its Fresnel term.  Rough: the albedo is
+# an expectation over visible normals, written in the microfacet variable m
+# (a change of variables away from the evaluator's wi-space formula).
+
+
+def _albedo_mc(mat, wo, n=400_000, seed=1):
+    rng = np.random.default_rng(seed)
+    ok, wi, f, pdf = oracle.bsdf_sample_batch(mat, wo, rng.random((n, 3), dtype=np.float32))
+    w = np.zeros((n, 3))
+    w[ok] = f[ok] * np.abs(wi[ok, 1:2]) / pdf[ok, None]
+    return w.mean(0)
+
+
+@pytest.mark.parametrize("dist", [GGX, BECKMANN])
+@pytest.mark.parametrize("theta", [20, 50, 75])
+def test_smooth_conductor_albedo_is_schlick(dist, theta):
+    # F = F0 + (1 - F0)(1 - cos)^5 per channel (fresnel.h:185-187); dropping
+    # the Schlick term changes the 75-degree value by up to 52 %
+    t = math.radians(theta)
+    wo = np.array([math.sin(t), math.cos(t), 0.0], np.float32)
+    F0 = np.array([0.9, 0.6, 0.3])
+    mat = oracle.material(CONDUCTOR, dist, F0=tuple(F0), alpha=1e-3)
+    want = F0 + (1.0 - F0) * (1.0 - math.cos(t)) ** 5
+    assert np.allclose(_albedo_mc(mat, wo), want, rtol=5e-3)
+
+
+@pytest.mark.parametrize("dist", [GGX, BECKMANN])
+@pytest.mark.parametrize("theta", [20, 50, 75])
+def test_smooth_dielectric_albedo_radiance_convention(dist, theta):
+    # Radiance crossing into the denser medium is scaled by (n_view/n_light)^2:
+    # from outside R + (1 - R)/eta^2, from inside R + (1 - R) eta^2 below the
+    # critical angle (1 beyond it).  The reference's eval_refraction
+    # (bsdfs.h:566-583) carries the adjoint factor instead, which would give
+    # R + (1 - R) = 1 from either side.
+    eta = 1.5
+    t = math.radians(theta)
+    wo = np.array([math.sin(t), math.cos(t), 0.0], np.float32)
+    mat = oracle.material(DIELECTRIC, dist, alpha=1e-3, eta=eta)
+    R = oracle.fresnel_dielectric(math.cos(t), eta)
+    assert abs(_albedo_mc(mat, wo)[0] - (R + (1 - R) / eta ** 2)) < 3e-3
+    Ri = oracle.fresnel_dielectric(-math.cos(t), eta)
+    assert abs(_albedo_mc(mat, -wo)[0] - (Ri + (1 - Ri) * eta ** 2)) < 5e-3 * (Ri + (1 - Ri) * eta ** 2)
+
+
+def _fresnel_np(c, eta):
+    sin2t = (1 - c * c) / eta ** 2
+    cost = np.sqrt(np.maximum(0.0, 1 - sin2t))
+    rp = (eta * c - cost) / (eta * c + cost)
+    rs = (c - eta * cost) / (c + eta * cost)
+    return np.where(sin2t >= 1, 1.0, 0.5 * (rp * rp + rs * rs)), sin2t, cost
+
+
+def _albedo_m_space(dist, alpha, wo, kind, eta=1.5):
+    """Directional albedo as an integral over microfacet normals m (upper
+    hemisphere grid): E = E_{m ~ VNDF(wo)}[F G2/G1] for reflection plus
+    (n_view/n_light)^2 E[(1 - F) G2/G1] for refraction, using only the
+    oracle's D, Lambda and the Fresnel closed form, which other tests pin."""
+    m, dm = sphere_grid(1200, 2400, upper_only=True)
+    m64 = m.astype(np.float64)
+    D, _ = oracle.mf_eval_batch(dist, alpha, m)
+    s = 1.0 if wo[1] > 0 else -1.0
+    ms = m64 * s
+    c = ms @ wo.astype(np.float64)
+    _, lam_o = oracle.mf_eval_batch(dist, alpha, wo[None, :])
+    vndf = np.maximum(c, 0) * D / abs(float(wo[1])) / (1 + lam_o[0])
+    wr = 2 * c[:, None] * ms - wo
+    _, lam_r = oracle.mf_eval_batch(dist, alpha, wr.astype(np.float32))
+    okr = (c > 0) & (wr[:, 1] * wo[1] > 0)
+    g_r = (1 + lam_o[0]) / (1 + lam_o[0] + lam_r)
+    if kind == "conductor":   # F0 = 1: F = 1
+        return float(np.sum((vndf * g_r * dm)[okr]))
+    e = eta if wo[1] > 0 else 1 / eta
+    F, sin2t, cost = _fresnel_np(c, e)
+    wt = -wo / e + ms * (c / e - cost)[:, None]
+    _, lam_t = oracle.mf_eval_batch(dist, alpha, wt.astype(np.float32))
+    okt = (c > 0) & (sin2t < 1) & (wt[:, 1] * wo[1] < 0)
+    g_t = (1 + lam_o[0]) / (1 + lam_o[0] + lam_t)
+    Er = np.sum((vndf * F * g_r * dm)[okr])
+    Et = np.sum((vndf * (1 - F) * g_t * dm)[okt])
+    return float(Er + Et / e ** 2)
+
+
+def _albedo_wi_space(mat, wo):
+    w, dw = sphere_grid(2000, 4000)
+    f, _ = oracle.bsdf_eval_batch(mat, wo, w)
+    return float(np.sum(f[:, 0] * np.abs(w[:, 1]) * dw))
+
+
+@pytest.mark.parametrize("dist", [GGX, BECKMANN])
+@pytest.mark.parametrize("alpha,theta", [(0.2, 30), (0.5, 70), (0.9, 30)])
+def test_rough_conductor_albedo_m_space(dist, alpha, theta):
+    t = math.radians(theta)
+    wo = np.array([math.sin(t) * 0.6, math.cos(t), math.sin(t) * 0.8], np.float32)
+    mat = oracle.material(CONDUCTOR, dist, F0=(1, 1, 1), alpha=alpha)
+    ref = _albedo_m_space(dist, alpha, wo, "conductor")
+    assert abs(_albedo_wi_space(mat, wo) - ref) < 2e-3 * ref
+
+
+@pytest.mark.parametrize("dist", [GGX, BECKMANN])
+@pytest.mark.parametrize("alpha,theta,inside", [(0.1, 30, False), (0.3, 70, False), (0.6, 30, False),
+                                                (0.3, 25, True), (0.6, 25, True)])
+def test_rough_dielectric_albedo_m_space(dist, alpha, theta, inside):
+    t = math.radians(theta)
+    wo = np.array([math.sin(t) * 0.6, math.cos(t), math.sin(t) * 0.8], np.float32)
+    if inside:
+        wo = -wo
+    mat = oracle.material(DIELECTRIC, dist, alpha=alpha, eta=1.5)
+    ref = _albedo_m_space(dist, alpha, wo, "dielectric")
+    assert abs(_albedo_wi_space(mat, wo) - ref) < 2e-3 * ref
