@@ -94,7 +94,11 @@ typedef struct {
 /* Context ---------------------------------------------------------------- */
 /* Creates a context on HIP device `device`. Replaces the accelerator
  * construction of Scene::Scene / set_accelerator (src/core/scene.cpp:281,
- * src/core/scene.h:60-77). */
+ * src/core/scene.h:60-77). device = KP_DEVICE_HOST makes a host-only
+ * context: scene building, kp_bvh_build and the *_host queries work without
+ * any GPU (no HIP call is made); every call that needs the GPU returns
+ * KP_ERR_STATE. */
+#define KP_DEVICE_HOST (-1)
 int kp_create(kp_ctx** out, int device);
 void kp_destroy(kp_ctx* ctx);
 /* Last error text of the context (or of the last failed kp_create when ctx is
@@ -167,6 +171,18 @@ int kp_intersect_closest(kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hi
  * [tmin, tmax]. Replaces HitableInterface::occluded (src/core/hitable.h:15-33),
  * batched; 4 bytes per result (uint32). */
 int kp_occluded(kp_ctx* ctx, const kp_ray* rays, uint32_t n, uint32_t* out, int ptr_kind);
+/* Host closest hit / occlusion over the same BVH, on the calling thread's
+ * host core: bit-identical to kp_intersect_closest / kp_occluded (the same
+ * node test, instance transform and watertight triangle arithmetic). Safe to
+ * call from any number of threads at once on one context (the context is only
+ * read; kp_bvh_build, kp_geometry_clear and kp_destroy must not run
+ * concurrently). Errors are described by kp_last_error(NULL) of the calling
+ * thread. Replaces the const per-ray HitableInterface::intersect(const Ray&,
+ * Intersection&) / intersect(const Ray&, float&) (src/core/hitable.h:12-13;
+ * EmbreeAccel::intersect, src/core/accel.cpp:348-432), which the reference
+ * calls concurrently from its TBB tile workers (src/core/renderer.cpp:24-26). */
+int kp_intersect_closest_host(const kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hits);
+int kp_occluded_host(const kp_ctx* ctx, const kp_ray* rays, uint32_t n, uint32_t* out);
 /* Instrumented closest hit: same results, plus the totals of BVH8 nodes
  * visited and triangles tested (for the bytes-per-ray roofline figure). */
 int kp_intersect_closest_counted(kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hits,
@@ -185,6 +201,10 @@ int kp_bvh8_blob_info(const kp_bvh8_blob* b, uint64_t* n_nodes, uint64_t* n_tris
 const void* kp_bvh8_blob_nodes(const kp_bvh8_blob* b);
 const void* kp_bvh8_blob_tris(const kp_bvh8_blob* b);
 void kp_bvh8_blob_free(kp_bvh8_blob* b);
+/* Host closest hit / occlusion over a blob (same arithmetic as
+ * kp_intersect_closest_host; thread-safe, no device). Errors: kp_last_error(NULL). */
+int kp_bvh8_blob_closest(const kp_bvh8_blob* b, const kp_ray* rays, uint32_t n, kp_hit* hits);
+int kp_bvh8_blob_occluded(const kp_bvh8_blob* b, const kp_ray* rays, uint32_t n, uint32_t* out);
 
 /* Tuning: refill a wave once fewer than `lanes` (1..64) of its lanes are
  * busy (default 48). */

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkp_hip.so")
 
-SOURCES = ["capi.cpp", "bvh_build.cpp", "trace.hip", "render.hip", "guiding.hip"]
+SOURCES = ["capi.cpp", "bvh_build.cpp", "host_trace.cpp", "host_api.cpp", "trace.hip", "render.hip",
+           "guiding.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KP_OFFLOAD_ARCH", "gfx950")

@@ -2,9 +2,13 @@
 //
 // Mirrors src/core/accel.h:13-41 (Accelerator: add_trianglemesh, add_instances,
 // build, intersect x2) and replaces EmbreeAccel (accel.h:64-88,
-// accel.cpp:185-432).  Per-ray calls from existing callers go to the GPU as a
-// batch of one; new callers should use the batched overloads, which are the
-// fast path (one kernel launch for the whole batch).
+// accel.cpp:185-432).  The per-ray const intersect overloads, which the
+// reference calls concurrently from its TBB tile workers
+// (src/core/renderer.cpp:24-26), traverse the host copy of the same BVH on
+// the calling thread (kp_intersect_closest_host: thread-safe, hits
+// bit-identical to the GPU's); the batched overloads are the fast path (one
+// kernel launch for the whole batch) and must not be called concurrently on
+// one accelerator.
 #pragma once
 #include <functional>
 #include <map>
@@ -151,18 +155,20 @@ public:
         check(kp_bvh_build(ctx_, nullptr));
     }
 
+    // Per-ray closest hit on the calling thread (safe from many threads).
     bool intersect(const Ray& r, Intersection& isect) const override {
-        std::vector<Intersection> out(1);
-        std::vector<uint8_t> hit;
-        intersect({r}, out, hit);
-        if (hit[0]) isect = out[0];
-        return hit[0] != 0;
+        const kp_ray kr = to_kp(r);
+        kp_hit h;
+        check_host(kp_intersect_closest_host(ctx_, &kr, 1, &h));
+        if (h.prim == 0xFFFFFFFFu) return false;
+        fill(r, h, isect);
+        return true;
     }
 
     bool intersect(const Ray& r, float& t) const override {
-        kp_ray kr = to_kp(r);
+        const kp_ray kr = to_kp(r);
         kp_hit h;
-        check(kp_intersect_closest(ctx_, &kr, 1, &h, KP_PTR_HOST));
+        check_host(kp_intersect_closest_host(ctx_, &kr, 1, &h));
         if (h.prim == 0xFFFFFFFFu) return false;
         t = h.t;
         return true;
@@ -202,6 +208,10 @@ public:
 private:
     void check(int rc) const {
         if (rc != KP_OK) throw KpError(rc, kp_last_error(ctx_));
+    }
+    // host queries report through the calling thread's error text
+    static void check_host(int rc) {
+        if (rc != KP_OK) throw KpError(rc, kp_last_error(nullptr));
     }
 
     static void flatten(const TriangleMesh& m, std::vector<float>& xyz, std::vector<uint32_t>& idx) {

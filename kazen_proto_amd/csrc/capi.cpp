@@ -8,8 +8,6 @@
 #include "kp_ctx.h"
 
 namespace {
-thread_local std::string g_create_error;
-
 bool valid_kind(int k) { return k == KP_PTR_HOST || k == KP_PTR_DEVICE; }
 }  // namespace
 
@@ -17,7 +15,7 @@ namespace kp {
 
 int fail(kp_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
-    else g_create_error = msg;
+    else set_thread_error(msg);
     return code;
 }
 
@@ -92,6 +90,7 @@ int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out)
 }
 
 static void free_bvh(kp_ctx* c) {
+    c->host = HostBvh();
     if (c->d_nodes) (void)hipFree(c->d_nodes);
     if (c->d_tris) (void)hipFree(c->d_tris);
     if (c->d_spill) (void)hipFree(c->d_spill);
@@ -280,24 +279,27 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     }
     auto t1 = std::chrono::steady_clock::now();
     *seconds = std::chrono::duration<double>(t1 - t0).count();
-    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->device >= 0) KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_bvh(c);
-    const size_t nb = nodes.size() * sizeof(KpNode8), tbytes = tris.size() * sizeof(float);
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tbytes));
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_insts, rec.size() * sizeof(float)));
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_tlas_inst, top.tri_order.size() * 4));
-    KP_HIP_TRY(c, hipMemcpy(c->d_nodes, nodes.data(), nb, hipMemcpyHostToDevice));
-    KP_HIP_TRY(c, hipMemcpy(c->d_tris, tris.data(), tbytes, hipMemcpyHostToDevice));
-    KP_HIP_TRY(c, hipMemcpy(c->d_insts, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
-    KP_HIP_TRY(c, hipMemcpy(c->d_tlas_inst, top.tri_order.data(), top.tri_order.size() * 4, hipMemcpyHostToDevice));
     // Stack bound: the top-level marker, at most 8 entries
     // per top-level node on the path (instance entries and one group), the
     // return marker, 7 per prototype level.
     // Postponed leaves need no stack entries.
     c->spill_cap = 8u * top.max_depth + 7u * max_unit_depth + 16u;
-    c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
+    if (c->device >= 0) {
+        const size_t nb = nodes.size() * sizeof(KpNode8), tbytes = tris.size() * sizeof(float);
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tbytes));
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_insts, rec.size() * sizeof(float)));
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_tlas_inst, top.tri_order.size() * 4));
+        KP_HIP_TRY(c, hipMemcpy(c->d_nodes, nodes.data(), nb, hipMemcpyHostToDevice));
+        KP_HIP_TRY(c, hipMemcpy(c->d_tris, tris.data(), tbytes, hipMemcpyHostToDevice));
+        KP_HIP_TRY(c, hipMemcpy(c->d_insts, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
+        KP_HIP_TRY(c, hipMemcpy(c->d_tlas_inst, top.tri_order.data(), top.tri_order.size() * 4,
+                                hipMemcpyHostToDevice));
+        c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
+    }
     c->two_level = true;
     c->world_root = has_world ? node_off[0] : KP_INVALID;
     c->n_tlas_nodes = (uint32_t)top.nodes.size();
@@ -311,6 +313,14 @@ static int build_two_level(kp_ctx* c, const BuildOptions& bo, double* seconds) {
     // A traversal visits every node of the meshes' BVH and of the top level
     // once and, per instance entered, each node of its prototype once.
     c->max_steps = (uint32_t)std::min<uint64_t>(steps, 0xFFFFFFF0ull);
+    // the host copy answers the thread-safe per-ray queries
+    c->host.nodes = std::move(nodes);
+    c->host.tris = std::move(tris);
+    c->host.two_level = true;
+    c->host.n_tlas = c->n_tlas_nodes;
+    c->host.world_root = c->world_root;
+    c->host.insts = std::move(rec);
+    c->host.tlas_inst = std::move(top.tri_order);
     c->built = true;
     render_scene_dirty(c);
     return KP_OK;
@@ -323,6 +333,12 @@ extern "C" {
 int kp_create(kp_ctx** out, int device) {
     if (!out) return kp::fail(nullptr, KP_ERR_INVALID, "kp_create: out is NULL");
     *out = nullptr;
+    if (device == KP_DEVICE_HOST) {   // scene building and host queries only
+        kp_ctx* h = new kp_ctx();
+        h->device = KP_DEVICE_HOST;
+        *out = h;
+        return KP_OK;
+    }
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0)
@@ -365,6 +381,13 @@ int kp_create(kp_ctx** out, int device) {
 
 void kp_destroy(kp_ctx* c) {
     if (!c) return;
+    if (c->device < 0) {
+        kp::free_bvh(c);
+        kp::render_state_free(c);
+        kp::guide_state_free(c);
+        delete c;
+        return;
+    }
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     kp::render_state_free(c);
@@ -381,7 +404,7 @@ void kp_destroy(kp_ctx* c) {
 }
 
 const char* kp_last_error(const kp_ctx* c) {
-    return c ? c->err.c_str() : g_create_error.c_str();
+    return c ? c->err.c_str() : kp::thread_error();
 }
 
 int kp_set_stream(kp_ctx* c, void* s) {
@@ -392,7 +415,7 @@ int kp_set_stream(kp_ctx* c, void* s) {
 
 int kp_synchronize(kp_ctx* c) {
     if (!c) return KP_ERR_INVALID;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     uint32_t flags = 0;
     KP_HIP_TRY(c, hipMemcpy(&flags, c->d_words + 1, 4, hipMemcpyDeviceToHost));
@@ -501,8 +524,10 @@ int kp_prim_locate(kp_ctx* c, uint32_t prim, uint32_t* kind, uint32_t* index, ui
 
 int kp_geometry_clear(kp_ctx* c) {
     if (!c) return KP_ERR_INVALID;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
-    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->device >= 0) {
+        KP_USE_DEVICE(c);
+        KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     kp::free_bvh(c);
     c->soup.clear();
     c->mesh_first.clear();
@@ -534,7 +559,7 @@ int kp_instance_get(kp_ctx* c, uint32_t instance_id, uint32_t* blas_id, float* o
 
 int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     if (!c) return KP_ERR_INVALID;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    if (c->device >= 0) KP_USE_DEVICE(c);
     const uint32_t n = (uint32_t)(c->soup.size() / 9);
     kp::BuildOptions bo;
     if (o) {
@@ -576,18 +601,20 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     auto t0 = std::chrono::steady_clock::now();
     if (!kp::build_bvh8(soup, n_all, bo, h, err)) return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: " + err);
     auto t1 = std::chrono::steady_clock::now();
-    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->device >= 0) KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     kp::free_bvh(c);
-    const size_t nb = h.nodes.size() * sizeof(KpNode8), tb = h.tris.size() * sizeof(float);
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tb));
-    KP_HIP_TRY(c, hipMemcpy(c->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice));
-    KP_HIP_TRY(c, hipMemcpy(c->d_tris, h.tris.data(), tb, hipMemcpyHostToDevice));
     // Worst-case stack: 7 pushes per level below the root, plus slack; the
     // top KP_TRACE_STACK entries live in LDS.
     c->spill_cap = 7u * h.max_depth + 8u;
-    c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
-    KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
+    if (c->device >= 0) {
+        const size_t nb = h.nodes.size() * sizeof(KpNode8), tb = h.tris.size() * sizeof(float);
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tb));
+        KP_HIP_TRY(c, hipMemcpy(c->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice));
+        KP_HIP_TRY(c, hipMemcpy(c->d_tris, h.tris.data(), tb, hipMemcpyHostToDevice));
+        c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
+    }
     c->n_tris_built = n_all;
     c->n_tris_stored = n_all;
     c->n_world_prims = n;
@@ -597,6 +624,9 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     // A traversal visits every node and leaf at most once.
     c->max_steps = 2u * (uint32_t)h.nodes.size() + n_all + 64u;
     c->build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    // the host copy answers the thread-safe per-ray queries
+    c->host.nodes = std::move(h.nodes);
+    c->host.tris = std::move(h.tris);
     c->built = true;
     kp::render_scene_dirty(c);
     return KP_OK;
@@ -627,7 +657,7 @@ static int closest_common(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hit
     if (!valid_kind(kind)) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
     if (n == 0) return KP_OK;
     if (!rays || !hits) return kp::fail(c, KP_ERR_INVALID, "NULL rays or hits");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     if (count) KP_HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, 16, c->stream));
     int rc;
     if (kind == KP_PTR_DEVICE) {
@@ -669,7 +699,7 @@ int kp_occluded(kp_ctx* c, const kp_ray* rays, uint32_t n, uint32_t* out, int ki
     if (!valid_kind(kind)) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
     if (n == 0) return KP_OK;
     if (!rays || !out) return kp::fail(c, KP_ERR_INVALID, "NULL rays or out");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     int rc;
     if (kind == KP_PTR_DEVICE) return kp::trace_occluded_dev(c, (const float4*)rays, n, out);
     const size_t rb = (size_t)n * sizeof(kp_ray), ob = (size_t)n * 4;
@@ -682,44 +712,26 @@ int kp_occluded(kp_ctx* c, const kp_ray* rays, uint32_t n, uint32_t* out, int ki
     return kp_synchronize(c);
 }
 
-struct kp_bvh8_blob {
-    kp::Bvh8Host h;
-};
-
-int kp_bvh8_build_host(const float* tris9, uint32_t n, const kp_build_opts* o, kp_bvh8_blob** out) {
-    if (!out || !tris9) return kp::fail(nullptr, KP_ERR_INVALID, "kp_bvh8_build_host: NULL argument");
-    *out = nullptr;
-    kp::BuildOptions bo;
-    if (o) {
-        if (o->leaf_size) bo.leaf_size = o->leaf_size;
-        if (o->sah_bins) bo.sah_bins = o->sah_bins;
-    }
-    kp_bvh8_blob* b = new kp_bvh8_blob();
-    std::string err;
-    if (!kp::build_bvh8(tris9, n, bo, b->h, err)) {
-        delete b;
-        return kp::fail(nullptr, KP_ERR_INVALID, "kp_bvh8_build_host: " + err);
-    }
-    *out = b;
+int kp_intersect_closest_host(const kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hits) {
+    if (!c) return kp::fail(nullptr, KP_ERR_INVALID, "kp_intersect_closest_host: NULL context");
+    if (!c->built) return kp::fail(nullptr, KP_ERR_STATE, "kp_intersect_closest_host before kp_bvh_build");
+    if (n && (!rays || !hits)) return kp::fail(nullptr, KP_ERR_INVALID, "kp_intersect_closest_host: NULL rays or hits");
+    kp::host_closest(c->host, (const float*)rays, n, (float*)hits);
     return KP_OK;
 }
 
-int kp_bvh8_blob_info(const kp_bvh8_blob* b, uint64_t* nn, uint64_t* nt, uint32_t* md) {
-    if (!b) return KP_ERR_INVALID;
-    if (nn) *nn = b->h.nodes.size();
-    if (nt) *nt = b->h.tri_order.size();
-    if (md) *md = b->h.max_depth;
+int kp_occluded_host(const kp_ctx* c, const kp_ray* rays, uint32_t n, uint32_t* out) {
+    if (!c) return kp::fail(nullptr, KP_ERR_INVALID, "kp_occluded_host: NULL context");
+    if (!c->built) return kp::fail(nullptr, KP_ERR_STATE, "kp_occluded_host before kp_bvh_build");
+    if (n && (!rays || !out)) return kp::fail(nullptr, KP_ERR_INVALID, "kp_occluded_host: NULL rays or out");
+    kp::host_occluded(c->host, (const float*)rays, n, out);
     return KP_OK;
 }
-
-const void* kp_bvh8_blob_nodes(const kp_bvh8_blob* b) { return b ? b->h.nodes.data() : nullptr; }
-const void* kp_bvh8_blob_tris(const kp_bvh8_blob* b) { return b ? b->h.tris.data() : nullptr; }
-void kp_bvh8_blob_free(kp_bvh8_blob* b) { delete b; }
 
 int kp_device_alloc(kp_ctx* c, uint64_t bytes, void** ptr) {
     if (!c || !ptr) return KP_ERR_INVALID;
     *ptr = nullptr;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, hipMalloc(ptr, bytes ? bytes : 1));
     return KP_OK;
 }
@@ -727,7 +739,7 @@ int kp_device_alloc(kp_ctx* c, uint64_t bytes, void** ptr) {
 int kp_device_free(kp_ctx* c, void* ptr) {
     if (!c) return KP_ERR_INVALID;
     if (!ptr) return KP_OK;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     KP_HIP_TRY(c, hipFree(ptr));
     return KP_OK;
@@ -736,7 +748,7 @@ int kp_device_free(kp_ctx* c, void* ptr) {
 int kp_copy_to_device(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
     if (!c || (!dst && bytes) || (!src && bytes)) return KP_ERR_INVALID;
     if (!bytes) return KP_OK;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     return KP_OK;
@@ -745,7 +757,7 @@ int kp_copy_to_device(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
 int kp_copy_to_host(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
     if (!c || (!dst && bytes) || (!src && bytes)) return KP_ERR_INVALID;
     if (!bytes) return KP_OK;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     return KP_OK;
@@ -755,14 +767,14 @@ int kp_device_copy(kp_ctx* c, void* dst, const void* src, uint64_t bytes) {
     if (!c || (bytes && (!dst || !src))) return KP_ERR_INVALID;
     if (bytes % 16) return kp::fail(c, KP_ERR_INVALID, "kp_device_copy: bytes must be a multiple of 16");
     if (!bytes) return KP_OK;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, kp::launch_copy16(dst, src, bytes / 16, c->cus, c->stream));
     return KP_OK;
 }
 
 int kp_timer_begin(kp_ctx* c) {
     if (!c) return KP_ERR_INVALID;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     if (!c->ev_begin) KP_HIP_TRY(c, hipEventCreate(&c->ev_begin));
     if (!c->ev_end) KP_HIP_TRY(c, hipEventCreate(&c->ev_end));
     KP_HIP_TRY(c, hipEventRecord(c->ev_begin, c->stream));

@@ -601,7 +601,7 @@ static int guide_eval_common(kp_ctx* c, const float* in6, uint32_t n, float* out
     int rc = guide_ready(c);
     if (rc) return rc;
     if (n == 0) return KP_OK;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     float *d_in = nullptr, *d_out = nullptr, *d_dirs = nullptr, *d_u = nullptr, *d_do = nullptr, *d_pdf = nullptr;
     auto cleanup = [&]() {
         for (float* p : {d_in, d_out, d_dirs, d_u, d_do, d_pdf})
@@ -638,7 +638,7 @@ extern "C" {
 
 int kp_guiding_init(kp_ctx* c, uint64_t seed) {
     if (!c) return KP_ERR_INVALID;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     int rc = kp::guide_alloc(c);
     if (rc) return rc;
     // Glorot-uniform weights from a PCG32 stream, zero biases; the output
@@ -667,7 +667,7 @@ int kp_guiding_init(kp_ctx* c, uint64_t seed) {
 int kp_guiding_set_params(kp_ctx* c, const float* params, uint32_t n) {
     if (!c || !params) return KP_ERR_INVALID;
     if (n != KP_G_PARAMS) return kp::fail(c, KP_ERR_INVALID, "guiding parameter count must be 17696");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     int rc = kp::guide_alloc(c);
     if (rc) return rc;
     return kp::guide_set_params(c, params);
@@ -705,7 +705,7 @@ int kp_guiding_train_step(kp_ctx* c, const float* in6, const float* dirs, const 
     if (rc) return rc;
     GuideState* g = c->guide;
     if (n == 0 || n > g->max_batch) return kp::fail(c, KP_ERR_INVALID, "batch must hold 1..262144 samples");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, hipMemcpyAsync(g->in6, in6, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
     KP_HIP_TRY(c, hipMemcpyAsync(g->dirs, dirs, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
     KP_HIP_TRY(c, hipMemcpyAsync(g->weights, weights, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));

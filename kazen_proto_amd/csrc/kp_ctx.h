@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "host_trace.h"
 #include "kp_bvh.h"
 #include "kp_kernels.h"
 
@@ -42,6 +43,7 @@ struct kp_ctx {
     uint32_t n_tris_stored = 0;      // triangles stored on the device
     uint32_t n_nodes = 0, n_bvh2 = 0, max_depth = 0;
     double build_seconds = 0.0;
+    kp::HostBvh host;                // host copy of the device BVH (per-ray queries on host cores)
     uint4* d_nodes = nullptr;
     float4* d_tris = nullptr;
     uint2* d_spill = nullptr;
@@ -70,6 +72,9 @@ struct kp_ctx {
 };
 
 namespace kp {
+// Per-thread error text (host_api.cpp): kp_last_error(NULL).
+void set_thread_error(const std::string& msg);
+const char* thread_error();
 // Records a HIP failure on the context and returns KP_ERR_HIP.
 int hip_fail(kp_ctx* c, hipError_t e, const char* what);
 int fail(kp_ctx* c, int code, const std::string& msg);
@@ -91,6 +96,15 @@ void guide_state_free(kp_ctx* c);
 // Marks the integrator's device copy of the scene stale (meshes changed).
 void render_scene_dirty(kp_ctx* c);
 }  // namespace kp
+
+// Makes the context's device current; host-only contexts (KP_DEVICE_HOST)
+// fail every call that needs the GPU.
+#define KP_USE_DEVICE(ctx)                                                                           \
+    do {                                                                                             \
+        if ((ctx)->device < 0)                                                                       \
+            return kp::fail((ctx), KP_ERR_STATE, "this call needs a HIP device; the context is host-only"); \
+        KP_HIP_TRY((ctx), hipSetDevice((ctx)->device));                                              \
+    } while (0)
 
 #define KP_HIP_TRY(ctx, expr)                                  \
     do {                                                       \

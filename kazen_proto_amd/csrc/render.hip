@@ -1409,7 +1409,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
         ncdf += r->n_cdf;
         depth = std::max(depth, s->max_depth);
     }
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     if (nt >= (1ull << 28) || nn >= (1ull << 24))
         return fail(c, KP_ERR_INVALID, "kp_batch_create: more than 2^28 triangles or 2^24 BVH8 nodes in the batch");
     BatchState* b = new BatchState();
@@ -1780,7 +1780,7 @@ int kp_instance_set_material(kp_ctx* c, uint32_t instance_id, uint32_t material_
 
 int kp_release_buffers(kp_ctx* c) {
     if (!c) return KP_ERR_INVALID;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->render && c->render->path_mem) {
         KP_HIP_TRY(c, hipFree(c->render->path_mem));
@@ -1806,14 +1806,14 @@ int kp_blas_set_material(kp_ctx* c, uint32_t blas_id, uint32_t material_id) {
 int kp_film_reset(kp_ctx* c, uint32_t w, uint32_t h) {
     if (!c) return KP_ERR_INVALID;
     if (w == 0 || h == 0 || (uint64_t)w * h > (1u << 26)) return kp::fail(c, KP_ERR_INVALID, "bad film size");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     return kp::film_alloc(c, w, h);
 }
 
 int kp_render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     if (!c || !d) return KP_ERR_INVALID;
     if (d->spp == 0) return KP_OK;
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     return kp::render_accumulate(c, d);
 }
 
@@ -1822,7 +1822,7 @@ int kp_film_read(kp_ctx* c, float* out, int kind) {
     RenderState* r = kp::rstate(c);
     if (!r->film_sum) return kp::fail(c, KP_ERR_STATE, "no film (kp_film_reset first)");
     if (kind != KP_PTR_HOST && kind != KP_PTR_DEVICE) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     const uint32_t npix = r->fw * r->fh;
     const float inv = r->film_count ? 1.0f / (float)r->film_count : 0.0f;
     float* dst = kind == KP_PTR_DEVICE ? out : r->film_out;
@@ -1848,7 +1848,7 @@ int kp_integrate_rays(kp_ctx* c, const kp_ray* rays, uint32_t n, const kp_render
     if (kind != KP_PTR_HOST && kind != KP_PTR_DEVICE) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
     if (n == 0) return KP_OK;
     if (!rays || !rgb) return kp::fail(c, KP_ERR_INVALID, "NULL rays or output");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     return kp::integrate_rays(c, rays, n, d, rgb, kind);
 }
 
@@ -1863,7 +1863,7 @@ int kp_batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, kp_batch** out
     if (!c || !out || (n && !scenes)) return KP_ERR_INVALID;
     *out = nullptr;
     if (n == 0 || n > 4096) return kp::fail(c, KP_ERR_INVALID, "kp_batch_create: 1..4096 scenes");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     kp::BatchState* b = nullptr;
     int rc = kp::batch_create(c, scenes, n, &b);
     if (rc) return rc;
@@ -1916,7 +1916,7 @@ int kp_batch_film_read(kp_batch* h, uint32_t scene, float* out, int kind) {
     kp_ctx* c = b->c;
     if (scene >= b->fw.size()) return kp::fail(c, KP_ERR_INVALID, "kp_batch_film_read: no such scene in the last render");
     if (kind != KP_PTR_HOST && kind != KP_PTR_DEVICE) return kp::fail(c, KP_ERR_INVALID, "bad pointer kind");
-    KP_HIP_TRY(c, hipSetDevice(c->device));
+    KP_USE_DEVICE(c);
     const uint32_t npix = b->fw[scene] * b->fh[scene], f0 = b->film_first[scene];
     const float inv = 1.0f / (float)b->film_spp[scene];
     float* dst = kind == KP_PTR_DEVICE ? out : b->film_out + 3 * (size_t)f0;

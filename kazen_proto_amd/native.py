@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libkp_hip.so")
 KP_OK = 0
 KP_PTR_HOST = 0
 KP_PTR_DEVICE = 1
+DEVICE_HOST = -1   # host-only context: scene building and host queries, no GPU
 
 _p = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -145,6 +146,10 @@ SIGNATURES = {
     "kp_bvh8_blob_nodes": (_p, [_p]),
     "kp_bvh8_blob_tris": (_p, [_p]),
     "kp_bvh8_blob_free": (None, [_p]),
+    "kp_bvh8_blob_closest": (_i32, [_p, _p, _u32, _p]),
+    "kp_bvh8_blob_occluded": (_i32, [_p, _p, _u32, _p]),
+    "kp_intersect_closest_host": (_i32, [_p, _p, _u32, _p]),
+    "kp_occluded_host": (_i32, [_p, _p, _u32, _p]),
     "kp_set_trace_tuning": (_i32, [_p, _u32]),
     "kp_set_pass_paths": (_i32, [_p, _u32]),
     "kp_release_buffers": (_i32, [_p]),
@@ -270,8 +275,48 @@ def bvh8_build_host(tris9, leaf_size=4, sah_bins=32):
         lib().kp_bvh8_blob_free(blob)
 
 
+class HostBvh8:
+    """Host-only BVH8 (kp_bvh8_build_host) with host closest-hit / occlusion
+    queries (no device, thread-safe)."""
+
+    def __init__(self, tris9, leaf_size=4, sah_bins=32):
+        tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+        o = BuildOpts()
+        o.leaf_size = leaf_size
+        o.sah_bins = sah_bins
+        self._h = _p()
+        rc = lib().kp_bvh8_build_host(_np_ptr(tris9), tris9.shape[0], ctypes.byref(o), ctypes.byref(self._h))
+        if rc != KP_OK:
+            raise KpError(rc, lib().kp_last_error(None).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kp_bvh8_blob_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def closest(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 4), np.float32)
+        rc = lib().kp_bvh8_blob_closest(self._h, _np_ptr(rays), rays.shape[0], _np_ptr(out))
+        if rc != KP_OK:
+            raise KpError(rc, lib().kp_last_error(None).decode())
+        return out
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint32)
+        rc = lib().kp_bvh8_blob_occluded(self._h, _np_ptr(rays), rays.shape[0], _np_ptr(out))
+        if rc != KP_OK:
+            raise KpError(rc, lib().kp_last_error(None).decode())
+        return out
+
+
 class Context:
-    """One HIP device context: scene, BVH and queries (one stream)."""
+    """One HIP device context: scene, BVH and queries (one stream).
+    device=DEVICE_HOST makes a host-only context (no GPU needed)."""
 
     def __init__(self, device=0):
         self._h = _p()
@@ -408,6 +453,24 @@ class Context:
                                                            ctypes.byref(nv), ctypes.byref(nt)))
             return out, int(nv.value), int(nt.value)
         self._check(lib().kp_intersect_closest(self._h, rp, n, hp, kind))
+        return out
+
+    def intersect_closest_host(self, rays):
+        """Closest hit on host cores (kp_intersect_closest_host): [n,4] hits,
+        bit-identical to intersect_closest."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 4), np.float32)
+        rc = lib().kp_intersect_closest_host(self._h, _np_ptr(rays), rays.shape[0], _np_ptr(out))
+        if rc != KP_OK:
+            raise KpError(rc, lib().kp_last_error(None).decode())
+        return out
+
+    def occluded_host(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint32)
+        rc = lib().kp_occluded_host(self._h, _np_ptr(rays), rays.shape[0], _np_ptr(out))
+        if rc != KP_OK:
+            raise KpError(rc, lib().kp_last_error(None).decode())
         return out
 
     def occluded(self, rays, out=None):

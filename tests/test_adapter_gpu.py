@@ -76,3 +76,8 @@ def test_adapter_program(tmp_path):
         g, tw, yw = want[int(i)]
         assert got == "1" and int(geom) == g and int(prim) in (0, 1)
         assert abs(float(t) - tw) < 1e-5 and abs(float(py) - yw) < 1e-5
+
+    # 16 threads calling the per-ray const intersect at once (host traversal)
+    # agree bit for bit with the batched GPU intersect of the same accelerator
+    n, n_hit, bad = (int(x) for x in out["threads"][0])
+    assert n == 20000 and n_hit > 5000 and bad == 0
