@@ -119,6 +119,15 @@ int kp_synchronize(kp_ctx* ctx);
  * Scene::add_mesh (src/core/scene.h:144-160). */
 int kp_mesh_add(kp_ctx* ctx, const float* xyz, uint32_t n_verts, const uint32_t* idx,
                 uint32_t n_tris, const float* xform12, uint32_t* mesh_id);
+/* Vertex normals of a mesh for shading: 9 floats per triangle (the normals
+ * at its three vertices, in the mesh's triangle order and object space; the
+ * mesh's transform is applied as its inverse transpose, then they are
+ * normalised). The integrator then shades with the barycentric interpolation
+ * of the normals and moves the shading point off the flat triangle (Hanika
+ * 2021), as TriangleMesh::post_hit does (src/core/shape.cpp:615-654); the
+ * geometric normal still decides which side a ray leaves from. Meshes without
+ * normals shade with their geometric normal. */
+int kp_mesh_set_normals(kp_ctx* ctx, uint32_t mesh_id, const float* normals9);
 /* Builds the BVH over every mesh added so far and uploads it. Replaces
  * Accelerator::build (src/core/accel.h:32; EmbreeAccel::build accel.h:75-77,
  * add_instances accel.cpp:317-346) and Scene::build_bvh (scene.h:162-164). */
@@ -141,6 +150,9 @@ int kp_get_stats(kp_ctx* ctx, kp_stats* out);
 int kp_blas_add(kp_ctx* ctx, const float* xyz, uint32_t n_verts, const uint32_t* idx, uint32_t n_tris,
                 uint32_t* blas_id);
 int kp_blas_set_material(kp_ctx* ctx, uint32_t blas_id, uint32_t material_id);
+/* Prototype vertex normals (object space, 9 floats per triangle; each
+ * instance applies the inverse transpose of its placement). */
+int kp_blas_set_normals(kp_ctx* ctx, uint32_t blas_id, const float* normals9);
 int kp_instance_add(kp_ctx* ctx, uint32_t blas_id, const float* xform12, uint32_t* instance_id);
 /* Material of one instance, overriding its prototype's
  * (kp_blas_set_material) for the integrator. */
@@ -183,6 +195,21 @@ int kp_occluded(kp_ctx* ctx, const kp_ray* rays, uint32_t n, uint32_t* out, int 
  * calls concurrently from its TBB tile workers (src/core/renderer.cpp:24-26). */
 int kp_intersect_closest_host(const kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hits);
 int kp_occluded_host(const kp_ctx* ctx, const kp_ray* rays, uint32_t n, uint32_t* out);
+/* kp_intersect_closest for device pointers that allocates nothing: no
+ * hipMalloc, hipFree or allocating library call during the call. No
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it). The
+ * traversal's work counter, error word and stack spill area live in a static
+ * 64 MiB device array of the library (one per device, held until the process
+ * ends) instead of the context's buffers; results are bit-identical to
+ * kp_intersect_closest. rays / hits must be device memory; the call is
+ * asynchronous on the context stream and kernel-detected errors are reported
+ * by the next kp_synchronize. A BVH whose traversal stack does not fit the
+ * scratch for one block of 256 lanes is rejected with KP_ERR_INVALID (deeper
+ * BVHs run on fewer blocks). Every call on one device shares that device's
+ * scratch: calls that overlap in time on one device (any context, any
+ * stream) race and must be serialised by the caller. */
+int kp_intersect_closest_noalloc(kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hits);
 /* Instrumented closest hit: same results, plus the totals of BVH8 nodes
  * visited and triangles tested (for the bytes-per-ray roofline figure). */
 int kp_intersect_closest_counted(kp_ctx* ctx, const kp_ray* rays, uint32_t n, kp_hit* hits,
@@ -226,6 +253,8 @@ int kp_release_buffers(kp_ctx* ctx);
 #define KP_MAT_CONDUCTOR 1    /* microfacet reflection, Schlick F0
                                  (KpPrincipleSpecularReflection, bsdfs.cpp:688-730) */
 #define KP_MAT_DIELECTRIC 2   /* rough dielectric (KpGlass, bsdfs.cpp:441-554) */
+#define KP_MAT_GLOSS 3        /* microfacet reflection with dielectric Fresnel F(eta, m.wo)
+                                 (KpGloss, bsdfs.cpp:359-439; OSL microfacet(..., refract = 0)) */
 #define KP_DIST_GGX 0
 #define KP_DIST_BECKMANN 1
 
@@ -234,17 +263,29 @@ int kp_release_buffers(kp_ctx* ctx);
  * process_closure, src/shading/bsdf.cpp:184-328). Le != 0 makes every mesh
  * using the material a one-sided area light (GeometryLight,
  * src/core/light.cpp:52-85), emitting on the side its winding normal
- * (v1-v0)x(v2-v0) points to. */
+ * (v1-v0)x(v2-v0) points to. Microfacet roughness is anisotropic as
+ * MicrofacetParams' xalpha / yalpha (src/shading/bsdfs.h:41-46,
+ * microfacet.h:60-93): alpha along the shading frame's tangent, alpha_y
+ * along its bitangent (the frame of src/base/utils.h:183-210). */
 typedef struct {
     int32_t type;        /* KP_MAT_* */
     int32_t dist;        /* KP_DIST_* (microfacet closures) */
     float albedo[3];     /* Lambert reflectance */
     float F0[3];         /* conductor normal-incidence reflectance */
-    float alpha;         /* microfacet roughness (> 0) */
-    float eta;           /* dielectric n_inside / n_outside */
+    float alpha;         /* microfacet roughness along the tangent (> 0) */
+    float eta;           /* dielectric and gloss: n_inside / n_outside */
     float Le[3];         /* emitted radiance */
-    float pad;
+    float alpha_y;       /* roughness along the bitangent; 0 = alpha (isotropic) */
 } kp_material;
+
+/* One lobe of a composite material: the closure weight (the OSL closure
+ * colour) and the lobe's parameters (its Le is ignored). */
+#define KP_MAX_LOBES 8
+typedef struct {
+    float weight[3];
+    float pad;
+    kp_material lobe;
+} kp_lobe;
 
 /* One render: width x height pixels, spp samples per pixel starting at sample
  * index spp_offset (so renders resume), paths of up to max_depth scattering
@@ -277,6 +318,15 @@ typedef struct {
 } kp_render_stats;
 
 int kp_material_add(kp_ctx* ctx, const kp_material* m, uint32_t* material_id);
+/* A material of 1..8 weighted lobes emitting Le (may be NULL): replaces the
+ * CompositeClosure an OSL shader group builds (src/shading/bsdf.h:132-196,
+ * bsdf.cpp:17-74). f = sum_i weight_i f_i; at a vertex with path throughput
+ * beta, lobe i is sampled with probability p_i = dot(weight_i, beta) /
+ * sum_j dot(weight_j, beta) (the reference's compute_pdfs, bsdf.h:157-176;
+ * its sampler picks uniformly instead) and the density is sum_i p_i pdf_i.
+ * kp_material_add is the one-lobe case with weight 1. */
+int kp_material_add_composite(kp_ctx* ctx, const kp_lobe* lobes, uint32_t n_lobes, const float* Le,
+                              uint32_t* material_id);
 /* Assigns a material to a mesh (default 0). */
 int kp_mesh_set_material(kp_ctx* ctx, uint32_t mesh_id, uint32_t material_id);
 /* Film: FP32 RGB sum + Kahan compensation per pixel, zeroed here. */

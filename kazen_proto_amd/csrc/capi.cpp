@@ -108,7 +108,7 @@ static void free_bvh(kp_ctx* c) {
 }
 
 // Row-major 3x4 affine inverse in double; false if singular or non-finite.
-static bool affine_inverse(const float* m, double* inv) {
+bool affine_inverse(const float* m, double* inv) {
     double a[9];
     for (int r = 0; r < 3; ++r)
         for (int k = 0; k < 3; ++k) a[3 * r + k] = m[4 * r + k];
@@ -371,6 +371,9 @@ int kp_create(kp_ctx** out, int device) {
     if ((e = kp::trace_occupancy_five(&b5c, &b5o)) != hipSuccess) return bail(e, "occupancy");
     c->grid_closest5 = (uint32_t)(c->cus * (b5c > 0 ? b5c : 1));
     c->grid_occluded5 = (uint32_t)(c->cus * (b5o > 0 ? b5o : 1));
+    void* scr = nullptr;
+    if ((e = kp::noalloc_scratch(device, &scr)) != hipSuccess) return bail(e, "scratch symbol");
+    c->d_scratch = (unsigned char*)scr;
     int sp = 0, sg = 0;
     if ((e = kp::shade_occupancy(&sp, &sg)) != hipSuccess) return bail(e, "occupancy");
     c->shade_blocks = (uint32_t)(c->cus * (sp > 0 ? sp : 1));
@@ -419,6 +422,13 @@ int kp_synchronize(kp_ctx* c) {
     KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
     uint32_t flags = 0;
     KP_HIP_TRY(c, hipMemcpy(&flags, c->d_words + 1, 4, hipMemcpyDeviceToHost));
+    if (c->noalloc_used) {   // errors of kp_intersect_closest_noalloc launches
+        uint32_t f2 = 0;
+        uint32_t* ew = (uint32_t*)(c->d_scratch + KP_SCRATCH_ERR_OFFSET);
+        KP_HIP_TRY(c, hipMemcpy(&f2, ew, 4, hipMemcpyDeviceToHost));
+        if (f2) KP_HIP_TRY(c, kp::launch_reset_u32(ew, 1u, c->stream));
+        flags |= f2;
+    }
     if (flags) {
         KP_HIP_TRY(c, hipMemsetAsync(c->d_words + 1, 0, 4, c->stream));
         KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -456,7 +466,68 @@ int kp_mesh_add(kp_ctx* c, const float* xyz, uint32_t n_verts, const uint32_t* i
     c->mesh_first.push_back((uint32_t)(base / 9));
     c->mesh_count.push_back(n_tris);
     c->mesh_material.push_back(0);
+    static const float eye[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    c->mesh_xf.insert(c->mesh_xf.end(), xf ? xf : eye, (xf ? xf : eye) + 12);
+    if (!c->soup_has_n.empty()) {
+        c->soup_has_n.resize(c->soup.size() / 9, 0);
+        c->soup_n.resize(c->soup.size(), 0.0f);
+    }
+    kp::render_scene_dirty(c);
     if (mesh_id) *mesh_id = id;
+    return KP_OK;
+}
+
+// Unit vertex normals of n triangles (9 floats each) under the inverse
+// transpose of the row-major 3x4 matrix xf (NULL: as given).
+static int normals_to(kp_ctx* c, const float* n9, uint32_t n, const float* xf, float* out, const char* who) {
+    double inv[12];
+    if (xf && !kp::affine_inverse(xf, inv))
+        return kp::fail(c, KP_ERR_INVALID, std::string(who) + ": mesh transform not invertible");
+    for (size_t i = 0; i < 3 * (size_t)n; ++i) {
+        const float* v = n9 + 3 * i;
+        double w[3], len = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            // (M^-1)^T n: column a of the inverse's linear part
+            w[a] = xf ? inv[a] * v[0] + inv[4 + a] * v[1] + inv[8 + a] * v[2] : (double)v[a];
+            len += w[a] * w[a];
+        }
+        len = std::sqrt(len);
+        if (!(len > 0.0) || !std::isfinite(len))
+            return kp::fail(c, KP_ERR_INVALID, std::string(who) + ": zero or non-finite normal");
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = (float)(w[a] / len);
+    }
+    return KP_OK;
+}
+
+int kp_mesh_set_normals(kp_ctx* c, uint32_t mesh_id, const float* normals9) {
+    if (!c) return KP_ERR_INVALID;
+    if (mesh_id >= c->mesh_first.size()) return kp::fail(c, KP_ERR_INVALID, "kp_mesh_set_normals: bad mesh id");
+    if (!normals9) return kp::fail(c, KP_ERR_INVALID, "kp_mesh_set_normals: NULL normals");
+    const uint32_t first = c->mesh_first[mesh_id], n = c->mesh_count[mesh_id];
+    std::vector<float> w(9 * (size_t)n);
+    int rc = normals_to(c, normals9, n, &c->mesh_xf[12 * (size_t)mesh_id], w.data(), "kp_mesh_set_normals");
+    if (rc) return rc;
+    c->soup_has_n.resize(c->soup.size() / 9, 0);
+    c->soup_n.resize(c->soup.size(), 0.0f);
+    std::memcpy(&c->soup_n[9 * (size_t)first], w.data(), w.size() * sizeof(float));
+    std::memset(&c->soup_has_n[first], 1, n);
+    kp::render_scene_dirty(c);
+    return KP_OK;
+}
+
+int kp_blas_set_normals(kp_ctx* c, uint32_t blas_id, const float* normals9) {
+    if (!c) return KP_ERR_INVALID;
+    if (blas_id >= c->blas_first.size()) return kp::fail(c, KP_ERR_INVALID, "kp_blas_set_normals: bad prototype id");
+    if (!normals9) return kp::fail(c, KP_ERR_INVALID, "kp_blas_set_normals: NULL normals");
+    const uint32_t first = c->blas_first[blas_id], n = c->blas_count[blas_id];
+    std::vector<float> w(9 * (size_t)n);
+    int rc = normals_to(c, normals9, n, nullptr, w.data(), "kp_blas_set_normals");
+    if (rc) return rc;
+    c->blas_has_n.resize(c->blas_soup.size() / 9, 0);
+    c->blas_n.resize(c->blas_soup.size(), 0.0f);
+    std::memcpy(&c->blas_n[9 * (size_t)first], w.data(), w.size() * sizeof(float));
+    std::memset(&c->blas_has_n[first], 1, n);
+    kp::render_scene_dirty(c);
     return KP_OK;
 }
 
@@ -479,6 +550,10 @@ int kp_blas_add(kp_ctx* c, const float* xyz, uint32_t n_verts, const uint32_t* i
     c->blas_first.push_back((uint32_t)(base / 9));
     c->blas_count.push_back(n_tris);
     c->blas_material.push_back(0);
+    if (!c->blas_has_n.empty()) {
+        c->blas_has_n.resize(c->blas_soup.size() / 9, 0);
+        c->blas_n.resize(c->blas_soup.size(), 0.0f);
+    }
     if (blas_id) *blas_id = id;
     return KP_OK;
 }
@@ -533,6 +608,11 @@ int kp_geometry_clear(kp_ctx* c) {
     c->mesh_first.clear();
     c->mesh_count.clear();
     c->mesh_material.clear();
+    c->mesh_xf.clear();
+    c->soup_n.clear();
+    c->soup_has_n.clear();
+    c->blas_n.clear();
+    c->blas_has_n.clear();
     c->blas_soup.clear();
     c->blas_first.clear();
     c->blas_count.clear();
@@ -686,6 +766,38 @@ static int closest_common(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hit
 
 int kp_intersect_closest(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hits, int kind) {
     return closest_common(c, rays, n, hits, kind, false, nullptr, nullptr);
+}
+
+int kp_intersect_closest_noalloc(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hits) {
+    if (!c) return KP_ERR_INVALID;
+    if (!c->built) return kp::fail(c, KP_ERR_STATE, "kp_intersect_closest_noalloc before kp_bvh_build");
+    if (n == 0) return KP_OK;
+    if (!rays || !hits) return kp::fail(c, KP_ERR_INVALID, "kp_intersect_closest_noalloc: NULL rays or hits");
+    KP_USE_DEVICE(c);
+    if (!c->d_scratch) return kp::fail(c, KP_ERR_STATE, "kp_intersect_closest_noalloc: no scratch on this device");
+    // one stack spill slice of spill_cap entries per lane of the grid; fewer
+    // blocks than the regular launch when the BVH is deep
+    const uint64_t block_bytes = (uint64_t)KP_TRACE_BLOCK * c->spill_cap * sizeof(uint2);
+    const uint64_t fit = (KP_SCRATCH_BYTES - KP_SCRATCH_SPILL_OFFSET) / block_bytes;
+    if (fit == 0)
+        return kp::fail(c, KP_ERR_INVALID,
+                        "kp_intersect_closest_noalloc: the BVH needs " + std::to_string(block_bytes) +
+                            " bytes of traversal stack per block; the scratch holds " +
+                            std::to_string(KP_SCRATCH_BYTES - KP_SCRATCH_SPILL_OFFSET));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(c->grid_closest, fit);
+    kp::TraceArgs a = kp::trace_base(c);
+    a.rays = (const float4*)rays;
+    a.hits = (float4*)hits;
+    a.n = n;
+    a.counter = (uint32_t*)c->d_scratch;
+    a.err = (uint32_t*)(c->d_scratch + KP_SCRATCH_ERR_OFFSET);
+    a.spill = (uint2*)(c->d_scratch + KP_SCRATCH_SPILL_OFFSET);
+    a.spill_blocks = grid;
+    c->noalloc_used = true;
+    KP_HIP_TRY(c, kp::launch_reset_u32(a.err, 1u, c->stream));   // launch_closest clears the counter
+    hipError_t e = kp::launch_closest(a, grid, false, c->stream);
+    if (e != hipSuccess) return kp::hip_fail(c, e, "kp_intersect_closest_noalloc launch");
+    return KP_OK;
 }
 
 int kp_intersect_closest_counted(kp_ctx* c, const kp_ray* rays, uint32_t n, kp_hit* hits, int kind,

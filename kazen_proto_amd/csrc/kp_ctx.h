@@ -22,6 +22,12 @@ struct kp_ctx {
     std::vector<uint32_t> mesh_first; // first global primitive of each mesh
     std::vector<uint32_t> mesh_count;
     std::vector<uint32_t> mesh_material;
+    std::vector<float> mesh_xf;       // 12 per mesh: the object-to-world rows kp_mesh_add applied
+    // shading normals (kp_mesh_set_normals / kp_blas_set_normals): 9 floats
+    // per triangle (world space for meshes, object space for prototypes) and
+    // a per-triangle flag; empty while no mesh has normals
+    std::vector<float> soup_n, blas_n;
+    std::vector<uint8_t> soup_has_n, blas_has_n;
 
     // instancing: prototype meshes (object space) and their placements
     std::vector<float> blas_soup;
@@ -58,6 +64,9 @@ struct kp_ctx {
 
     // small device words: [0] work counter, [1] error flags
     uint32_t* d_words = nullptr;
+    // this device's static scratch (kp_intersect_closest_noalloc), looked up at kp_create
+    unsigned char* d_scratch = nullptr;
+    bool noalloc_used = false;       // kp_synchronize also reads the scratch's error word
     unsigned long long* d_counters = nullptr;
 
     // staging for host-pointer calls
@@ -72,6 +81,8 @@ struct kp_ctx {
 };
 
 namespace kp {
+// Row-major 3x4 affine inverse in double; false if singular or non-finite.
+bool affine_inverse(const float* m, double* inv);
 // Per-thread error text (host_api.cpp): kp_last_error(NULL).
 void set_thread_error(const std::string& msg);
 const char* thread_error();

@@ -7,6 +7,13 @@
 #define KP_TRACE_BLOCK 256
 #define KP_TRACE_STACK 8
 
+// Static device scratch of the allocation-free closest-hit entry point:
+// 64 MiB per device (counter, error word, stack spill area).
+#define KP_SCRATCH_BYTES (64u << 20)
+#define KP_SCRATCH_ERR_OFFSET 256u
+#define KP_SCRATCH_SPILL_OFFSET 512u
+#define KP_SCRATCH_MAX_DEVICES 64
+
 namespace kp {
 
 struct TraceArgs {
@@ -46,6 +53,9 @@ hipError_t trace_occupancy(int* blocks_per_cu_closest, int* blocks_per_cu_occlud
 hipError_t trace_occupancy_five(int* blocks_per_cu_closest, int* blocks_per_cu_occluded);
 hipError_t shade_occupancy(int* blocks_per_cu_plain, int* blocks_per_cu_guided);
 hipError_t launch_reset_u32(uint32_t* p, uint32_t n, hipStream_t s);
+// Device address of the static scratch on `device` (made current by the
+// caller), looked up once per device.
+hipError_t noalloc_scratch(int device, void** ptr);
 hipError_t launch_copy16(void* dst, const void* src, uint64_t n16, int cus, hipStream_t s);
 
 }  // namespace kp

@@ -86,8 +86,10 @@ struct SceneRec {
 };
 
 struct RenderArgs {
-    const float4* stris;     // per primitive: {v0, mat}, {v1, light}, {v2, 0}
-    const Material* mats;
+    const float4* stris;     // per primitive: {v0, mat}, {v1, light}, {v2, normal record or KP_INVALID}
+    const float4* snorm;     // per normal record: three float4 {n_k, 0} (unit, world space)
+    const MatRec* mats;
+    const Lobe* lobes;
     const LightRec* lights;
     const uint32_t* light_tris;
     const float* light_cdf;
@@ -335,16 +337,20 @@ __global__ void k_rebase_tris(float4* tris, uint32_t n, uint32_t prim_off) {
     v.w = __uint_as_float(__float_as_uint(v.w) + prim_off);
     tris[3 * (size_t)i] = v;
 }
-// Shading records: material id += mat_off, light id (if >= 0) += light_off.
-__global__ void k_rebase_stris(float4* st, uint32_t n, uint32_t mat_off, uint32_t light_off) {
+// Shading records: material id += mat_off, light id (if >= 0) += light_off,
+// normal record (if any) += norm_off.
+__global__ void k_rebase_stris(float4* st, uint32_t n, uint32_t mat_off, uint32_t light_off, uint32_t norm_off) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    float4 a = st[3 * (size_t)i], b = st[3 * (size_t)i + 1];
+    float4 a = st[3 * (size_t)i], b = st[3 * (size_t)i + 1], c = st[3 * (size_t)i + 2];
     a.w = __uint_as_float(__float_as_uint(a.w) + mat_off);
     const int lid = __float_as_int(b.w);
     if (lid >= 0) b.w = __int_as_float(lid + (int)light_off);
+    const uint32_t nid = __float_as_uint(c.w);
+    if (nid != KP_INVALID) c.w = __uint_as_float(nid + norm_off);
     st[3 * (size_t)i] = a;
     st[3 * (size_t)i + 1] = b;
+    st[3 * (size_t)i + 2] = c;
 }
 __global__ void k_rebase_lights(LightRec* l, uint32_t n, uint32_t lt_off, uint32_t cdf_off, uint32_t* lt,
                                 uint32_t n_lt, uint32_t prim_off) {
@@ -442,7 +448,11 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
         bool alive = false, guided = false, recorded = false;
         uint32_t rec = 0;
         f3 beta = mk(0, 0, 0), beta_pre = beta, L = beta, P = beta, ng = beta, nrm = beta, wo = beta;
-        uint32_t mat_id = 0;
+        Bsdf bs;
+        bs.lobes = a.lobes;
+        bs.n = 0;
+        bs.beta = beta;
+        bs.inv_sum = 1.0f;
         Pcg rng;
         rng.state = 0;
         rng.inc = 1;
@@ -493,12 +503,26 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                 const float4 t0 = a.stris[3 * (size_t)prim], t1 = a.stris[3 * (size_t)prim + 1],
                              t2 = a.stris[3 * (size_t)prim + 2];
                 const f3 v0 = ld3(t0), v1 = ld3(t1), v2 = ld3(t2);
-                mat_id = __float_as_uint(t0.w);
-                const Material& m = a.mats[mat_id];
+                const MatRec m = a.mats[__float_as_uint(t0.w)];
                 const int lid = __float_as_int(t1.w);
                 ng = normalize(cross(v1 - v0, v2 - v0));
                 const float b0 = 1.0f - hit.z - hit.w;
                 P = v0 * b0 + v1 * hit.z + v2 * hit.w;
+                f3 ns = ng;
+                const uint32_t nid = __float_as_uint(t2.w);
+                if (nid != KP_INVALID) {
+                    // TriangleMesh::post_hit (shape.cpp:615-654): interpolated
+                    // shading normal; the shading point moved off the flat
+                    // triangle (Hanika, "Hacking the Shadow Terminator", 2021)
+                    const float4* nr = a.snorm + 3 * (size_t)nid;
+                    const f3 n0 = ld3(nr[0]), n1 = ld3(nr[1]), n2 = ld3(nr[2]);
+                    ns = normalize(n0 * b0 + n1 * hit.z + n2 * hit.w);
+                    f3 tu = P - v0, tw = P - v1, tx = P - v2;
+                    tu = tu - n0 * fminf(0.0f, dot(tu, n0));
+                    tw = tw - n1 * fminf(0.0f, dot(tw, n1));
+                    tx = tx - n2 * fminf(0.0f, dot(tx, n2));
+                    P = P + (tu * b0 + tw * hit.z + tx * hit.w);
+                }
                 const f3 dir = mk(r1.x, r1.y, r1.z);
                 const f3 Le = mk(m.Le[0], m.Le[1], m.Le[2]);
                 const float cos_hit = dot(dir, ng);
@@ -515,11 +539,14 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
 #pragma unroll
                     for (int i = 0; i < kDraws; ++i) u[i] = rng.nextf();
                     const f3 wo_w = -dir;
-                    nrm = (m.type == 2) ? ng : (dot(wo_w, ng) < 0.0f ? -ng : ng);
+                    // two-sided closures see the side the path arrives on; a
+                    // transmitting material keeps its normals' orientation
+                    nrm = (!(m.flags & KP_MATF_TRANSMIT) && dot(wo_w, ng) < 0.0f) ? -ns : ns;
                     const Frame fr(nrm);
                     wo = fr.to_local(wo_w);
+                    bs = bsdf_at(m, a.lobes, beta);
                     if constexpr (kGuided) {
-                        const bool eligible = m.type == 0 || (m.type == 1 && m.alpha >= a.min_alpha);
+                        const bool eligible = !(m.flags & KP_MATF_TRANSMIT) && m.guide_alpha >= a.min_alpha;
                         guided = eligible && a.guide_on;
                         recorded = recorded && eligible;
                     }
@@ -545,7 +572,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                                 const f3 wi = fr.to_local(wi_w);
                                 f3 f;
                                 float pdf_b;
-                                bsdf_eval(m, wo, wi, f, pdf_b);
+                                bsdf_eval(bs, wo, wi, f, pdf_b);
                                 if (!is_zero(f)) {
                                     nee = true;
                                     pdf_l = inv_nl * (1.0f / lr.area) * (dist * dist) / cos_l;
@@ -570,7 +597,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                     }
                     if (alive) {
                         if constexpr (kGuided) use_guide = guided && !(u[8] < a.bsdf_frac);
-                        if (!use_guide) bs_ok = bsdf_sample(m, wo, u[5], u[6], u[7], bs_wi, bs_f, bs_pdf);
+                        if (!use_guide) bs_ok = bsdf_sample(bs, wo, u[5], u[6], u[7], bs_wi, bs_f, bs_pdf);
                     }
                 }
             }
@@ -627,7 +654,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                 if (use_guide) {
                     wi = fr.to_local(mk(ga.dx, ga.dy, ga.dz));
                     float pdf_bsdf;
-                    bsdf_eval(a.mats[mat_id], wo, wi, f, pdf_bsdf);
+                    bsdf_eval(bs, wo, wi, f, pdf_bsdf);
                     pdf = a.bsdf_frac * pdf_bsdf + (1.0f - a.bsdf_frac) * ga.pdf_b;
                     ok = !is_zero(f) && pdf > 0.0f;
                 } else if (guided && ok) {
@@ -780,10 +807,14 @@ hipError_t shade_occupancy(int* blocks_per_cu_plain, int* blocks_per_cu_guided) 
 // =================================================================== host ==
 struct RenderState {
     bool scene_ready = false;
-    std::vector<kp::Material> mats;
+    std::vector<kp::MatRec> mats;    // materials (kp_material_add*), lobes in `lobes`
+    std::vector<kp::Lobe> lobes;
     // device scene
     float4* stris = nullptr;
-    kp::Material* d_mats = nullptr;
+    float4* d_snorm = nullptr;       // shading-normal records (3 float4 each)
+    uint32_t n_snorm = 0;
+    kp::MatRec* d_mats = nullptr;
+    kp::Lobe* d_lobes = nullptr;
     kp::LightRec* d_lights = nullptr;
     uint32_t* d_light_tris = nullptr;
     float* d_light_cdf = nullptr;
@@ -806,6 +837,11 @@ namespace kp {
 
 static void free_scene(RenderState* r) {
     if (r->stris) (void)hipFree(r->stris);
+    if (r->d_snorm) (void)hipFree(r->d_snorm);
+    if (r->d_lobes) (void)hipFree(r->d_lobes);
+    r->d_snorm = nullptr;
+    r->n_snorm = 0;
+    r->d_lobes = nullptr;
     if (r->d_mats) (void)hipFree(r->d_mats);
     if (r->d_lights) (void)hipFree(r->d_lights);
     if (r->d_light_tris) (void)hipFree(r->d_light_tris);
@@ -847,6 +883,36 @@ static float tri_area(const float* t) {
     return 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
 }
 
+// World-space unit vertex normals of triangle t of placed mesh mi (the
+// kp_mesh_add meshes, then one entry per instance), if that mesh has them.
+// Instances carry their prototype's normals under the inverse transpose of
+// the object-to-world matrix, i.e. the transpose of the world-to-object one.
+static bool shading_normals(const kp_ctx* c, size_t mi, uint32_t t, float* out9) {
+    const size_t n_mesh = c->mesh_first.size();
+    if (mi < n_mesh) {
+        const size_t p = (size_t)c->mesh_first[mi] + t;
+        if (p >= c->soup_has_n.size() || !c->soup_has_n[p]) return false;
+        std::memcpy(out9, &c->soup_n[9 * p], 9 * sizeof(float));
+        return true;
+    }
+    const size_t k = mi - n_mesh;
+    const uint32_t b = c->inst_blas[k];
+    const size_t p = (size_t)c->blas_first[b] + t;
+    if (p >= c->blas_has_n.size() || !c->blas_has_n[p]) return false;
+    const float* src = &c->blas_n[9 * p];
+    const float* w = &c->inst_w2o[12 * k];
+    for (int v = 0; v < 3; ++v) {
+        double n[3], len = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            n[a] = (double)w[a] * src[3 * v] + (double)w[4 + a] * src[3 * v + 1] + (double)w[8 + a] * src[3 * v + 2];
+            len += n[a] * n[a];
+        }
+        len = std::sqrt(len);
+        for (int a = 0; a < 3; ++a) out9[3 * v + a] = len > 0.0 ? (float)(n[a] / len) : 0.0f;
+    }
+    return true;
+}
+
 // Uploads primitives, materials and the light table (one area light per
 // emissive mesh, sampled by area: TriangleMesh::setup_dpdf, shape.cpp:700-709).
 static int prepare_scene(kp_ctx* c) {
@@ -880,13 +946,14 @@ static int prepare_scene(kp_ctx* c) {
         }
     }
     std::vector<float4> st(3 * (size_t)n);
+    std::vector<float4> snorm;   // shading-normal records, in primitive order
     std::vector<uint32_t> light_tris;
     std::vector<float> cdf;
     std::vector<LightRec> lights;
     for (size_t mi = 0; mi < g_first.size(); ++mi) {
         const uint32_t mat = g_mat[mi];
         if (mat >= r->mats.size()) return fail(c, KP_ERR_INVALID, "mesh material id out of range");
-        const Material& m = r->mats[mat];
+        const MatRec& m = r->mats[mat];
         const bool emissive = m.Le[0] != 0.0f || m.Le[1] != 0.0f || m.Le[2] != 0.0f;
         int32_t lid = -1;
         if (emissive) {
@@ -915,13 +982,19 @@ static int prepare_scene(kp_ctx* c) {
         for (uint32_t t = 0; t < g_count[mi]; ++t) {
             const uint32_t p = g_first[mi] + t;
             const float* v = &world[9 * (size_t)p];
-            uint32_t matbits = mat;
-            float fm, fl;
+            uint32_t matbits = mat, nid = KP_INVALID;
+            float nrm[9];
+            if (shading_normals(c, mi, t, nrm)) {
+                nid = (uint32_t)(snorm.size() / 3);
+                for (int k = 0; k < 3; ++k) snorm.push_back(make_float4(nrm[3 * k], nrm[3 * k + 1], nrm[3 * k + 2], 0.0f));
+            }
+            float fm, fl, fn;
             std::memcpy(&fm, &matbits, 4);
             std::memcpy(&fl, &lid, 4);
+            std::memcpy(&fn, &nid, 4);
             st[3 * (size_t)p] = make_float4(v[0], v[1], v[2], fm);
             st[3 * (size_t)p + 1] = make_float4(v[3], v[4], v[5], fl);
-            st[3 * (size_t)p + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+            st[3 * (size_t)p + 2] = make_float4(v[6], v[7], v[8], fn);
         }
     }
     for (int a = 0; a < 3; ++a) {
@@ -935,8 +1008,15 @@ static int prepare_scene(kp_ctx* c) {
         }
     KP_HIP_TRY(c, hipMalloc((void**)&r->stris, st.size() * sizeof(float4)));
     KP_HIP_TRY(c, hipMemcpy(r->stris, st.data(), st.size() * sizeof(float4), hipMemcpyHostToDevice));
-    KP_HIP_TRY(c, hipMalloc((void**)&r->d_mats, r->mats.size() * sizeof(Material)));
-    KP_HIP_TRY(c, hipMemcpy(r->d_mats, r->mats.data(), r->mats.size() * sizeof(Material), hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMalloc((void**)&r->d_mats, r->mats.size() * sizeof(MatRec)));
+    KP_HIP_TRY(c, hipMemcpy(r->d_mats, r->mats.data(), r->mats.size() * sizeof(MatRec), hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMalloc((void**)&r->d_lobes, r->lobes.size() * sizeof(Lobe)));
+    KP_HIP_TRY(c, hipMemcpy(r->d_lobes, r->lobes.data(), r->lobes.size() * sizeof(Lobe), hipMemcpyHostToDevice));
+    r->n_snorm = (uint32_t)(snorm.size() / 3);
+    if (!snorm.empty()) {
+        KP_HIP_TRY(c, hipMalloc((void**)&r->d_snorm, snorm.size() * sizeof(float4)));
+        KP_HIP_TRY(c, hipMemcpy(r->d_snorm, snorm.data(), snorm.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
     r->n_lights = (uint32_t)lights.size();
     r->n_light_tris = (uint32_t)light_tris.size();
     r->n_cdf = (uint32_t)cdf.size();
@@ -1124,7 +1204,9 @@ static RenderArgs base_render_args(kp_ctx* c) {
     RenderState* r = rstate(c);
     RenderArgs a{};
     a.stris = r->stris;
+    a.snorm = r->d_snorm;
     a.mats = r->d_mats;
+    a.lobes = r->d_lobes;
     a.lights = r->d_lights;
     a.light_tris = r->d_light_tris;
     a.light_cdf = r->d_light_cdf;
@@ -1237,7 +1319,9 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     if ((rc = ensure_paths(c, n_paths))) return rc;
     RenderArgs a{};
     a.stris = r->stris;
+    a.snorm = r->d_snorm;
     a.mats = r->d_mats;
+    a.lobes = r->d_lobes;
     a.lights = r->d_lights;
     a.light_tris = r->d_light_tris;
     a.light_cdf = r->d_light_cdf;
@@ -1333,7 +1417,9 @@ struct BatchState {
     uint4* nodes = nullptr;
     float4* tris = nullptr;
     float4* stris = nullptr;
-    Material* mats = nullptr;
+    float4* snorm = nullptr;
+    MatRec* mats = nullptr;
+    Lobe* lobes = nullptr;
     LightRec* lights = nullptr;
     uint32_t* light_tris = nullptr;
     float* light_cdf = nullptr;
@@ -1373,7 +1459,8 @@ static void batch_drop_graph(BatchState* b) {
 
 static void batch_free(BatchState* b) {
     batch_drop_graph(b);
-    for (void* p : {(void*)b->nodes, (void*)b->tris, (void*)b->stris, (void*)b->mats, (void*)b->lights,
+    for (void* p : {(void*)b->nodes, (void*)b->tris, (void*)b->stris, (void*)b->snorm, (void*)b->mats,
+                    (void*)b->lobes, (void*)b->lights,
                     (void*)b->light_tris, (void*)b->light_cdf, (void*)b->spill, (void*)b->film_sum,
                     (void*)b->film_comp, (void*)b->film_out, b->path_mem, (void*)b->d_scenes, (void*)b->d_pass})
         if (p) (void)hipFree(p);
@@ -1388,7 +1475,7 @@ static bool verbose() {
 }
 
 static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState** out) {
-    uint64_t nn = 0, nt = 0, nm = 0, nl = 0, nlt = 0, ncdf = 0;
+    uint64_t nn = 0, nt = 0, nm = 0, nl = 0, nlt = 0, ncdf = 0, nlb = 0, nsn = 0;
     uint32_t depth = 0;
     for (uint32_t i = 0; i < n; ++i) {
         kp_ctx* s = scenes[i];
@@ -1404,6 +1491,8 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
         nn += s->n_nodes;
         nt += s->n_tris_built;
         nm += r->mats.size();
+        nlb += r->lobes.size();
+        nsn += r->n_snorm;
         nl += r->n_lights;
         nlt += r->n_light_tris;
         ncdf += r->n_cdf;
@@ -1425,14 +1514,33 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
     alloc((void**)&b->nodes, nn * KP_NODE_BYTES);
     alloc((void**)&b->tris, nt * KP_TRI_BYTES);
     alloc((void**)&b->stris, nt * 3 * sizeof(float4));
-    alloc((void**)&b->mats, nm * sizeof(Material));
+    alloc((void**)&b->mats, nm * sizeof(MatRec));
+    alloc((void**)&b->lobes, nlb * sizeof(Lobe));
+    alloc((void**)&b->snorm, nsn * 3 * sizeof(float4));
     alloc((void**)&b->lights, nl * sizeof(LightRec));
     alloc((void**)&b->light_tris, nlt * 4);
     alloc((void**)&b->light_cdf, ncdf * 4);
     alloc((void**)&b->d_pass, 64);
     if (e != hipSuccess) return bail(e, "kp_batch_create: allocation");
-    uint32_t on = 0, ot = 0, om = 0, ol = 0, olt = 0, ocdf = 0;
+    uint32_t on = 0, ot = 0, om = 0, ol = 0, olt = 0, ocdf = 0, osn = 0;
     uint64_t steps = 0;   // per-ray step limit: a ray stays in its own scene's BVH
+    // materials and lobes merged on the host (lobe ranges rebased), uploaded once
+    std::vector<MatRec> all_mats;
+    std::vector<Lobe> all_lobes;
+    for (uint32_t i = 0; i < n; ++i) {
+        const RenderState* r = scenes[i]->render;
+        for (MatRec m : r->mats) {
+            m.first += (uint32_t)all_lobes.size();
+            all_mats.push_back(m);
+        }
+        all_lobes.insert(all_lobes.end(), r->lobes.begin(), r->lobes.end());
+    }
+    e = hipMemcpyAsync(b->mats, all_mats.data(), all_mats.size() * sizeof(MatRec), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->lobes, all_lobes.data(), all_lobes.size() * sizeof(Lobe), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);   // the host vectors are the sources
+    if (e != hipSuccess) return bail(e, "kp_batch_create: materials");
     for (uint32_t i = 0; i < n; ++i) {
         kp_ctx* s = scenes[i];
         const RenderState* r = s->render;
@@ -1448,7 +1556,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
         copy(b->nodes + 8 * (size_t)on, s->d_nodes, (size_t)sn * KP_NODE_BYTES);
         copy(b->tris + 3 * (size_t)ot, s->d_tris, (size_t)st * KP_TRI_BYTES);
         copy(b->stris + 3 * (size_t)ot, r->stris, (size_t)st * 3 * sizeof(float4));
-        copy(b->mats + om, r->d_mats, (size_t)sm * sizeof(Material));
+        if (r->n_snorm) copy(b->snorm + 3 * (size_t)osn, r->d_snorm, (size_t)r->n_snorm * 3 * sizeof(float4));
         if (r->n_lights) {
             copy(b->lights + ol, r->d_lights, (size_t)r->n_lights * sizeof(LightRec));
             copy(b->light_tris + olt, r->d_light_tris, (size_t)r->n_light_tris * 4);
@@ -1460,7 +1568,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
         hipLaunchKernelGGL(k_rebase_tris, dim3((st + 255) / 256), dim3(256), 0, c->stream, b->tris + 3 * (size_t)ot,
                            st, ot);
         hipLaunchKernelGGL(k_rebase_stris, dim3((st + 255) / 256), dim3(256), 0, c->stream,
-                           b->stris + 3 * (size_t)ot, st, om, ol);
+                           b->stris + 3 * (size_t)ot, st, om, ol, osn);
         const uint32_t nmax = std::max(r->n_lights, r->n_light_tris);
         if (nmax)
             hipLaunchKernelGGL(k_rebase_lights, dim3((nmax + 255) / 256), dim3(256), 0, c->stream, b->lights + ol,
@@ -1473,6 +1581,7 @@ static int batch_create(kp_ctx* c, kp_ctx* const* scenes, uint32_t n, BatchState
         on += sn;
         ot += st;
         om += sm;
+        osn += r->n_snorm;
         ol += r->n_lights;
         olt += r->n_light_tris;
         ocdf += r->n_cdf;
@@ -1610,7 +1719,9 @@ static int batch_render(BatchState* b, const kp_render_desc* descs, uint32_t max
     KP_HIP_TRY(c, hipMemcpy(b->ps.totals, zero2, 16, hipMemcpyHostToDevice));
     RenderArgs a{};
     a.stris = b->stris;
+    a.snorm = b->snorm;
     a.mats = b->mats;
+    a.lobes = b->lobes;
     a.lights = b->lights;
     a.light_tris = b->light_tris;
     a.light_cdf = b->light_cdf;
@@ -1740,26 +1851,74 @@ static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t p
     return KP_OK;
 }
 
+// Validates the lobes and appends one material (its lobes to the lobe table).
+static int add_material(kp_ctx* c, const kp_lobe* lobes, uint32_t n, const float* Le, const char* who,
+                        uint32_t* id) {
+    static_assert(sizeof(Lobe) == 48 && sizeof(MatRec) == 32, "closure record layout");
+    const std::string w(who);
+    std::vector<Lobe> ls;
+    MatRec mr{};
+    mr.guide_alpha = INFINITY;
+    for (uint32_t i = 0; i < n; ++i) {
+        const kp_material& m = lobes[i].lobe;
+        if (m.type < 0 || m.type > 3 || m.dist < 0 || m.dist > 1)
+            return fail(c, KP_ERR_INVALID, w + ": bad closure type or distribution");
+        if (m.type != 0 && !(m.alpha > 0.0f && std::isfinite(m.alpha)))
+            return fail(c, KP_ERR_INVALID, w + ": microfacet alpha must be > 0");
+        if (m.type != 0 && !(m.alpha_y >= 0.0f && std::isfinite(m.alpha_y)))
+            return fail(c, KP_ERR_INVALID, w + ": alpha_y must be >= 0 (0 = alpha)");
+        if ((m.type == 2 || m.type == 3) && !(m.eta > 0.0f && std::isfinite(m.eta)))
+            return fail(c, KP_ERR_INVALID, w + ": eta must be > 0");
+        for (int k = 0; k < 3; ++k)
+            if (!(lobes[i].weight[k] >= 0.0f && std::isfinite(lobes[i].weight[k])))
+                return fail(c, KP_ERR_INVALID, w + ": lobe weights must be finite and >= 0");
+        Lobe l{};
+        l.type = m.type;
+        l.dist = m.dist;
+        for (int k = 0; k < 3; ++k) {
+            l.w[k] = lobes[i].weight[k];
+            l.c[k] = m.type == 1 ? m.F0[k] : m.albedo[k];
+        }
+        l.ax = m.alpha;
+        l.ay = m.alpha_y > 0.0f ? m.alpha_y : m.alpha;
+        l.eta = m.eta;
+        if (m.type == 2) mr.flags |= KP_MATF_TRANSMIT;
+        if (m.type != 0) mr.guide_alpha = std::fmin(mr.guide_alpha, std::fmin(l.ax, l.ay));
+        ls.push_back(l);
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (!(Le[k] >= 0.0f && std::isfinite(Le[k]))) return fail(c, KP_ERR_INVALID, w + ": Le must be finite and >= 0");
+        mr.Le[k] = Le[k];
+    }
+    RenderState* r = rstate(c);
+    mr.first = (uint32_t)r->lobes.size();
+    mr.n = n;
+    r->lobes.insert(r->lobes.end(), ls.begin(), ls.end());
+    r->mats.push_back(mr);
+    r->scene_ready = false;
+    if (id) *id = (uint32_t)(r->mats.size() - 1);
+    return KP_OK;
+}
+
 }  // namespace kp
 
 extern "C" {
 
 int kp_material_add(kp_ctx* c, const kp_material* m, uint32_t* id) {
     if (!c || !m) return KP_ERR_INVALID;
-    if (m->type < 0 || m->type > 2 || m->dist < 0 || m->dist > 1)
-        return kp::fail(c, KP_ERR_INVALID, "kp_material_add: bad closure type or distribution");
-    if (m->type != 0 && !(m->alpha > 0.0f))
-        return kp::fail(c, KP_ERR_INVALID, "kp_material_add: microfacet alpha must be > 0");
-    if (m->type == 2 && !(m->eta > 0.0f))
-        return kp::fail(c, KP_ERR_INVALID, "kp_material_add: eta must be > 0");
-    RenderState* r = kp::rstate(c);
-    kp::Material g;
-    static_assert(sizeof(kp::Material) == sizeof(kp_material), "material layout");
-    std::memcpy(&g, m, sizeof(g));
-    r->mats.push_back(g);
-    r->scene_ready = false;
-    if (id) *id = (uint32_t)(r->mats.size() - 1);
-    return KP_OK;
+    kp_lobe one{};
+    one.weight[0] = one.weight[1] = one.weight[2] = 1.0f;
+    one.lobe = *m;
+    return kp::add_material(c, &one, 1, m->Le, "kp_material_add", id);
+}
+
+int kp_material_add_composite(kp_ctx* c, const kp_lobe* lobes, uint32_t n_lobes, const float* Le,
+                              uint32_t* id) {
+    if (!c) return KP_ERR_INVALID;
+    if (!lobes || n_lobes == 0 || n_lobes > KP_MAX_LOBES)
+        return kp::fail(c, KP_ERR_INVALID, "kp_material_add_composite: 1..8 lobes");
+    const float zero[3] = {0.0f, 0.0f, 0.0f};
+    return kp::add_material(c, lobes, n_lobes, Le ? Le : zero, "kp_material_add_composite", id);
 }
 
 int kp_mesh_set_material(kp_ctx* c, uint32_t mesh_id, uint32_t material_id) {

@@ -24,12 +24,37 @@
 //    index, which makes the answer independent of traversal order.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kp_kernels.h"
 #include "kp_layout.h"
 
 #pragma clang fp contract(off)
 
 namespace kp {
+
+// Scratch of the allocation-free closest-hit entry point
+// (kp_intersect_closest_noalloc): work counter at byte 0, error word at byte
+// 256, per-lane stack spill area from byte KP_SCRATCH_SPILL_OFFSET.  Each
+// device holds its own copy for the life of the process; host code must use
+// the address noalloc_scratch() returns for the device, never this name.
+static __device__ __align__(256) unsigned char scratch[KP_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+    static std::mutex mu;
+    static void* per_device[KP_SCRATCH_MAX_DEVICES] = {};
+    if (device < 0 || device >= KP_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!per_device[device]) {
+        // the caller has made `device` current
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        per_device[device] = p;
+    }
+    *ptr = per_device[device];
+    return hipSuccess;
+}
 
 
 struct RayPre {

@@ -53,7 +53,15 @@ class Material(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("dist", ctypes.c_int32),
                 ("albedo", ctypes.c_float * 3), ("F0", ctypes.c_float * 3),
                 ("alpha", ctypes.c_float), ("eta", ctypes.c_float),
-                ("Le", ctypes.c_float * 3), ("pad", ctypes.c_float)]
+                ("Le", ctypes.c_float * 3), ("alpha_y", ctypes.c_float)]
+
+
+class Lobe(ctypes.Structure):
+    _fields_ = [("weight", ctypes.c_float * 3), ("pad", ctypes.c_float), ("lobe", Material)]
+
+
+MAX_LOBES = 8
+MAT_LAMBERT, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_GLOSS = 0, 1, 2, 3
 
 
 class RenderDesc(ctypes.Structure):
@@ -99,6 +107,7 @@ def make_material(d):
     m.alpha = d["alpha"]
     m.eta = d["eta"]
     m.Le[:] = d["Le"]
+    m.alpha_y = d.get("alpha_y", 0.0)
     return m
 
 
@@ -127,6 +136,9 @@ SIGNATURES = {
     "kp_set_stream": (_i32, [_p, _p]),
     "kp_synchronize": (_i32, [_p]),
     "kp_mesh_add": (_i32, [_p, _p, _u32, _p, _u32, _p, ctypes.POINTER(_u32)]),
+    "kp_mesh_set_normals": (_i32, [_p, _u32, _p]),
+    "kp_blas_set_normals": (_i32, [_p, _u32, _p]),
+    "kp_material_add_composite": (_i32, [_p, _p, _u32, _p, ctypes.POINTER(_u32)]),
     "kp_bvh_build": (_i32, [_p, _p]),
     "kp_blas_add": (_i32, [_p, _p, _u32, _p, _u32, ctypes.POINTER(_u32)]),
     "kp_blas_set_material": (_i32, [_p, _u32, _u32]),
@@ -149,6 +161,7 @@ SIGNATURES = {
     "kp_bvh8_blob_closest": (_i32, [_p, _p, _u32, _p]),
     "kp_bvh8_blob_occluded": (_i32, [_p, _p, _u32, _p]),
     "kp_intersect_closest_host": (_i32, [_p, _p, _u32, _p]),
+    "kp_intersect_closest_noalloc": (_i32, [_p, _p, _u32, _p]),
     "kp_occluded_host": (_i32, [_p, _p, _u32, _p]),
     "kp_set_trace_tuning": (_i32, [_p, _u32]),
     "kp_set_pass_paths": (_i32, [_p, _u32]),
@@ -378,6 +391,31 @@ class Context:
         n = tris9.shape[0]
         return self.blas_add(tris9.reshape(-1, 3), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3))
 
+    def mesh_set_normals(self, mesh_id, normals9):
+        """Vertex normals [n,9] of a kp_mesh_add mesh (its object space)."""
+        n = np.ascontiguousarray(normals9, np.float32).reshape(-1, 9)
+        self._check(lib().kp_mesh_set_normals(self._h, mesh_id, _np_ptr(n)))
+
+    def blas_set_normals(self, blas_id, normals9):
+        n = np.ascontiguousarray(normals9, np.float32).reshape(-1, 9)
+        self._check(lib().kp_blas_set_normals(self._h, blas_id, _np_ptr(n)))
+
+    def material_add(self, d):
+        """Adds a scenes.material() or scenes.composite() dict; returns its id."""
+        mid = _u32()
+        if d.get("lobes"):
+            arr = (Lobe * len(d["lobes"]))()
+            for i, (w, m) in enumerate(d["lobes"]):
+                arr[i].weight[:] = [float(x) for x in w]
+                arr[i].lobe = make_material(m)
+            le = np.ascontiguousarray(d["Le"], np.float32)
+            self._check(lib().kp_material_add_composite(self._h, ctypes.cast(arr, _p), len(d["lobes"]),
+                                                        _np_ptr(le), ctypes.byref(mid)))
+        else:
+            m = make_material(d)
+            self._check(lib().kp_material_add(self._h, ctypes.byref(m), ctypes.byref(mid)))
+        return mid.value
+
     def blas_set_material(self, blas_id, material_id):
         self._check(lib().kp_blas_set_material(self._h, blas_id, material_id))
 
@@ -455,6 +493,14 @@ class Context:
         self._check(lib().kp_intersect_closest(self._h, rp, n, hp, kind))
         return out
 
+    def intersect_closest_noalloc(self, rays, out, n=None):
+        """kp_intersect_closest_noalloc: rays and out are DeviceBuffers; traces
+        the first n rays (all by default). Asynchronous on the context stream."""
+        n = rays.nbytes // 32 if n is None else n
+        assert n * 32 <= rays.nbytes and n * 16 <= out.nbytes
+        self._check(lib().kp_intersect_closest_noalloc(self._h, rays.ptr, n, out.ptr))
+        return out
+
     def intersect_closest_host(self, rays):
         """Closest hit on host cores (kp_intersect_closest_host): [n,4] hits,
         bit-identical to intersect_closest."""
@@ -490,19 +536,21 @@ class Context:
     # ------------------------------------------------------------ render --
     def load_scene(self, scene, leaf_size=4, instances=INSTANCES_AUTO):
         """Adds every mesh and material of a scenes.Scene and builds the BVH."""
-        mids = []
-        for md in scene.materials:
-            mid = _u32()
-            m = make_material(md)
-            self._check(lib().kp_material_add(self._h, ctypes.byref(m), ctypes.byref(mid)))
-            mids.append(mid.value)
-        for tris, mi in zip(scene.meshes, scene.mesh_material):
+        mids = [self.material_add(md) for md in scene.materials]
+        mnorm = list(getattr(scene, "mesh_normals", []))
+        for k, (tris, mi) in enumerate(zip(scene.meshes, scene.mesh_material)):
             mesh = self.mesh_add_soup(tris)
             self._check(lib().kp_mesh_set_material(self._h, mesh, mids[mi]))
+            if k < len(mnorm) and mnorm[k] is not None:
+                self.mesh_set_normals(mesh, mnorm[k])
         protos = []
-        for tris, mi in zip(getattr(scene, "prototypes", []), getattr(scene, "prototype_material", [])):
+        pnorm = list(getattr(scene, "prototype_normals", []))
+        for k, (tris, mi) in enumerate(zip(getattr(scene, "prototypes", []),
+                                           getattr(scene, "prototype_material", []))):
             b = self.blas_add_soup(tris)
             self.blas_set_material(b, mids[mi])
+            if k < len(pnorm) and pnorm[k] is not None:
+                self.blas_set_normals(b, pnorm[k])
             protos.append(b)
         inst_mat = getattr(scene, "instance_material", [])
         for k, (pi, o2w) in enumerate(getattr(scene, "instances", [])):

@@ -12,15 +12,23 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-LAMBERT, CONDUCTOR, DIELECTRIC = 0, 1, 2
+LAMBERT, CONDUCTOR, DIELECTRIC, GLOSS = 0, 1, 2, 3
 GGX, BECKMANN = 0, 1
 
 
 def material(type_=LAMBERT, dist=GGX, albedo=(0.0, 0.0, 0.0), F0=(0.0, 0.0, 0.0), alpha=0.5,
-             eta=1.5, Le=(0.0, 0.0, 0.0)):
+             eta=1.5, Le=(0.0, 0.0, 0.0), alpha_y=0.0):
+    """One closure; alpha along the shading frame's tangent, alpha_y along its
+    bitangent (0: isotropic)."""
     return dict(type=int(type_), dist=int(dist), albedo=tuple(map(float, albedo)),
                 F0=tuple(map(float, F0)), alpha=float(alpha), eta=float(eta),
-                Le=tuple(map(float, Le)))
+                Le=tuple(map(float, Le)), alpha_y=float(alpha_y))
+
+
+def composite(lobes, Le=(0.0, 0.0, 0.0)):
+    """Material of 1..8 weighted lobes [(weight rgb, material dict)] emitting Le
+    (a CompositeClosure of the reference, src/shading/bsdf.h:132-196)."""
+    return dict(type=-1, lobes=[(tuple(map(float, w)), m) for w, m in lobes], Le=tuple(map(float, Le)))
 
 
 @dataclass
@@ -46,15 +54,58 @@ class Scene:
     prototype_material: list = field(default_factory=list)
     instances: list = field(default_factory=list)
     instance_material: list = field(default_factory=list)   # per instance: material or None (prototype's)
+    # shading normals: per mesh / prototype None or [n,9] vertex normals (the
+    # mesh's space; meshes are world space here)
+    mesh_normals: list = field(default_factory=list)
+    prototype_normals: list = field(default_factory=list)
 
-    def add_mesh(self, tris9, mat_index):
+    def add_mesh(self, tris9, mat_index, normals=None):
         self.meshes.append(np.ascontiguousarray(tris9, np.float32).reshape(-1, 9))
         self.mesh_material.append(int(mat_index))
+        while len(self.mesh_normals) < len(self.meshes) - 1:
+            self.mesh_normals.append(None)
+        self.mesh_normals.append(None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9))
 
-    def add_prototype(self, tris9, mat_index):
+    def add_prototype(self, tris9, mat_index, normals=None):
         self.prototypes.append(np.ascontiguousarray(tris9, np.float32).reshape(-1, 9))
         self.prototype_material.append(int(mat_index))
+        while len(self.prototype_normals) < len(self.prototypes) - 1:
+            self.prototype_normals.append(None)
+        self.prototype_normals.append(None if normals is None else
+                                      np.ascontiguousarray(normals, np.float32).reshape(-1, 9))
         return len(self.prototypes) - 1
+
+    def shading_normals(self):
+        """(normals [N,9] float32, has [N] uint8) in primitive-id order, world
+        space and unit length as the library forms them (instances: the
+        prototype's normals under the transpose of the world-to-object
+        matrix), or None when no mesh has normals."""
+        mn = list(self.mesh_normals) + [None] * (len(self.meshes) - len(self.mesh_normals))
+        pn = list(self.prototype_normals) + [None] * (len(self.prototypes) - len(self.prototype_normals))
+        if all(x is None for x in mn) and all(x is None for x in pn):
+            return None
+        outs, has = [], []
+
+        def unit(v):
+            v = v.reshape(-1, 3).astype(np.float64)
+            return (v / np.sqrt(np.sum(v * v, axis=1, keepdims=True))).astype(np.float32).reshape(-1, 9)
+
+        for m, nrm in zip(self.meshes, mn):
+            outs.append(np.zeros((m.shape[0], 9), np.float32) if nrm is None else unit(nrm))
+            has.append(np.full(m.shape[0], 0 if nrm is None else 1, np.uint8))
+        for pi, o2w in self.instances:
+            nrm = pn[pi]
+            n = self.prototypes[pi].shape[0]
+            if nrm is None:
+                outs.append(np.zeros((n, 9), np.float32))
+                has.append(np.zeros(n, np.uint8))
+                continue
+            m = np.asarray(o2w, np.float64)
+            w2o = np.linalg.inv(np.vstack([m, [0, 0, 0, 1]]))[:3, :3].astype(np.float32).astype(np.float64)
+            v = unit(nrm).reshape(-1, 3).astype(np.float64) @ w2o   # (W^T n) as rows
+            outs.append(unit(v.reshape(-1, 9)))
+            has.append(np.ones(n, np.uint8))
+        return np.concatenate(outs), np.concatenate(has)
 
     def add_instance(self, proto, o2w, mat_index=None):
         self.instances.append((int(proto), np.asarray(o2w, np.float32).reshape(3, 4)))
@@ -307,6 +358,41 @@ def materials_scene(width=32, height=32, spp=16, max_depth=6, rr_depth=3, seed=1
     for m, c in zip(mats, centers):
         s.materials.append(m)
         s.add_mesh(uv_sphere(24, 12, center=c, radius=0.3), len(s.materials) - 1)
+    lt = len(s.materials)
+    s.materials.append(material(LAMBERT, albedo=(0.2, 0.2, 0.2), Le=(4.0, 6.0, 8.0)))
+    s.add_mesh(quad((-0.95, 0.2, -0.6), (-0.95, 0.2, -0.2), (-0.95, 0.6, -0.2), (-0.95, 0.6, -0.6)), lt)
+    return s
+
+
+def sphere_normals(tris9, center):
+    """Smooth-shading vertex normals of a sphere mesh: (v - center) / |v - center|."""
+    v = np.asarray(tris9, np.float64).reshape(-1, 3) - np.asarray(center, np.float64)
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).reshape(-1, 9).astype(np.float32)
+
+
+def closures_scene(width=32, height=32, spp=16, max_depth=6, rr_depth=3, seed=13):
+    """The closures the reference's shader groups build beyond single lobes:
+    composites (plastic = Lambert + gloss; brushed metal = anisotropic GGX
+    conductor + a little Lambert), an anisotropic Beckmann dielectric, a gloss
+    sphere, and smooth-shaded spheres (vertex normals) in the Cornell room."""
+    s = cornell_box(width=width, height=height, spp=spp, max_depth=max_depth, rr_depth=rr_depth,
+                    seed=seed, sphere_lon=24, sphere_lat=12)
+    s.name = "closures"
+    s.meshes.pop()
+    s.mesh_material.pop()
+    s.mesh_normals = [None] * len(s.meshes)
+    plastic = composite([((0.7, 0.2, 0.2), material(LAMBERT, albedo=(1, 1, 1))),
+                         ((1, 1, 1), material(GLOSS, GGX, alpha=0.15, eta=1.5))])
+    brushed = composite([((1, 1, 1), material(CONDUCTOR, GGX, F0=(0.9, 0.8, 0.5), alpha=0.08, alpha_y=0.5)),
+                         ((0.1, 0.1, 0.1), material(LAMBERT, albedo=(1, 1, 1)))])
+    mats = [(plastic, (-0.5, -0.65, -0.4), True),
+            (brushed, (0.5, -0.65, -0.5), True),
+            (material(DIELECTRIC, BECKMANN, alpha=0.1, alpha_y=0.35, eta=1.5), (-0.35, -0.1, 0.35), False),
+            (material(GLOSS, BECKMANN, alpha=0.3, eta=1.6), (0.45, 0.0, 0.3), True)]
+    for m, c, smooth in mats:
+        s.materials.append(m)
+        t = uv_sphere(24, 12, center=c, radius=0.3)
+        s.add_mesh(t, len(s.materials) - 1, normals=sphere_normals(t, c) if smooth else None)
     lt = len(s.materials)
     s.materials.append(material(LAMBERT, albedo=(0.2, 0.2, 0.2), Le=(4.0, 6.0, 8.0)))
     s.add_mesh(quad((-0.95, 0.2, -0.6), (-0.95, 0.2, -0.2), (-0.95, 0.6, -0.2), (-0.95, 0.6, -0.6)), lt)

@@ -24,7 +24,14 @@ class Material(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("dist", ctypes.c_int32),
                 ("albedo", ctypes.c_float * 3), ("F0", ctypes.c_float * 3),
                 ("alpha", ctypes.c_float), ("eta", ctypes.c_float),
-                ("Le", ctypes.c_float * 3), ("pad", ctypes.c_float)]
+                ("Le", ctypes.c_float * 3), ("alpha_y", ctypes.c_float)]
+
+
+class Lobe(ctypes.Structure):
+    _fields_ = [("weight", ctypes.c_float * 3), ("pad", ctypes.c_float), ("lobe", Material)]
+
+
+MAX_LOBES = 8
 
 
 class RenderDesc(ctypes.Structure):
@@ -79,14 +86,18 @@ def lib():
             "or_bvh8_closest": (None, [_p, _u64, _p, _u64, _p, _u32, _p, _p, _p, _i32]),
             "or_bsdf_eval": (None, [_p, _p, _p, _p, _p]),
             "or_bsdf_sample": (_i32, [_p, _p, _p, _p, _p, _p]),
-            "or_mf_D": (ctypes.c_double, [_i32, ctypes.c_float, _p]),
-            "or_mf_lambda": (ctypes.c_double, [_i32, ctypes.c_float, _p]),
-            "or_mf_sample_vndf": (None, [_i32, ctypes.c_float, _p, _p, _p]),
+            "or_mf_D": (ctypes.c_double, [_i32, ctypes.c_float, ctypes.c_float, _p]),
+            "or_mf_lambda": (ctypes.c_double, [_i32, ctypes.c_float, ctypes.c_float, _p]),
+            "or_mf_sample_vndf": (None, [_i32, ctypes.c_float, ctypes.c_float, _p, _p, _p]),
             "or_fresnel_dielectric": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
             "or_erfinv": (ctypes.c_double, [ctypes.c_double]),
             "or_bsdf_eval_batch": (None, [_p, _p, _p, _i32, _p, _p]),
             "or_bsdf_sample_batch": (None, [_p, _p, _p, _i32, _p, _p, _p, _p]),
-            "or_mf_eval_batch": (None, [_i32, ctypes.c_float, _p, _i32, _p, _p]),
+            "or_mf_eval_batch": (None, [_i32, ctypes.c_float, ctypes.c_float, _p, _i32, _p, _p]),
+            "or_composite_eval_batch": (None, [_p, _p, _u32, _p, _p, _p, _i32, _p, _p]),
+            "or_composite_sample_batch": (None, [_p, _p, _u32, _p, _p, _p, _i32, _p, _p, _p, _p]),
+            "or_scene_set_composite": (_i32, [_p, _u32, _p, _u32, _p]),
+            "or_scene_set_normals": (_i32, [_p, _p, _p]),
             "or_mlp_param_count": (_i32, []),
             "or_mlp_encode": (None, [_p, _p]),
             "or_mlp_forward": (None, [_p, _i32, _p, _i32, _p]),
@@ -212,8 +223,9 @@ def bvh8_closest(nodes, tris12, rays, threads=0):
     return hits, int(nv.value), int(nt.value)
 
 
-def material(type_, dist=0, albedo=(0, 0, 0), F0=(0, 0, 0), alpha=0.5, eta=1.5, Le=(0, 0, 0)):
+def material(type_, dist=0, albedo=(0, 0, 0), F0=(0, 0, 0), alpha=0.5, eta=1.5, Le=(0, 0, 0), alpha_y=0.0):
     m = Material()
+    m.alpha_y = alpha_y
     m.type = type_
     m.dist = dist
     m.albedo[:] = albedo
@@ -267,26 +279,65 @@ def bsdf_sample_batch(mat, wo, u3):
     return ok.astype(bool), wi, f, pdf
 
 
-def mf_eval_batch(dist, alpha, w):
+def lobes(pairs):
+    """ctypes array of Lobe from [(weight rgb, Material), ...]."""
+    arr = (Lobe * len(pairs))()
+    for i, (w, m) in enumerate(pairs):
+        arr[i].weight[:] = [float(x) for x in w]
+        arr[i].lobe = m
+    return arr
+
+
+def composite_eval_batch(lobe_arr, wo, wi, beta=None):
+    wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32), np.shape(wi)), np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    n = wi.shape[0]
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    b = None if beta is None else np.ascontiguousarray(beta, np.float32)
+    lib().or_composite_eval_batch(ctypes.cast(lobe_arr, ctypes.c_void_p), None, len(lobe_arr),
+                                  _ptr(b) if b is not None else None, _ptr(wo), _ptr(wi), n, _ptr(f), _ptr(pdf))
+    return f, pdf
+
+
+def composite_sample_batch(lobe_arr, wo, u3, beta=None):
+    u3 = np.ascontiguousarray(u3, np.float32)
+    n = u3.shape[0]
+    wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float32), (n, 3)), np.float32)
+    wi = np.zeros((n, 3), np.float32)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    ok = np.zeros(n, np.int32)
+    b = None if beta is None else np.ascontiguousarray(beta, np.float32)
+    lib().or_composite_sample_batch(ctypes.cast(lobe_arr, ctypes.c_void_p), None, len(lobe_arr),
+                                    _ptr(b) if b is not None else None, _ptr(wo), _ptr(u3), n, _ptr(wi),
+                                    _ptr(f), _ptr(pdf), _ptr(ok))
+    return ok.astype(bool), wi, f, pdf
+
+
+def mf_eval_batch(dist, alpha, w, alpha_y=None):
     w = np.ascontiguousarray(w, np.float32)
     n = w.shape[0]
     D = np.zeros(n, np.float64)
     lam = np.zeros(n, np.float64)
-    lib().or_mf_eval_batch(dist, alpha, _ptr(w), n, _ptr(D), _ptr(lam))
+    lib().or_mf_eval_batch(dist, alpha, alpha if alpha_y is None else alpha_y, _ptr(w), n, _ptr(D), _ptr(lam))
     return D, lam
 
 
-def mf_D(dist, alpha, m):
-    return lib().or_mf_D(dist, alpha, _ptr(np.ascontiguousarray(m, np.float32)))
+def mf_D(dist, alpha, m, alpha_y=None):
+    return lib().or_mf_D(dist, alpha, alpha if alpha_y is None else alpha_y,
+                         _ptr(np.ascontiguousarray(m, np.float32)))
 
 
-def mf_lambda(dist, alpha, w):
-    return lib().or_mf_lambda(dist, alpha, _ptr(np.ascontiguousarray(w, np.float32)))
+def mf_lambda(dist, alpha, w, alpha_y=None):
+    return lib().or_mf_lambda(dist, alpha, alpha if alpha_y is None else alpha_y,
+                              _ptr(np.ascontiguousarray(w, np.float32)))
 
 
-def mf_sample_vndf(dist, alpha, wo, u2):
+def mf_sample_vndf(dist, alpha, wo, u2, alpha_y=None):
     m = np.zeros(3, np.float32)
-    lib().or_mf_sample_vndf(dist, alpha, _ptr(np.ascontiguousarray(wo, np.float32)),
+    lib().or_mf_sample_vndf(dist, alpha, alpha if alpha_y is None else alpha_y,
+                            _ptr(np.ascontiguousarray(wo, np.float32)),
                             _ptr(np.ascontiguousarray(u2, np.float32)), _ptr(m))
     return m
 
@@ -369,6 +420,23 @@ class Scene:
             lib().or_scene_free(self._h)
             self._h = None
 
+    def set_composite(self, material, pairs, Le=None):
+        """Material `material` becomes a composite of [(weight rgb, Material)]."""
+        arr = lobes(pairs)
+        le = None if Le is None else np.ascontiguousarray(Le, np.float32)
+        rc = lib().or_scene_set_composite(self._h, material, ctypes.cast(arr, ctypes.c_void_p), len(pairs),
+                                          _ptr(le) if le is not None else None)
+        if rc != 0:
+            raise RuntimeError("or_scene_set_composite failed")
+
+    def set_normals(self, normals9, has):
+        """Per-triangle vertex normals [n, 9] (world space) and a per-triangle flag."""
+        self._norms = np.ascontiguousarray(normals9, np.float32).reshape(-1, 9)
+        self._has = np.ascontiguousarray(has, np.uint8)
+        assert self._norms.shape[0] == self._tris.shape[0] == self._has.shape[0]
+        if lib().or_scene_set_normals(self._h, _ptr(self._norms), _ptr(self._has)) != 0:
+            raise RuntimeError("or_scene_set_normals failed")
+
     def render(self, desc, threads=0, want_sample0=False, guide=None):
         out = np.zeros((desc.height, desc.width, 3), np.float64)
         s0 = np.zeros((desc.height, desc.width, 3), np.float32) if want_sample0 else None
@@ -388,6 +456,27 @@ class Scene:
         if rc != 0:
             raise RuntimeError("or_integrate_rays failed (%d)" % rc)
         return out
+
+
+def _material_from(d):
+    return material(d["type"], d["dist"], d["albedo"], d["F0"], d["alpha"], d["eta"], d["Le"],
+                    alpha_y=d.get("alpha_y", 0.0))
+
+
+def scene_from(s):
+    """Oracle scene of a scene description (meshes, instances, materials as
+    dicts of type / dist / albedo / F0 / alpha / alpha_y / eta / Le, or
+    composites {"lobes": [(weight, dict)], "Le": ...}, optional shading
+    normals from s.shading_normals())."""
+    single = [_material_from(m) if not m.get("lobes") else material(0, Le=m["Le"]) for m in s.materials]
+    sc = Scene(s.soup(), s.tri_material(), s.tri_light(), single)
+    for i, m in enumerate(s.materials):
+        if m.get("lobes"):
+            sc.set_composite(i, [(w, _material_from(d)) for w, d in m["lobes"]], m["Le"])
+    sn = s.shading_normals() if hasattr(s, "shading_normals") else None
+    if sn is not None:
+        sc.set_normals(*sn)
+    return sc
 
 
 def instanced_closest(meshes, instances, rays, threads=0):

@@ -95,9 +95,11 @@ void or_bvh8_closest(const void* nodes, uint64_t n_nodes, const void* tris,
 
 /* ---- Shading ------------------------------------------------------------ */
 /* Material record, shared layout with the scene description:
- *   type: 0 Lambert, 1 conductor (Schlick F0), 2 rough dielectric
+ *   type: 0 Lambert, 1 conductor (Schlick F0), 2 rough dielectric,
+ *         3 gloss (microfacet reflection with dielectric Fresnel, KpGloss)
  *   dist: 0 GGX, 1 Beckmann
- *   params: albedo[3] | F0[3] | alpha | eta | Le[3]                       */
+ *   params: albedo[3] | F0[3] | alpha (along the frame's x) | eta | Le[3] |
+ *           alpha_y (along z; 0 = alpha)                                  */
 typedef struct {
     int32_t type;
     int32_t dist;
@@ -106,8 +108,17 @@ typedef struct {
     float alpha;
     float eta;
     float Le[3];
-    float pad;
+    float alpha_y;
 } or_material;   /* 64 bytes */
+
+/* One lobe of a composite material: closure weight and parameters (the
+ * lobe's Le is ignored). */
+#define or_max_lobes 8
+typedef struct {
+    float weight[3];
+    float pad;
+    or_material lobe;
+} or_lobe;   /* 80 bytes */
 
 /* BSDF evaluation in the y-up local frame (cos theta = w.y). wo points
  * toward the previous vertex, wi toward the next one.  Writes f[3] (without
@@ -118,10 +129,11 @@ void or_bsdf_eval(const or_material* m, const float* wo, const float* wi,
  * the sample is invalid. */
 int or_bsdf_sample(const or_material* m, const float* wo, const float* u3,
                    float* wi, float* f, float* pdf);
-/* Microfacet pieces, for the closed-form tests. */
-double or_mf_D(int dist, float alpha, const float* m);
-double or_mf_lambda(int dist, float alpha, const float* w);
-void or_mf_sample_vndf(int dist, float alpha, const float* wo, const float* u2,
+/* Microfacet pieces, for the closed-form tests (roughness ax along x, ay
+ * along z). */
+double or_mf_D(int dist, float ax, float ay, const float* m);
+double or_mf_lambda(int dist, float ax, float ay, const float* w);
+void or_mf_sample_vndf(int dist, float ax, float ay, const float* wo, const float* u2,
                        float* m);
 double or_fresnel_dielectric(double cos_i, double eta);
 double or_erfinv(double x);
@@ -130,8 +142,19 @@ void or_bsdf_eval_batch(const or_material* m, const float* wo, const float* wi, 
                         float* f, float* pdf);
 void or_bsdf_sample_batch(const or_material* m, const float* wo, const float* u3, int n,
                           float* wi, float* f, float* pdf, int32_t* ok);
-void or_mf_eval_batch(int dist, float alpha, const float* w, int n, double* D,
+void or_mf_eval_batch(int dist, float ax, float ay, const float* w, int n, double* D,
                       double* lambda);
+/* Composite materials (n_lobes weighted lobes; with lobes == NULL the single
+ * material `single`): f = sum_i w_i f_i and pdf = sum_i p_i pdf_i with the
+ * lobe-pick probabilities p_i = dot(w_i, beta) / sum_j dot(w_j, beta)
+ * (beta NULL = (1,1,1)); the sampler picks lobe i by u[0] and re-uses u[0]
+ * rescaled within the lobe. */
+void or_composite_eval_batch(const or_lobe* lobes, const or_material* single, uint32_t n_lobes,
+                             const float* beta, const float* wo, const float* wi, int n,
+                             float* f, float* pdf);
+void or_composite_sample_batch(const or_lobe* lobes, const or_material* single, uint32_t n_lobes,
+                               const float* beta, const float* wo, const float* u3, int n,
+                               float* wi, float* f, float* pdf, int32_t* ok);
 
 /* ---- Path integrator ---------------------------------------------------- */
 typedef struct {
@@ -152,6 +175,13 @@ or_scene* or_scene_create(const float* tris, uint32_t n_tris,
                           const int32_t* tri_material, const int32_t* tri_light,
                           const or_material* materials, uint32_t n_materials);
 void or_scene_free(or_scene* s);
+/* Replaces material `material` of the scene by a composite of n_lobes lobes
+ * (1..8) emitting Le (may be NULL). */
+int or_scene_set_composite(or_scene* s, uint32_t material, const or_lobe* lobes, uint32_t n_lobes,
+                           const float* Le);
+/* Per-triangle vertex normals (9 floats per triangle, world space, unit);
+ * has[i] = 0: triangle i shades with its geometric normal. */
+int or_scene_set_normals(or_scene* s, const float* normals9, const uint8_t* has);
 /* FP64 render: out[h*w*3] = mean radiance per pixel. sample0 (optional,
  * may be NULL): radiance of the first sample of every pixel, FP32. */
 int or_render(const or_scene* s, const or_render_desc* d, double* out,

@@ -180,46 +180,58 @@ double erfinv_d(double x) {
     return y;
 }
 
-// Isotropic D(m) in the y-up frame.
-double mf_D(int dist, double alpha, V3 m) {
+// Microfacet distribution with roughness ax along the frame's x axis (the
+// tangent) and ay along z (the bitangent), y-up: the stretched form of Heitz
+// 2014 eq. 87 as microfacet.h:60-76,267-286 uses it (stretched_roughness:
+// m.x with xalpha, m.z with yalpha); ax == ay takes the isotropic form.
+double mf_D(int dist, double ax, double ay, V3 m) {
     double c = m.y;
     if (c <= 0.0) return 0.0;
-    double c2 = c * c, t2 = (1.0 - c2) / c2, a2 = alpha * alpha;
-    if (dist == 0) {
-        double k = 1.0 + t2 / a2;
-        return 1.0 / (kPiD * a2 * c2 * c2 * k * k);
+    double c2 = c * c;
+    double e;   // tan^2(theta) * stretched roughness
+    if (ax == ay) {
+        e = ((1.0 - c2) / c2) / (ax * ax);
+    } else {
+        e = ((double)m.x * m.x / (ax * ax) + (double)m.z * m.z / (ay * ay)) / c2;
     }
-    return std::exp(-t2 / a2) / (kPiD * a2 * c2 * c2);
+    if (dist == 0) {
+        double k = 1.0 + e;
+        return 1.0 / (kPiD * ax * ay * c2 * c2 * k * k);
+    }
+    return std::exp(-e) / (kPiD * ax * ay * c2 * c2);
 }
 
-// Smith Lambda; exact forms (GGX: microfacet.h:101-104; Beckmann: the erf form
-// that microfacet.h:134-141 approximates).
-double mf_lambda(int dist, double alpha, V3 w) {
+// Smith Lambda with the projected roughness of Heitz 2014 eq. 80
+// (microfacet.h:78-93,243-257); exact forms (GGX: microfacet.h:101-104;
+// Beckmann: the erf form that microfacet.h:134-141 approximates).
+double mf_lambda(int dist, double ax, double ay, V3 w) {
     double c = w.y;
     double c2 = c * c;
     double s2 = std::max(0.0, 1.0 - c2);
     if (s2 == 0.0 || c == 0.0) return 0.0;
-    double tan_t = std::sqrt(s2) / std::fabs(c);
-    if (dist == 0) {
-        double at = alpha * tan_t;
-        return 0.5 * (-1.0 + std::sqrt(1.0 + at * at));
+    double at;   // projected alpha * tan(theta)
+    if (ax == ay) {
+        at = ax * (std::sqrt(s2) / std::fabs(c));
+    } else {
+        at = std::sqrt((double)w.x * w.x * ax * ax + (double)w.z * w.z * ay * ay) / std::fabs(c);
     }
-    double a = 1.0 / (alpha * tan_t);
+    if (dist == 0) return 0.5 * (-1.0 + std::sqrt(1.0 + at * at));
+    double a = 1.0 / at;
     return 0.5 * (std::erf(a) - 1.0) + std::exp(-a * a) / (2.0 * a * std::sqrt(kPiD));
 }
 
-double mf_G1(int dist, double alpha, V3 w) { return 1.0 / (1.0 + mf_lambda(dist, alpha, w)); }
-double mf_G2(int dist, double alpha, V3 wo, V3 wi) {
-    return 1.0 / (1.0 + mf_lambda(dist, alpha, wo) + mf_lambda(dist, alpha, wi));
+double mf_G1(int dist, double ax, double ay, V3 w) { return 1.0 / (1.0 + mf_lambda(dist, ax, ay, w)); }
+double mf_G2(int dist, double ax, double ay, V3 wo, V3 wi) {
+    return 1.0 / (1.0 + mf_lambda(dist, ax, ay, wo) + mf_lambda(dist, ax, ay, wi));
 }
 
 // pdf of m under the distribution of visible normals seen from wo (Heitz
 // 2014, eq. 2); wo may lie in either hemisphere.
-double mf_pdf_vndf(int dist, double alpha, V3 wo, V3 m) {
+double mf_pdf_vndf(int dist, double ax, double ay, V3 wo, V3 m) {
     double c = std::fabs((double)wo.y);
     if (c == 0.0) return 0.0;
     double wm = std::fabs((double)dot(wo, m));
-    return mf_G1(dist, alpha, wo) * wm * mf_D(dist, alpha, m) / c;
+    return mf_G1(dist, ax, ay, wo) * wm * mf_D(dist, ax, ay, m) / c;
 }
 
 // Visible-slope sampling for the Beckmann distribution at alpha = 1:
@@ -256,14 +268,15 @@ void beckmann_sample11(double cos_t, double u1, double u2, double& sx, double& s
     sy = erfinv_d(2.0 * std::max(u2, 1e-6) - 1.0);
 }
 
-// Sample a visible normal seen from wo (upper-hemisphere normal returned).
-V3 mf_sample_vndf(int dist, float alpha, V3 wo, float u1, float u2) {
+// Sample a visible normal seen from wo (upper-hemisphere normal returned);
+// anisotropic by stretching x with ax and z with ay.
+V3 mf_sample_vndf(int dist, float ax, float ay, V3 wo, float u1, float u2) {
     V3 w = wo;
     if (w.y < 0.0f) w = -w;
     if (dist == 0) {
         // GGX: E. Heitz, "Sampling the GGX Distribution of Visible Normals",
         // JCGT 7(4), 2018, written for the y-up frame.
-        V3 vh = normalize(v3(alpha * w.x, w.y, alpha * w.z));
+        V3 vh = normalize(v3(ax * w.x, w.y, ay * w.z));
         float lensq = vh.x * vh.x + vh.z * vh.z;
         V3 t1 = lensq > 0.0f ? v3(vh.z, 0.0f, -vh.x) * (1.0f / std::sqrt(lensq)) : v3(1.0f, 0.0f, 0.0f);
         V3 t2 = cross(vh, t1);
@@ -274,12 +287,12 @@ V3 mf_sample_vndf(int dist, float alpha, V3 wo, float u1, float u2) {
         float s = 0.5f * (1.0f + vh.y);
         p2 = (1.0f - s) * std::sqrt(std::max(0.0f, 1.0f - p1 * p1)) + s * p2;
         V3 nh = t1 * p1 + t2 * p2 + vh * std::sqrt(std::max(0.0f, 1.0f - p1 * p1 - p2 * p2));
-        return normalize(v3(alpha * nh.x, std::max(1e-7f, nh.y), alpha * nh.z));
+        return normalize(v3(ax * nh.x, std::max(1e-7f, nh.y), ay * nh.z));
     }
     // Beckmann: stretch, sample slopes, rotate, unstretch (Heitz & d'Eon,
     // "Importance Sampling Microfacet-Based BSDFs using the Distribution of
     // Visible Normals", EGSR 2014, sec. 4).
-    V3 ws = normalize(v3(alpha * w.x, w.y, alpha * w.z));
+    V3 ws = normalize(v3(ax * w.x, w.y, ay * w.z));
     double cos_t = ws.y;
     double phi = std::atan2((double)ws.z, (double)ws.x);
     if (ws.x == 0.0f && ws.z == 0.0f) phi = 0.0;
@@ -288,8 +301,8 @@ V3 mf_sample_vndf(int dist, float alpha, V3 wo, float u1, float u2) {
     double cp = std::cos(phi), sp = std::sin(phi);
     double rx = cp * sx - sp * sy;
     double rz = sp * sx + cp * sy;
-    rx *= alpha;
-    rz *= alpha;
+    rx *= ax;
+    rz *= ay;
     return normalize(v3((float)-rx, 1.0f, (float)-rz));
 }
 
@@ -323,21 +336,60 @@ bool refract(V3 wo, V3 m, float eta_rel, V3& wi) {
     return true;
 }
 
-struct Mat {
+// One closure lobe: type (0 Lambert, 1 Schlick conductor, 2 rough dielectric,
+// 3 KpGloss), distribution, closure weight w, colour c (Lambert albedo or
+// conductor F0), roughness (ax, ay) and eta.
+struct Lobe {
     int type, dist;
-    V3 albedo, F0, Le;
-    float alpha, eta;
+    V3 w, c;
+    float ax, ay, eta;
 };
+
+// A material: up to 8 weighted lobes (CompositeClosure, bsdf.h:132-196) and
+// the emitted radiance; `dielectric` = a lobe transmits (the shading normal is
+// then not turned toward the viewer); guide_alpha = the smallest microfacet
+// roughness (guiding eligibility).
+struct Mat {
+    std::vector<Lobe> lobes;
+    V3 Le;
+    bool dielectric = false;
+    float guide_alpha = INFINITY;
+};
+
+Lobe load_lobe(const or_material& m, V3 w) {
+    Lobe l;
+    l.type = m.type;
+    l.dist = m.dist;
+    l.w = w;
+    l.c = m.type == 1 ? load3(m.F0) : load3(m.albedo);
+    l.ax = m.alpha;
+    l.ay = m.alpha_y > 0.0f ? m.alpha_y : m.alpha;
+    l.eta = m.eta;
+    return l;
+}
+
+void finish_mat(Mat& r) {
+    r.dielectric = false;
+    r.guide_alpha = INFINITY;
+    for (const Lobe& l : r.lobes) {
+        if (l.type == 2) r.dielectric = true;
+        if (l.type != 0) r.guide_alpha = std::min(r.guide_alpha, std::min(l.ax, l.ay));
+    }
+}
 
 Mat load_mat(const or_material& m) {
     Mat r;
-    r.type = m.type;
-    r.dist = m.dist;
-    r.albedo = load3(m.albedo);
-    r.F0 = load3(m.F0);
+    r.lobes.push_back(load_lobe(m, v3(1.0f, 1.0f, 1.0f)));
     r.Le = load3(m.Le);
-    r.alpha = m.alpha;
-    r.eta = m.eta;
+    finish_mat(r);
+    return r;
+}
+
+Mat load_composite(const or_lobe* lobes, uint32_t n, const float* Le) {
+    Mat r;
+    for (uint32_t i = 0; i < n; ++i) r.lobes.push_back(load_lobe(lobes[i].lobe, load3(lobes[i].weight)));
+    r.Le = Le ? load3(Le) : v3(0, 0, 0);
+    finish_mat(r);
     return r;
 }
 
@@ -347,25 +399,32 @@ V3 schlick(V3 F0, float c) {
     return v3(F0.x + (1.0f - F0.x) * k5, F0.y + (1.0f - F0.y) * k5, F0.z + (1.0f - F0.z) * k5);
 }
 
-// f (no cosine) and solid-angle pdf of wi given wo, in the local frame.
-void bsdf_eval(const Mat& m, V3 wo, V3 wi, V3& f, float& pdf) {
+// f (no cosine) and solid-angle pdf of wi given wo for one lobe, in the
+// local frame, without the lobe weight.
+void lobe_eval(const Lobe& m, V3 wo, V3 wi, V3& f, float& pdf) {
     f = v3(0, 0, 0);
     pdf = 0.0f;
     if (m.type == 0) {
         if (wo.y <= 0.0f || wi.y <= 0.0f) return;
-        f = m.albedo * (1.0f / kPi);
+        f = m.c * (1.0f / kPi);
         pdf = wi.y / kPi;
         return;
     }
-    if (m.type == 1) {
+    if (m.type == 1 || m.type == 3) {
+        // microfacet reflection: Schlick conductor (bsdfs.cpp:688-730) or
+        // KpGloss with dielectric Fresnel F(eta, m.wo) (bsdfs.cpp:359-395)
         if (wo.y <= 0.0f || wi.y <= 0.0f) return;
         V3 h = normalize(wo + wi);
-        double D = mf_D(m.dist, m.alpha, h);
-        double G = mf_G2(m.dist, m.alpha, wo, wi);
-        V3 F = schlick(m.F0, dot(wo, h));
+        double D = mf_D(m.dist, m.ax, m.ay, h);
+        double G = mf_G2(m.dist, m.ax, m.ay, wo, wi);
         float k = (float)(D * G / (4.0 * (double)wo.y * (double)wi.y));
-        f = F * k;
-        pdf = (float)(mf_G1(m.dist, m.alpha, wo) * D / (4.0 * (double)wo.y));
+        if (m.type == 1) {
+            f = schlick(m.c, dot(wo, h)) * k;
+        } else {
+            float F = (float)fresnel_dielectric(dot(wo, h), m.eta);
+            f = v3(F * k, F * k, F * k);
+        }
+        pdf = (float)(mf_G1(m.dist, m.ax, m.ay, wo) * D / (4.0 * (double)wo.y));
         return;
     }
     // rough dielectric, Walter et al. 2007 in the form of pbrt-v4's DielectricBxDF
@@ -380,9 +439,9 @@ void bsdf_eval(const Mat& m, V3 wo, V3 wi, V3& f, float& pdf) {
     if (wm.y < 0.0f) wm = -wm;
     if (dot(wm, wi) * ci < 0.0f || dot(wm, wo) * co < 0.0f) return;   // back-facing microfacet
     double F = fresnel_dielectric(dot(wo, wm), m.eta);
-    double D = mf_D(m.dist, m.alpha, wm);
-    double G = mf_G2(m.dist, m.alpha, wo, wi);
-    double pvn = mf_pdf_vndf(m.dist, m.alpha, wo, wm);
+    double D = mf_D(m.dist, m.ax, m.ay, wm);
+    double G = mf_G2(m.dist, m.ax, m.ay, wo, wi);
+    double pvn = mf_pdf_vndf(m.dist, m.ax, m.ay, wo, wm);
     if (refl) {
         double val = D * G * F / std::fabs(4.0 * (double)ci * (double)co);
         f = v3((float)val, (float)val, (float)val);
@@ -392,14 +451,15 @@ void bsdf_eval(const Mat& m, V3 wo, V3 wi, V3& f, float& pdf) {
         double denom = (dwi + dwo / etap);
         denom = denom * denom;
         double val = D * (1.0 - F) * G * std::fabs(dwi * dwo / (denom * (double)ci * (double)co));
-        val /= (double)etap * (double)etap;   // radiance transport
+        val /= (double)etap * (double)etap;   // radiance transport (D20)
         f = v3((float)val, (float)val, (float)val);
         pdf = (float)(pvn * std::fabs(dwi) / denom * (1.0 - F));
     }
 }
 
-// Samples wi; u5 picks reflection/refraction, (u6, u7) the direction.
-bool bsdf_sample(const Mat& m, V3 wo, float u5, float u6, float u7, V3& wi, V3& f, float& pdf) {
+// Samples wi for one lobe; u5 picks reflection/refraction, (u6, u7) the
+// direction.  f and pdf are the lobe's own (no weight).
+bool lobe_sample(const Lobe& m, V3 wo, float u5, float u6, float u7, V3& wi, V3& f, float& pdf) {
     f = v3(0, 0, 0);
     pdf = 0.0f;
     if (m.type == 0) {
@@ -408,20 +468,20 @@ bool bsdf_sample(const Mat& m, V3 wo, float u5, float u6, float u7, V3& wi, V3& 
         float phi = 2.0f * kPi * u6;
         wi = v3(r * std::cos(phi), std::sqrt(std::max(0.0f, 1.0f - u7)), r * std::sin(phi));
         if (wi.y <= 0.0f) return false;
-        f = m.albedo * (1.0f / kPi);
+        f = m.c * (1.0f / kPi);
         pdf = wi.y / kPi;
         return true;
     }
-    if (m.type == 1) {
+    if (m.type == 1 || m.type == 3) {
         if (wo.y <= 0.0f) return false;
-        V3 mm = mf_sample_vndf(m.dist, m.alpha, wo, u6, u7);
+        V3 mm = mf_sample_vndf(m.dist, m.ax, m.ay, wo, u6, u7);
         wi = reflect(wo, mm);
         if (wi.y <= 0.0f) return false;
-        bsdf_eval(m, wo, wi, f, pdf);
+        lobe_eval(m, wo, wi, f, pdf);
         return pdf > 0.0f;
     }
     if (wo.y == 0.0f) return false;
-    V3 mm = mf_sample_vndf(m.dist, m.alpha, wo, u6, u7);
+    V3 mm = mf_sample_vndf(m.dist, m.ax, m.ay, wo, u6, u7);
     double F = fresnel_dielectric(dot(wo, mm), m.eta);
     if (u5 < F) {
         wi = reflect(wo, mm);
@@ -432,7 +492,90 @@ bool bsdf_sample(const Mat& m, V3 wo, float u5, float u6, float u7, V3& wi, V3& 
         if (!refract(wo, mside, etap, wi)) return false;
         if (wo.y * wi.y >= 0.0f || wi.y == 0.0f) return false;
     }
-    bsdf_eval(m, wo, wi, f, pdf);
+    lobe_eval(m, wo, wi, f, pdf);
+    return pdf > 0.0f;
+}
+
+// Lobe selection probabilities at a vertex with throughput beta:
+// p_i = dot(w_i, beta) / sum_j dot(w_j, beta) (CompositeClosure::compute_pdfs,
+// bsdf.h:157-176, with the reference's integrator passing the throughput,
+// integrator.cpp:425); uniform if every product is 0.  One lobe: p = 1.
+struct LobeProbs {
+    float p[or_max_lobes];
+};
+
+LobeProbs lobe_probs(const Mat& m, V3 beta) {
+    LobeProbs r;
+    const uint32_t n = (uint32_t)m.lobes.size();
+    if (n == 1) {
+        r.p[0] = 1.0f;
+        return r;
+    }
+    float sum = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        r.p[i] = std::max(0.0f, dot(m.lobes[i].w, beta));
+        sum += r.p[i];
+    }
+    if (!(sum > 0.0f) || !std::isfinite(sum)) {
+        for (uint32_t i = 0; i < n; ++i) r.p[i] = 1.0f / (float)n;
+    } else {
+        const float inv = 1.0f / sum;
+        for (uint32_t i = 0; i < n; ++i) r.p[i] *= inv;
+    }
+    return r;
+}
+
+// Whole material: f = sum_i w_i f_i, pdf = sum_i p_i pdf_i (the one-sample
+// MIS combination of the lobes, bsdf.cpp:17-74; deviation D16: the lobe is
+// picked with p_i, not uniformly).
+void bsdf_eval(const Mat& m, const LobeProbs& lp, V3 wo, V3 wi, V3& f, float& pdf) {
+    const uint32_t n = (uint32_t)m.lobes.size();
+    if (n == 1) {
+        V3 fl;
+        lobe_eval(m.lobes[0], wo, wi, fl, pdf);
+        f = mul(m.lobes[0].w, fl);
+        return;
+    }
+    f = v3(0, 0, 0);
+    pdf = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 fl;
+        float pl;
+        lobe_eval(m.lobes[i], wo, wi, fl, pl);
+        f = f + mul(m.lobes[i].w, fl);
+        pdf = pdf + lp.p[i] * pl;
+    }
+}
+
+// Picks lobe i with probability p_i by u5 and re-uses u5 rescaled to
+// [0, 1) inside the lobe, samples it, and evaluates the whole material at
+// the sampled direction.
+bool bsdf_sample(const Mat& m, const LobeProbs& lp, V3 wo, float u5, float u6, float u7, V3& wi, V3& f,
+                 float& pdf) {
+    const uint32_t n = (uint32_t)m.lobes.size();
+    if (n == 1) {
+        V3 fl;
+        const bool ok = lobe_sample(m.lobes[0], wo, u5, u6, u7, wi, fl, pdf);
+        f = mul(m.lobes[0].w, fl);
+        return ok;
+    }
+    f = v3(0, 0, 0);
+    pdf = 0.0f;
+    uint32_t pick = n;
+    float lo = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (lp.p[i] > 0.0f && (u5 < lo + lp.p[i] || i == n - 1)) {
+            pick = i;
+            break;
+        }
+        lo += lp.p[i];
+    }
+    if (pick == n) return false;
+    const float ur = std::min((u5 - lo) / lp.p[pick], 0.99999994f);
+    V3 fl;
+    float pl;
+    if (!lobe_sample(m.lobes[pick], wo, std::max(ur, 0.0f), u6, u7, wi, fl, pl)) return false;
+    bsdf_eval(m, lp, wo, wi, f, pdf);
     return pdf > 0.0f;
 }
 
@@ -458,6 +601,8 @@ V3 offset_ray(V3 p, V3 n) {
 // ----------------------------------------------------------------- scene --
 struct or_scene {
     std::vector<float> tris;
+    std::vector<float> norms;        // 9 per triangle (vertex normals), empty = none
+    std::vector<uint8_t> has_norms;  // per triangle
     std::vector<int32_t> tri_mat, tri_light;
     std::vector<Mat> mats;
     struct Light {
@@ -574,6 +719,24 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
         V3 ng = normalize(cross(v1 - v0, v2 - v0));
         float b0 = 1.0f - h.u - h.v;
         V3 P = v0 * b0 + v1 * h.u + v2 * h.v;
+        V3 ns = ng;
+        if (!s.has_norms.empty() && s.has_norms[h.prim]) {
+            // TriangleMesh::post_hit (shape.cpp:615-654): interpolated shading
+            // normal, and the shading point moved off the flat triangle by the
+            // method of Hanika, "Hacking the Shadow Terminator" (2021): each
+            // vertex's offset to P, projected onto the tangent plane of its
+            // normal where P lies below it, blended barycentrically.  World
+            // space throughout (the reference dots object-space normals with
+            // object-space offsets, identical for identity transforms: D21).
+            const float* nv = &s.norms[9 * (size_t)h.prim];
+            V3 n0 = load3(nv), n1 = load3(nv + 3), n2 = load3(nv + 6);
+            ns = normalize(n0 * b0 + n1 * h.u + n2 * h.v);
+            V3 tu = P - v0, tw = P - v1, tx = P - v2;
+            tu = tu - n0 * std::min(0.0f, dot(tu, n0));
+            tw = tw - n1 * std::min(0.0f, dot(tw, n1));
+            tx = tx - n2 * std::min(0.0f, dot(tx, n2));
+            P = P + (tu * b0 + tw * h.u + tx * h.v);
+        }
         const Mat& m = s.mats[s.tri_mat[h.prim]];
         float cos_hit = dot(dir, ng);
         if (!is_zero(m.Le) && cos_hit < 0.0f) {
@@ -593,10 +756,13 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
         const int draws = g ? 12 : 8;
         for (int i = 0; i < draws; ++i) u[i] = rng.nextf();
         V3 wo_w = -dir;
-        V3 n = (m.type == 2) ? ng : (dot(wo_w, ng) < 0.0f ? -ng : ng);
+        // two-sided closures see the side the path arrives on; a transmitting
+        // material keeps the normals' orientation (inside / outside)
+        const V3 n = (!m.dielectric && dot(wo_w, ng) < 0.0f) ? -ns : ns;
         Frame fr(n);
         V3 wo = fr.to_local(wo_w);
-        const bool guided = g && g->sample && (m.type == 0 || (m.type == 1 && m.alpha >= g->min_alpha));
+        const LobeProbs lp = lobe_probs(m, beta);
+        const bool guided = g && g->sample && !m.dielectric && m.guide_alpha >= g->min_alpha;
         double gout[32];
         if (guided) {
             const float in6[6] = {(P.x - g->bmin.x) * g->binv.x, (P.y - g->bmin.y) * g->binv.y,
@@ -628,7 +794,7 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
                     V3 wi = fr.to_local(wi_w);
                     V3 f;
                     float pdf_b;
-                    bsdf_eval(m, wo, wi, f, pdf_b);
+                    bsdf_eval(m, lp, wo, wi, f, pdf_b);
                     if (!is_zero(f)) {
                         float pdf_l = (1.0f / (float)nl) * (1.0f / lt.area) * (dist * dist) / cos_l;
                         if (guided) pdf_b = g->frac * pdf_b + (1.0f - g->frac) * guide_pdf(wi_w);
@@ -660,11 +826,11 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
             const V3 ww = v3(wd[0], wd[1], wd[2]);
             wi = fr.to_local(ww);
             float pdf_bsdf;
-            bsdf_eval(m, wo, wi, f, pdf_bsdf);
+            bsdf_eval(m, lp, wo, wi, f, pdf_bsdf);
             pdf = g->frac * pdf_bsdf + (1.0f - g->frac) * guide_pdf(ww);
             if (is_zero(f) || !(pdf > 0.0f)) break;
         } else {
-            if (!bsdf_sample(m, wo, u[5], u[6], u[7], wi, f, pdf)) break;
+            if (!bsdf_sample(m, lp, wo, u[5], u[6], u[7], wi, f, pdf)) break;
             if (guided) pdf = g->frac * pdf + (1.0f - g->frac) * guide_pdf(fr.to_world(wi));
         }
         beta = mul(beta, f) * (std::fabs(wi.y) / pdf);
@@ -684,7 +850,7 @@ extern "C" {
 void or_bsdf_eval(const or_material* m, const float* wo, const float* wi, float* f, float* pdf) {
     Mat mm = load_mat(*m);
     V3 ff;
-    bsdf_eval(mm, load3(wo), load3(wi), ff, *pdf);
+    bsdf_eval(mm, lobe_probs(mm, v3(1, 1, 1)), load3(wo), load3(wi), ff, *pdf);
     f[0] = ff.x; f[1] = ff.y; f[2] = ff.z;
 }
 
@@ -692,16 +858,16 @@ int or_bsdf_sample(const or_material* m, const float* wo, const float* u3, float
                    float* pdf) {
     Mat mm = load_mat(*m);
     V3 w, ff;
-    bool ok = bsdf_sample(mm, load3(wo), u3[0], u3[1], u3[2], w, ff, *pdf);
+    bool ok = bsdf_sample(mm, lobe_probs(mm, v3(1, 1, 1)), load3(wo), u3[0], u3[1], u3[2], w, ff, *pdf);
     wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
     f[0] = ff.x; f[1] = ff.y; f[2] = ff.z;
     return ok ? 1 : 0;
 }
 
-double or_mf_D(int dist, float alpha, const float* m) { return mf_D(dist, alpha, load3(m)); }
-double or_mf_lambda(int dist, float alpha, const float* w) { return mf_lambda(dist, alpha, load3(w)); }
-void or_mf_sample_vndf(int dist, float alpha, const float* wo, const float* u2, float* m) {
-    V3 r = mf_sample_vndf(dist, alpha, load3(wo), u2[0], u2[1]);
+double or_mf_D(int dist, float ax, float ay, const float* m) { return mf_D(dist, ax, ay, load3(m)); }
+double or_mf_lambda(int dist, float ax, float ay, const float* w) { return mf_lambda(dist, ax, ay, load3(w)); }
+void or_mf_sample_vndf(int dist, float ax, float ay, const float* wo, const float* u2, float* m) {
+    V3 r = mf_sample_vndf(dist, ax, ay, load3(wo), u2[0], u2[1]);
     m[0] = r.x; m[1] = r.y; m[2] = r.z;
 }
 double or_fresnel_dielectric(double cos_i, double eta) { return fresnel_dielectric(cos_i, eta); }
@@ -709,30 +875,47 @@ double or_erfinv(double x) { return erfinv_d(x); }
 
 void or_bsdf_eval_batch(const or_material* m, const float* wo, const float* wi, int n, float* f,
                         float* pdf) {
-    Mat mm = load_mat(*m);
-    for (int i = 0; i < n; ++i) {
-        V3 ff;
-        bsdf_eval(mm, load3(wo + 3 * i), load3(wi + 3 * i), ff, pdf[i]);
-        f[3 * i] = ff.x; f[3 * i + 1] = ff.y; f[3 * i + 2] = ff.z;
-    }
+    or_composite_eval_batch(nullptr, m, 1, nullptr, wo, wi, n, f, pdf);
 }
 
 void or_bsdf_sample_batch(const or_material* m, const float* wo, const float* u3, int n, float* wi,
                           float* f, float* pdf, int32_t* ok) {
-    Mat mm = load_mat(*m);
+    or_composite_sample_batch(nullptr, m, 1, nullptr, wo, u3, n, wi, f, pdf, ok);
+}
+
+static Mat composite_or_single(const or_lobe* lobes, const or_material* single, uint32_t n) {
+    return lobes ? load_composite(lobes, n, nullptr) : load_mat(*single);
+}
+
+void or_composite_eval_batch(const or_lobe* lobes, const or_material* single, uint32_t n_lobes,
+                             const float* beta, const float* wo, const float* wi, int n, float* f, float* pdf) {
+    const Mat mm = composite_or_single(lobes, single, n_lobes);
+    const LobeProbs lp = lobe_probs(mm, beta ? load3(beta) : v3(1, 1, 1));
+    for (int i = 0; i < n; ++i) {
+        V3 ff;
+        bsdf_eval(mm, lp, load3(wo + 3 * i), load3(wi + 3 * i), ff, pdf[i]);
+        f[3 * i] = ff.x; f[3 * i + 1] = ff.y; f[3 * i + 2] = ff.z;
+    }
+}
+
+void or_composite_sample_batch(const or_lobe* lobes, const or_material* single, uint32_t n_lobes,
+                               const float* beta, const float* wo, const float* u3, int n, float* wi,
+                               float* f, float* pdf, int32_t* ok) {
+    const Mat mm = composite_or_single(lobes, single, n_lobes);
+    const LobeProbs lp = lobe_probs(mm, beta ? load3(beta) : v3(1, 1, 1));
     for (int i = 0; i < n; ++i) {
         V3 w, ff;
-        ok[i] = bsdf_sample(mm, load3(wo + 3 * i), u3[3 * i], u3[3 * i + 1], u3[3 * i + 2], w, ff, pdf[i]);
+        ok[i] = bsdf_sample(mm, lp, load3(wo + 3 * i), u3[3 * i], u3[3 * i + 1], u3[3 * i + 2], w, ff, pdf[i]);
         wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
         f[3 * i] = ff.x; f[3 * i + 1] = ff.y; f[3 * i + 2] = ff.z;
     }
 }
 
-void or_mf_eval_batch(int dist, float alpha, const float* w, int n, double* D, double* lambda) {
+void or_mf_eval_batch(int dist, float ax, float ay, const float* w, int n, double* D, double* lambda) {
     for (int i = 0; i < n; ++i) {
         V3 v = load3(w + 3 * i);
-        D[i] = mf_D(dist, alpha, v);
-        lambda[i] = mf_lambda(dist, alpha, v);
+        D[i] = mf_D(dist, ax, ay, v);
+        lambda[i] = mf_lambda(dist, ax, ay, v);
     }
 }
 
@@ -764,6 +947,23 @@ or_scene* or_scene_create(const float* tris, uint32_t n_tris, const int32_t* tri
     }
     s->bvh = or_bvh_build(tris, n_tris);
     return s;
+}
+
+int or_scene_set_composite(or_scene* s, uint32_t material, const or_lobe* lobes, uint32_t n_lobes,
+                           const float* Le) {
+    if (!s || material >= s->mats.size() || !lobes || n_lobes == 0 || n_lobes > or_max_lobes) return 1;
+    s->mats[material] = load_composite(lobes, n_lobes, Le);
+    for (auto& l : s->lights)
+        if (!l.tris.empty() && (uint32_t)s->tri_mat[l.tris[0]] == material) l.Le = s->mats[material].Le;
+    return 0;
+}
+
+int or_scene_set_normals(or_scene* s, const float* normals9, const uint8_t* has) {
+    if (!s || !normals9 || !has) return 1;
+    const size_t n = s->tri_mat.size();
+    s->norms.assign(normals9, normals9 + 9 * n);
+    s->has_norms.assign(has, has + n);
+    return 0;
 }
 
 void or_scene_free(or_scene* s) {

@@ -27,9 +27,7 @@ def rel_mse(img, ref):
 
 
 def oracle_scene(s):
-    mats = [oracle.material(m["type"], m["dist"], m["albedo"], m["F0"], m["alpha"], m["eta"], m["Le"])
-            for m in s.materials]
-    return oracle.Scene(s.soup(), s.tri_material(), s.tri_light(), mats)
+    return oracle.scene_from(s)
 
 
 def desc(s, spp, seed):

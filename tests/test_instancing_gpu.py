@@ -263,9 +263,7 @@ def test_instanced_render_matches_oracle_statistically():
     s = instanced_room(spp=16, size=24)
     img = render(s, 256, native.INSTANCES_TWO_LEVEL)
     f = flattened(s)
-    mats = [oracle.material(m["type"], m["dist"], m["albedo"], m["F0"], m["alpha"], m["eta"], m["Le"])
-            for m in f.materials]
-    sc = oracle.Scene(f.soup(), f.tri_material(), f.tri_light(), mats)
+    sc = oracle.scene_from(f)
     d = oracle.RenderDesc()
     d.width, d.height, d.spp, d.max_depth, d.rr_depth, d.seed = f.width, f.height, 256, f.max_depth, f.rr_depth, 99
     d.eye[:], d.lookat[:], d.up[:] = f.eye, f.lookat, f.up

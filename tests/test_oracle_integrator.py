@@ -9,9 +9,7 @@ from kazen_proto_amd import scenes
 
 
 def oracle_scene(s):
-    mats = [oracle.material(m["type"], m["dist"], m["albedo"], m["F0"], m["alpha"], m["eta"], m["Le"])
-            for m in s.materials]
-    return oracle.Scene(s.soup(), s.tri_material(), s.tri_light(), mats)
+    return oracle.scene_from(s)
 
 
 def desc(s, spp=None, spp_offset=0):
@@ -120,3 +118,34 @@ def test_guided_network_changes_paths_not_the_mean():
     for img in (a, b):
         m, e = mean_err(img)
         assert abs(m - mp) < 4 * np.hypot(e, ep), (m, mp, e, ep)
+
+
+@pytest.mark.parametrize("rr", [64, 2])
+def test_composite_furnace_closed_form(rr):
+    # a composite of two Lambert lobes (weights 0.3 and 0.4, albedos 1 and
+    # 0.5) scatters like Lambert(0.5): the furnace closed form holds whatever
+    # lobe-pick probabilities the throughput gives; per-channel weights make
+    # the channels differ (albedo 0.5 / 0.5 / 0.35)
+    s = scenes.furnace_box(albedo=0.5, Le=1.0, max_depth=4, rr_depth=rr, spp=64)
+    s.materials[0] = scenes.composite(
+        [((0.3, 0.3, 0.2), scenes.material(scenes.LAMBERT, albedo=(1, 1, 1))),
+         ((0.4, 0.4, 0.3), scenes.material(scenes.LAMBERT, albedo=(0.5, 0.5, 0.5)))], Le=(1, 1, 1))
+    img = oracle_scene(s).render(desc(s))
+    for ch, alb in enumerate((0.5, 0.5, 0.35)):
+        pix = img[..., ch].ravel()
+        err = pix.std() / np.sqrt(pix.size)
+        expect = scenes.furnace_expected(alb, 1.0, 4)
+        assert abs(pix.mean() - expect) < 5 * err + 1e-9, (ch, pix.mean(), expect, err)
+
+
+def test_geometric_shading_normals_change_nothing():
+    # vertex normals equal to the geometric normal: the interpolated normal is
+    # the geometric one and the shading point does not move
+    s = scenes.furnace_box(albedo=0.6, Le=1.0, max_depth=3, spp=16)
+    plain = oracle_scene(s).render(desc(s))
+    t = s.meshes[0].reshape(-1, 3, 3).astype(np.float64)
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    s.mesh_normals = [np.repeat(n[:, None, :], 3, axis=1).reshape(-1, 9).astype(np.float32)]
+    smooth = oracle_scene(s).render(desc(s))
+    assert np.allclose(plain, smooth, rtol=1e-5, atol=1e-6)

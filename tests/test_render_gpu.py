@@ -24,9 +24,7 @@ def rel_mse(img, ref):
 
 
 def oracle_scene(s):
-    mats = [oracle.material(m["type"], m["dist"], m["albedo"], m["F0"], m["alpha"], m["eta"], m["Le"])
-            for m in s.materials]
-    return oracle.Scene(s.soup(), s.tri_material(), s.tri_light(), mats)
+    return oracle.scene_from(s)
 
 
 def odesc(s, spp=None, seed=None, spp_offset=0):
@@ -117,6 +115,33 @@ def test_all_closures_match_oracle_statistics():
     c.close()
     e, e0 = rel_mse(img, ref), rel_mse(indep, ref)
     print("materials 256 spp relMSE gpu %.3e oracle %.3e" % (e, e0))
+    assert e < 1.6 * e0
+
+
+def test_composite_anisotropic_gloss_and_smooth_normals_per_path():
+    # composite closures, anisotropic microfacets, gloss and shading normals
+    # with the shading-point offset, path by path against the oracle
+    s = scenes.closures_scene(width=48, height=48)
+    frac, g, o = _per_path(s)
+    print("closures: %.4f of pixels agree path by path" % frac)
+    assert frac >= 0.999
+    # the scene's features change the image: flat normals / single lobes differ
+    flat = scenes.closures_scene(width=48, height=48)
+    flat.mesh_normals = [None] * len(flat.meshes)
+    fo = oracle_scene(flat).render(odesc(flat, spp=1), threads=CPU_THREADS)
+    assert np.mean(np.abs(fo - o) > 1e-3) > 0.02
+
+
+def test_composite_anisotropic_gloss_and_smooth_normals_statistics():
+    s = scenes.closures_scene(width=32, height=32)
+    osc = oracle_scene(s)
+    ref = osc.render(odesc(s, spp=4096, seed=910), threads=CPU_THREADS)
+    indep = osc.render(odesc(s, spp=256, seed=911), threads=CPU_THREADS)
+    c = gpu_ctx(s)
+    img = c.render(gdesc(s, spp=256, seed=912))
+    c.close()
+    e, e0 = rel_mse(img, ref), rel_mse(indep, ref)
+    print("closures 256 spp relMSE gpu %.3e oracle %.3e" % (e, e0))
     assert e < 1.6 * e0
 
 
