@@ -40,6 +40,24 @@ def profiled_traffic():
 BASELINE_METRIC = "incoherent Mrays/s (1 GPU); equal-relMSE time, guided vs plain GPU tracer"
 
 
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs, the CPUs this process
+    may use, the CPU share the environment grants (OMP_NUM_THREADS: one GPU's
+    share of a shared box) and the CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS")
+    share = int(share) if share and share.isdigit() and int(share) > 0 else None
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_share": share, "model": model}
+
+
 def rel_mse(img, ref):
     """mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)"""
     img = img.astype(np.float64)
@@ -190,15 +208,7 @@ def batch_leg(args):
     graph) after autotuning, and for comparison one scene after another."""
     from kazen_proto_amd import native, scenes
     t0 = time.time()
-    rng = np.random.default_rng(100)
-    ss = []
-    for i in range(args.batch_scenes):
-        target = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
-        grid = max(16, int(round(np.sqrt(max(target - 16 * 240, 1000) / 2.0))) + 1)
-        res = int(rng.choice([128, 256, 512, 1024]))
-        spp = int(rng.choice([4, 16, 64]))
-        ss.append(scenes.terrain_scene(grid=grid, n_spheres=16, sphere_lon=16, sphere_lat=8, width=res,
-                                       height=res, spp=spp, max_depth=8, seed=100 + i))
+    ss = scenes.batch_scenes(args.batch_scenes, seed=100)
     gen = time.time() - t0
     print("[bench] batch: %d scenes generated (%.1fs)" % (len(ss), gen), file=sys.stderr, flush=True)
     owner = native.Context(0)
@@ -276,7 +286,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = every CPU this process may use, capped by the environment's CPU share "
+                         "(OMP_NUM_THREADS) where one is set")
     ap.add_argument("--skip-primary", action="store_true",
                     help="launch only the incoherent set (profiling runs: one kernel population)")
     ap.add_argument("--leaf", type=int, default=4)
@@ -299,12 +311,12 @@ def main():
     ap.add_argument("--guiding-ref-spp", type=int, default=16384)
     ap.add_argument("--guiding-target", type=float, default=0.01)
     ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
-    ap.add_argument("--guiding-lr", type=float, default=2e-3)
+    ap.add_argument("--guiding-lr", type=float, default=1e-3)
     ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
     ap.add_argument("--guiding-pass-paths", type=int, default=1 << 24,
                     help="paths per guided pass (each pass trains once)")
-    ap.add_argument("--guiding-train-passes", type=int, default=64,
-                    help="passes that train the network before it is frozen (0 = all)")
+    ap.add_argument("--guiding-train-passes", type=int, default=0,
+                    help="passes that train the network before it is frozen (0 = all: online)")
     ap.add_argument("--guiding-after", type=int, default=2, help="training passes before guided sampling")
     ap.add_argument("--seed-guide", type=int, default=17)
     args = ap.parse_args()
@@ -379,23 +391,39 @@ def main():
     print("[bench] incoherent %.3f ms/launch %.1f Mrays/s; primary %s Mrays/s"
           % (ms_inc, mrays_inc, mrays_prim), file=sys.stderr, flush=True)
 
-    # algorithmic bytes per ray from the instrumented launch (untimed)
+    # algorithmic bytes per ray (untimed): B_ray = 32 (ray) + 20 (hit) +
+    # 128 n_node + 48 n_tri, with the node / triangle counts of the CPU
+    # oracle's traversal of the identical BVH8 on a sample of the rays; the
+    # kernel's own counts (its leaf postponing visits a few more nodes) beside
     _, nv, nt = ctx.intersect_closest(d_inc, out=h_inc, counted=True)
     n = inc.shape[0]
-    nodes_per_ray, tris_per_ray = nv / n, nt / n
-    b_ray = 32 + 16 + 128 * nodes_per_ray + 48 * tris_per_ray
+    k_nodes, k_tris = nv / n, nt / n
+    o_nodes, o_tris, o_sample = k_nodes, k_tris, 0
+    try:
+        import oracle
+        nodes8, tris12, _ = native.bvh8_build_host(tris, leaf_size=args.leaf)
+        o_sample = min(n, 262144)
+        _, onv, ont = oracle.bvh8_closest(nodes8, tris12, inc[:o_sample])
+        o_nodes, o_tris = onv / o_sample, ont / o_sample
+    except Exception as e:   # the oracle is a checker: without it the kernel's counts stand in
+        print("[bench] oracle node counts unavailable: %s" % e, file=sys.stderr, flush=True)
+    b_ray = 32 + 20 + 128 * o_nodes + 48 * o_tris
     achieved = b_ray * n / (ms_inc * 1e-3) / 1e9
+    traffic = profiled_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/closest_incoherent_microbench.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
-                "bytes_per_ray": round(b_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
-                "tris_per_ray": round(tris_per_ray, 3)}
+                "dram_gbs": None if traffic is None else round(traffic / (ms_inc * 1e-3) / 1e9, 1),
+                "bytes_per_ray": round(b_ray, 1), "nodes_per_ray": round(o_nodes, 3),
+                "tris_per_ray": round(o_tris, 3), "counts_source": "oracle BVH8 traversal of %d rays" % o_sample,
+                "kernel_nodes_per_ray": round(k_nodes, 3), "kernel_tris_per_ray": round(k_tris, 3)}
 
     cpu = None
     parity = None
     if not args.no_cpu:
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        host = host_info()
+        threads = args.cpu_threads or min(host["affinity_cpus"], host["cpu_share"] or host["affinity_cpus"])
         bvh = oracle.Bvh(tris)
         bvh.closest(inc[:65536], threads=threads)           # warm-up
         runs = []
@@ -407,10 +435,14 @@ def main():
         cpu = {"value": round(inc.shape[0] / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                "kind": "port",
                "sample": "all %d incoherent rays, oracle binary-SAH BVH + watertight test, "
-                         "median of 3 runs (%.2fs each)" % (inc.shape[0], sec)}
+                         "median of 3 runs (%.2fs each)" % (inc.shape[0], sec),
+               "host": host, "gpu_over_cpu": round(mrays_inc / (inc.shape[0] / sec / 1e6), 1)}
         got = h_inc.numpy()
         mm = int(np.count_nonzero(np.any(got.view(np.uint32) != hits_cpu.view(np.uint32), axis=1)))
-        parity = {"incoherent_mismatches": mm, "rays": int(n)}
+        ties = bvh.count_near_ties(inc, hits_cpu, threads=threads)
+        parity = {"incoherent_mismatches": mm, "rays": int(n), "near_ties": ties,
+                  "near_ties_note": "rays whose closest hit has a second primitive within 1 ulp of t "
+                                    "(resolved by the lowest primitive id on both sides)"}
 
     path_trace = None
     if not args.no_pathtrace:

@@ -12,7 +12,11 @@ AcceleratorType / IntegratorType enums.
 Differences: only the path integrator exists (the others raise); OSL shader
 groups are reduced to the integrator's closure table by shader name, as in
 scene_xml; the film is written as PFM (the reference writes through OIIO);
-one progress callback per chunk of samples, with the whole film as the tile.
+one progress callback per chunk of samples, with the whole film as the tile;
+a mesh emits (and is sampled as an area light) exactly when its shader group
+emits, whatever is_light says (a mismatch is listed in the scene's
+warnings).  The vector and matrix types of pyzen.vec / pyzen.mat are in
+pyzen_types; `import kazen_proto_amd.pyzen as pyzen` gives all three.
 """
 import enum
 import math
@@ -70,14 +74,37 @@ class Film:
             write_pfm(self.filename, self.image)
 
 
+def _matrix(world):
+    if world is None:
+        return None
+    m = world.as_array() if hasattr(world, "as_array") else np.asarray(world, np.float64)
+    return m.reshape(4, 4)
+
+
 def _apply(world, tris):
     """Row-major 4x4 object-to-world matrix applied to [n, 9] vertices."""
-    if world is None:
+    m = _matrix(world)
+    if m is None:
         return np.asarray(tris, np.float32)
-    m = np.asarray(world, np.float64).reshape(4, 4)
     v = np.asarray(tris, np.float64).reshape(-1, 3)
     v = v @ m[:3, :3].T + m[:3, 3]
     return v.reshape(-1, 9).astype(np.float32)
+
+
+def _apply_normals(world, n9):
+    """Normals [n, 9] under the inverse transpose of the matrix, unit length."""
+    m = _matrix(world)
+    v = np.asarray(n9, np.float64).reshape(-1, 3)
+    if m is not None:
+        v = v @ np.linalg.inv(m[:3, :3])
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).reshape(-1, 9).astype(np.float32)
+
+
+def _rows(seq, n):
+    """A list of pyzen vectors (or nested sequences / arrays) -> [k, n] array."""
+    if hasattr(seq, "__len__") and len(seq) and hasattr(seq[0], "v"):
+        return np.asarray([x.v for x in seq], np.float64).reshape(-1, n)
+    return np.asarray(seq, np.float64).reshape(-1, n)
 
 
 class Scene:
@@ -96,10 +123,14 @@ class Scene:
         self.accelerator = AcceleratorType.HIP
         self.integrator = IntegratorType.PathIntegrator
         self._built = False
+        self._names = []      # per mesh of the description: its name
+        self._lights = []     # per mesh: is_light as given
 
     # ---- files and settings
     def parse_from_file(self, file_path):
         self._s = scene_xml.load_xml_scene(str(file_path))
+        self._names = [""] * len(self._s.meshes)
+        self._lights = [any(x != 0.0 for x in self._s.materials[m]["Le"]) for m in self._s.mesh_material]
         self._built = False
 
     def set_film(self, w, h, xres=None, yres=None, f=""):
@@ -123,7 +154,7 @@ class Scene:
 
     # ---- shader groups (reduced to the closure table)
     def set_shader_search_path(self, path):
-        pass
+        self._search = str(path)
 
     def begin_shader_group(self, name):
         self._group, self._group_shader, self._group_params = name, None, {}
@@ -153,8 +184,17 @@ class Scene:
 
     def end_shader_group(self):
         if self._group is not None:
-            self._groups[self._group] = scene_xml._closure(self._group_shader or "", self._group_params,
-                                                           self._s.warnings, self._group)
+            name = self._group_shader or ""
+            osl = os.path.join(getattr(self, "_search", ""), name + ".osl")
+            m = None
+            if getattr(self, "_search", None) and os.path.exists(osl):
+                try:
+                    m = scene_xml.shader_closure(osl, self._group_params, self._s.warnings, self._group)
+                except ValueError as e:
+                    self._s.warnings.append("%s: %s; mapped by shader name" % (self._group, e))
+            if m is None:
+                m = scene_xml._closure(name, self._group_params, self._s.warnings, self._group)
+            self._groups[self._group] = m
         self._group = None
 
     def _material(self, shader_name):
@@ -171,11 +211,15 @@ class Scene:
     def add_sphere(self, world, p, r, name="", shader_name="", is_light=False):
         t = scenes.uv_sphere(64, 32, center=tuple(p), radius=float(r))
         self._s.add_mesh(_apply(world, t), self._material(shader_name))
+        self._names.append(str(name))
+        self._lights.append(bool(is_light))
         self._built = False
 
     def add_triangle(self, world, a, b, c, name="", shader_name="", is_light=False):
         t = np.asarray([tuple(a) + tuple(b) + tuple(c)], np.float32)
         self._s.add_mesh(_apply(world, t), self._material(shader_name))
+        self._names.append(str(name))
+        self._lights.append(bool(is_light))
         self._built = False
 
     def add_quad(self, world, c, d, u, w, h, name="", shader_name="", is_light=False):
@@ -188,16 +232,56 @@ class Scene:
         if np.dot(np.cross(p[1] - p[0], p[2] - p[0]), d) < 0:
             t = scenes.quad(p[0], p[3], p[2], p[1])
         self._s.add_mesh(_apply(world, t), self._material(shader_name))
+        self._names.append(str(name))
+        self._lights.append(bool(is_light))
         self._built = False
 
     def add_mesh(self, world, vs, ns, ts, idx, name="", shader_name="", is_light=False):
-        vs = np.asarray(vs, np.float32).reshape(-1, 3)
-        idx = np.asarray(idx, np.int64).reshape(-1, 3)
+        """TriangleMesh of vertices vs, optional vertex normals ns (one per
+        vertex: smooth shading), uvs ts (unused by the integrator) and faces
+        idx (src/core/scene.h:144-160).  Faces that name missing vertices are
+        an error when the BVH is built, as the reference's accelerator would
+        read past the vertex array."""
+        vs = _rows(vs, 3)
+        idx = _rows(idx, 3).astype(np.int64)
+        ns = _rows(ns, 3) if ns is not None and len(ns) else None
+        shader = shader_name if isinstance(shader_name, str) else ""
+        if idx.size and (idx.min() < 0 or idx.max() >= vs.shape[0]):
+            self._pending_error = "add_mesh %r: a face names a vertex beyond the %d given" % (name, vs.shape[0])
+            return None
         t = np.concatenate([vs[idx[:, 0]], vs[idx[:, 1]], vs[idx[:, 2]]], axis=1)
-        self._s.add_mesh(_apply(world, t), self._material(shader_name))
+        normals = None
+        if ns is not None and ns.shape[0] == vs.shape[0]:
+            normals = _apply_normals(world, np.concatenate([ns[idx[:, 0]], ns[idx[:, 1]], ns[idx[:, 2]]], axis=1))
+        mi = self._material(shader)
+        emits = any(x != 0.0 for x in self._s.materials[mi]["Le"])
+        if bool(is_light) != emits:
+            self._s.warnings.append("mesh %r: is_light=%s but its shader group %s" % (
+                name, bool(is_light), "emits" if emits else "emits nothing"))
+        self._s.add_mesh(_apply(world, t), mi, normals=normals)
+        self._names.append(str(name))
+        self._lights.append(bool(is_light))
         self._built = False
+        return None
 
     def build_bvh(self, names=None):
+        """Accelerator::build(names) (src/core/accel.h:32): with names, only
+        the meshes of those names stay in the scene."""
+        err = getattr(self, "_pending_error", None)
+        if err:
+            raise ValueError(err)
+        if names:
+            keep = set(names)
+            unknown = keep - set(self._names)
+            if unknown:
+                raise ValueError("build_bvh: unknown mesh names %s" % sorted(unknown))
+            sel = [i for i, n in enumerate(self._names) if n in keep]
+            s = self._s
+            s.meshes = [s.meshes[i] for i in sel]
+            s.mesh_material = [s.mesh_material[i] for i in sel]
+            s.mesh_normals = [s.mesh_normals[i] if i < len(s.mesh_normals) else None for i in sel]
+            self._names = [self._names[i] for i in sel]
+            self._lights = [self._lights[i] for i in sel]
         self._built = True
 
     @property

@@ -399,6 +399,23 @@ def closures_scene(width=32, height=32, spp=16, max_depth=6, rr_depth=3, seed=13
     return s
 
 
+def batch_scenes(n=32, seed=100):
+    """The batched-workload scenes: n terrain-style scenes with seeds seed ..
+    seed + n - 1, triangle counts log-uniform in [1e4, 2e6], resolutions from
+    {128^2, 256^2, 512^2, 1024^2}, spp from {4, 16, 64}, depth 8 (the scene
+    sizes and settings are drawn from one generator seeded with `seed`)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        target = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        grid = max(16, int(round(np.sqrt(max(target - 16 * 240, 1000) / 2.0))) + 1)
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.choice([4, 16, 64]))
+        out.append(terrain_scene(grid=grid, n_spheres=16, sphere_lon=16, sphere_lat=8, width=res,
+                                 height=res, spp=spp, max_depth=8, seed=seed + i))
+    return out
+
+
 def orient_toward(tris9, point):
     """Flips triangles so their winding normals face `point` (inward for a
     closed mesh around it)."""

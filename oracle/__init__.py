@@ -81,6 +81,7 @@ def lib():
             "or_bvh_node_count": (_u32, [_p]),
             "or_bvh_closest": (None, [_p, _p, _u32, _p, _i32]),
             "or_bvh_occluded": (None, [_p, _p, _u32, _p, _i32]),
+            "or_bvh_count_near_ties": (_u32, [_p, _p, _u32, _p, _i32]),
             "or_bvh_closest_timed": (ctypes.c_double, [_p, _p, _u32, _p, _i32]),
             "or_hardware_threads": (_i32, []),
             "or_bvh8_closest": (None, [_p, _u64, _p, _u64, _p, _u32, _p, _p, _p, _i32]),
@@ -109,6 +110,7 @@ def lib():
             "or_scene_create": (_p, [_p, _u32, _p, _p, _p, _u32]),
             "or_scene_free": (None, [_p]),
             "or_render": (_i32, [_p, _p, _p, _p, _i32]),
+            "or_render_pixels": (_i32, [_p, _p, _p, _u32, _p, _i32]),
             "or_integrate_rays": (_i32, [_p, _p, _u32, _p, _p, _i32]),
             "or_render_guided": (_i32, [_p, _p, _p, _p, _p, _i32]),
             "or_instanced_closest": (None, [_p, _p, _p, _p, _p, _p, _u32, _p, _u32, _p, _i32]),
@@ -197,6 +199,12 @@ class Bvh:
         hits = np.zeros((rays.shape[0], 4), np.float32)
         sec = lib().or_bvh_closest_timed(self._h, _ptr(rays), rays.shape[0], _ptr(hits), threads)
         return hits, sec
+
+    def count_near_ties(self, rays, hits, threads=0):
+        """Rays whose closest hit has a second primitive within 1 ulp of t."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.ascontiguousarray(hits, np.float32).reshape(-1, 4)
+        return int(lib().or_bvh_count_near_ties(self._h, _ptr(rays), rays.shape[0], _ptr(hits), threads))
 
     def occluded(self, rays, threads=0):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -446,6 +454,15 @@ class Scene:
         if rc != 0:
             raise RuntimeError("or_render failed (%d)" % rc)
         return (out, s0) if want_sample0 else out
+
+    def render_pixels(self, desc, pixels, threads=0):
+        """Mean radiance [n, 3] (FP64) of the listed row-major pixel indices."""
+        px = np.ascontiguousarray(pixels, np.uint32)
+        out = np.zeros((px.size, 3), np.float64)
+        rc = lib().or_render_pixels(self._h, ctypes.byref(desc), _ptr(px), px.size, _ptr(out), threads)
+        if rc != 0:
+            raise RuntimeError("or_render_pixels failed (%d)" % rc)
+        return out
 
     def integrate_rays(self, rays, desc, threads=0, guide=None):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

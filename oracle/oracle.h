@@ -74,6 +74,10 @@ uint32_t or_bvh_node_count(const or_bvh* b);
 /* Closest hit through the oracle BVH, threaded over rays. */
 void or_bvh_closest(const or_bvh* b, const float* rays, uint32_t n_rays,
                     float* hits, int threads);
+/* Rays whose closest hit (hits from or_bvh_closest) has another primitive
+ * within 1 ulp of its t: the near ties of the closest-hit parity report. */
+uint32_t or_bvh_count_near_ties(const or_bvh* b, const float* rays, uint32_t n_rays,
+                                const float* hits, int threads);
 /* Occlusion (any hit in [tmin,tmax]) -> out[i] in {0,1}. */
 void or_bvh_occluded(const or_bvh* b, const float* rays, uint32_t n_rays,
                      uint8_t* out, int threads);
@@ -186,6 +190,11 @@ int or_scene_set_normals(or_scene* s, const float* normals9, const uint8_t* has)
  * may be NULL): radiance of the first sample of every pixel, FP32. */
 int or_render(const or_scene* s, const or_render_desc* d, double* out,
               float* sample0, int threads);
+/* The pixels listed (row-major indices) of or_render's image: out[3*i] is
+ * the mean radiance of pixel pixels[i] over d->spp samples from d->spp_offset,
+ * the same paths or_render traces for it. */
+int or_render_pixels(const or_scene* s, const or_render_desc* d, const uint32_t* pixels, uint32_t n,
+                     double* out, int threads);
 /* One path per caller ray (rays: 8 floats each, tmin/tmax ignored): path i
  * uses PCG32 sequence i and sample index d->spp_offset; out[3*i] radiance. */
 int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n,

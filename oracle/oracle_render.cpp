@@ -1018,6 +1018,34 @@ int or_integrate_rays_guided(const or_scene* s, const float* rays, uint32_t n, c
     return 0;
 }
 
+int or_render_pixels(const or_scene* s, const or_render_desc* d, const uint32_t* pixels, uint32_t n,
+                     double* out, int threads) {
+    if (!s || !d || (n && (!pixels || !out)) || d->width == 0 || d->height == 0) return 1;
+    Camera cam = make_camera(*d);
+    const uint32_t npix = d->width * d->height;
+    for (uint32_t i = 0; i < n; ++i)
+        if (pixels[i] >= npix) return 1;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const uint32_t p = pixels[i], px = p % d->width, py = p / d->width;
+            double acc[3] = {0, 0, 0};
+            for (uint32_t si = 0; si < d->spp; ++si) {
+                V3 L = path_radiance(*s, *d, cam, px, py, d->spp_offset + si, nullptr);
+                acc[0] += L.x; acc[1] += L.y; acc[2] += L.z;
+            }
+            for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = acc[k] / (double)d->spp;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    return 0;
+}
+
 int or_render(const or_scene* s, const or_render_desc* d, double* out, float* sample0,
               int threads) {
     return or_render_guided(s, d, nullptr, out, sample0, threads);

@@ -398,6 +398,34 @@ inline void bvh_closest_one(const or_bvh& b, const float* ray, float* hit) {
     else write_hit(hit, best_t, best_p, bu, bv);
 }
 
+// A second primitive (not `skip`) hit with t <= tmax_incl, through the BVH.
+inline bool bvh_other_hit(const or_bvh& b, const WRay& r, float tmax_incl, uint32_t skip) {
+    float inv[3];
+    safe_inv(r.d, inv);
+    uint32_t stack[256];
+    int sp = 0;
+    if (b.nodes.empty()) return false;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        uint32_t c = stack[--sp];
+        const Node& n = b.nodes[c];
+        float tt;
+        if (!box_hit(n, r.o, inv, r.tmin, tmax_incl, tt)) continue;
+        if (n.count) {
+            for (uint32_t i = 0; i < n.count; ++i) {
+                if (b.prim[n.first + i] == skip) continue;
+                const float* tv = &b.tris[9 * (size_t)(n.first + i)];
+                float t, u, v;
+                if (wt_hit(r, tv, tv + 3, tv + 6, tmax_incl, t, u, v)) return true;
+            }
+        } else {
+            stack[sp++] = n.first + 1;
+            stack[sp++] = n.first;
+        }
+    }
+    return false;
+}
+
 inline bool bvh_occluded_one(const or_bvh& b, const float* ray) {
     if (!ray_valid(ray)) return false;
     WRay r = make_wray(ray);
@@ -554,6 +582,25 @@ uint32_t or_count_near_ties(const float* tris, uint32_t n_tris, const float* ray
                 float t, u, v;
                 if (wt_hit(r, tv, tv + 3, tv + 6, lim, t, u, v)) { local++; break; }
             }
+        }
+        ties += local;
+    });
+    return ties.load();
+}
+
+uint32_t or_bvh_count_near_ties(const or_bvh* b, const float* rays, uint32_t n_rays, const float* hits,
+                                int threads) {
+    std::atomic<uint32_t> ties{0};
+    parallel_chunks(n_rays, threads, 256, [&](uint32_t lo, uint32_t hi) {
+        uint32_t local = 0;
+        for (uint32_t i = lo; i < hi; ++i) {
+            uint32_t p;
+            std::memcpy(&p, &hits[4 * (size_t)i + 1], 4);
+            if (p == 0xFFFFFFFFu || !ray_valid(rays + 8 * (size_t)i)) continue;
+            const WRay r = make_wray(rays + 8 * (size_t)i);
+            // no primitive lies before the closest t, so any other hit up to
+            // one ulp beyond it is a near tie
+            if (bvh_other_hit(*b, r, std::nextafter(hits[4 * (size_t)i], INFINITY), p)) local++;
         }
         ties += local;
     });

@@ -164,3 +164,64 @@ def test_batch_of_large_scenes_matches_renders_alone():
     assert st["extension_rays"] + st["shadow_rays"] == rays_alone
     b.close()
     owner.close()
+
+
+def test_batch_workload_32_uneven_scenes():
+    """The batched workload itself: 32 terrain scenes (seeds 100-131, 1e4-2e6
+    triangles, 128^2-1024^2, 4-64 spp, depth 8) rendered as one autotuned
+    batch through the replayed pass graph.  Every film equals the scene's
+    render alone bit for bit and the batch traces the same rays; then, at one
+    sample per pixel, 512 pixels of each of four scenes are traced path by
+    path by the oracle (Renderer::render's per-pixel paths,
+    src/core/renderer.cpp:18-90) and must agree."""
+    import os
+
+    import oracle
+    ss = scenes.batch_scenes(32, seed=100)
+    print("batch workload: %d scenes, %d triangles" % (len(ss), sum(s.n_tris for s in ss)), flush=True)
+    owner = native.Context(0)
+    ctxs, alone, rays_alone = [], [], 0
+    for s in ss:
+        c = native.Context(0)
+        c.load_scene(s)
+        alone.append(c.render(native.make_render_desc(s)))
+        st = c.render_stats()
+        rays_alone += st["extension_rays"] + st["shadow_rays"]
+        c.release_buffers()
+        ctxs.append(c)
+    print("rendered alone: %d rays" % rays_alone, flush=True)
+    b = native.Batch(owner, ctxs)
+    descs = [native.make_render_desc(s) for s in ss]
+    tuned = b.autotune(descs, passes=2).as_dict()
+    b.render(descs)
+    st = b.stats()
+    print("batch: %s, %d rays" % (tuned, st["extension_rays"] + st["shadow_rays"]), flush=True)
+    assert st["extension_rays"] + st["shadow_rays"] == rays_alone
+    _check(b, ss, alone, "32-scene workload")
+    # one sample per pixel against the oracle, path by path
+    d1 = [native.make_render_desc(s, spp=1) for s in ss]
+    b.render(d1)
+    rng = np.random.default_rng(5)
+    order = np.argsort([s.n_tris for s in ss])
+    picks = [int(order[0]), int(order[1]), int(order[len(ss) // 2]), int(order[-1])]
+    threads = min(16, os.cpu_count() or 1)
+    agree = total = 0
+    for i in picks:
+        s = ss[i]
+        img = b.film_read(i, s.width, s.height).reshape(-1, 3).astype(np.float64)
+        px = rng.choice(s.width * s.height, 512, replace=False)
+        d = oracle.RenderDesc()
+        d.width, d.height, d.spp, d.max_depth, d.rr_depth, d.seed = s.width, s.height, 1, s.max_depth, s.rr_depth, s.seed
+        d.spp_offset = 0
+        d.eye[:], d.lookat[:], d.up[:] = s.eye, s.lookat, s.up
+        d.tan_half_fov = s.tan_half_fov
+        ref = oracle.scene_from(s).render_pixels(d, px, threads=threads)
+        ok = np.all(np.abs(img[px] - ref) <= 2e-3 * np.abs(ref) + 1e-5, axis=1)
+        print("scene %d (%d tris): %d / %d paths agree" % (i, s.n_tris, ok.sum(), ok.size), flush=True)
+        agree += int(ok.sum())
+        total += ok.size
+    b.close()
+    for c in ctxs:
+        c.close()
+    owner.close()
+    assert agree >= 0.999 * total

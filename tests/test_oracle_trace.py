@@ -123,3 +123,30 @@ def test_bvh_empty_and_degenerate_rays():
     h = bvh.closest(bad)
     assert (h[:, 1].view(np.uint32) == 0xFFFFFFFF).all()
     assert np.array_equal(h.view(np.uint32), oracle.closest_bruteforce(tris, bad).view(np.uint32))
+
+
+def test_bvh_near_tie_count_equals_bruteforce():
+    # a grid of shared edges and a duplicated patch: rays through edges and
+    # through the duplicates tie; the BVH count must equal the brute force
+    rng = np.random.default_rng(12)
+    g = np.linspace(-1, 1, 9)
+    tris = []
+    for i in range(8):
+        for j in range(8):
+            a, b, c, d = (g[i], g[j], 0), (g[i + 1], g[j], 0), (g[i + 1], g[j + 1], 0), (g[i], g[j + 1], 0)
+            tris += [a + b + c, a + c + d]
+    tris = np.asarray(tris, np.float32)
+    tris = np.concatenate([tris, tris[:10], tris[20:30] + np.float32([0, 0, 1e-7] * 3)])
+    o = np.zeros((4000, 3), np.float32)
+    o[:, :2] = rng.uniform(-1, 1, (4000, 2))
+    o[:2000, :2] = np.round(o[:2000, :2] * 4) / 4        # many through edges and vertices
+    o[:, 2] = 2.0
+    rays = np.zeros((4000, 8), np.float32)
+    rays[:, :3] = o
+    rays[:, 6] = -1.0
+    rays[:, 7] = np.inf
+    bvh = oracle.Bvh(tris)
+    hits = bvh.closest(rays)
+    n = bvh.count_near_ties(rays, hits)
+    assert n == oracle.count_near_ties(tris, rays)
+    assert n > 1000
