@@ -37,7 +37,7 @@ def profiled_traffic():
     except OSError:
         pass
     return None
-BASELINE_METRIC = "incoherent Mrays/s (1 GPU); equal-relMSE time, guided vs plain GPU tracer"
+HEADLINE_METRIC = "incoherent Mrays/s (1 GPU); equal-relMSE time, guided vs plain GPU tracer"
 
 
 def host_info():
@@ -455,7 +455,7 @@ def main():
         guiding = guiding_equal_error(args)
 
     line = {
-        "metric": BASELINE_METRIC,
+        "metric": HEADLINE_METRIC,
         "value": round(mrays_inc, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,

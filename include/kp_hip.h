@@ -358,6 +358,17 @@ int kp_integrate_rays(kp_ctx* ctx, const kp_ray* rays, uint32_t n, const kp_rend
 int kp_guiding_init(kp_ctx* ctx, uint64_t seed);
 int kp_guiding_set_params(kp_ctx* ctx, const float* params, uint32_t n);
 int kp_guiding_get_params(kp_ctx* ctx, float* params, uint32_t n);
+/* Guiding checkpoint (save / load of the network state): the
+ * FP32 parameters, both Adam moments, the Adam step, the count of training
+ * passes and the position box, KP_GUIDE_CHECKPOINT_BYTES bytes in all, in a
+ * caller buffer (host memory). kp_guiding_save with buf NULL only reports
+ * the size through *written. Loading then training continues exactly as the
+ * saved context would have (training is deterministic for a given batch).
+ * The render settings of kp_guiding_configure are not part of it. No
+ * counterpart in the reference. */
+#define KP_GUIDE_CHECKPOINT_BYTES (64u + 3u * 4u * KP_GUIDE_PARAMS)
+int kp_guiding_save(kp_ctx* ctx, void* buf, uint64_t bytes, uint64_t* written);
+int kp_guiding_load(kp_ctx* ctx, const void* buf, uint64_t bytes);
 /* Network outputs (32 per sample); host pointers. */
 int kp_guiding_eval(kp_ctx* ctx, const float* in6, uint32_t n, float* out32);
 /* Mixture density of unit directions (host pointers). */

@@ -130,6 +130,8 @@ public:
             check(kp_mesh_add(ctx_, xyz.data(), (uint32_t)m.verts.size(), idx.data(), (uint32_t)m.indice.size(), xf,
                               &id));
             check(kp_mesh_set_material(ctx_, id, m.material_id));
+            std::vector<float> n9;
+            if (normals9(m, n9)) check(kp_mesh_set_normals(ctx_, id, n9.data()));
             places_.push_back({next, p.first, p.second});
             next += (uint32_t)m.indice.size();
         }
@@ -144,6 +146,8 @@ public:
                 check(kp_blas_add(ctx_, xyz.data(), (uint32_t)m.verts.size(), idx.data(), (uint32_t)m.indice.size(),
                                   &proto[p.first]));
                 check(kp_blas_set_material(ctx_, proto[p.first], m.material_id));
+                std::vector<float> n9;
+                if (normals9(m, n9)) check(kp_blas_set_normals(ctx_, proto[p.first], n9.data()));
             }
             float xf[12];
             rows(p.second, xf);
@@ -222,14 +226,26 @@ private:
         for (size_t i = 0; i < m.indice.size(); ++i)
             for (int k = 0; k < 3; ++k) idx[3 * i + k] = (uint32_t)m.indice[i][k];
     }
+    // Per-triangle vertex normals (9 floats each) when the mesh has one
+    // normal per vertex (TriangleMesh::norms, src/core/shape.h:188-236).
+    static bool normals9(const TriangleMesh& m, std::vector<float>& n9) {
+        if (m.norms.size() != m.verts.size() || m.norms.empty()) return false;
+        n9.resize(9 * m.indice.size());
+        for (size_t i = 0; i < m.indice.size(); ++i)
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) n9[9 * i + 3 * k + a] = m.norms[m.indice[i][k]][a];
+        return true;
+    }
     static void rows(const Mat4f& x, float* xf) {
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 4; ++c) xf[4 * r + c] = x.m[r][c];
     }
 
     // Global primitive id -> (placement, mesh, local id), then the fields
-    // post_hit fills (TriangleMesh::post_hit, src/core/shape.cpp:615-683,
-    // flat normals) in world space.
+    // post_hit fills (TriangleMesh::post_hit, src/core/shape.cpp:615-683) in
+    // world space: interpolated shading normal and the shading point moved
+    // off the flat triangle (Hanika 2021) when the mesh has vertex normals,
+    // tangent / bitangent, uv and dpdu / dpdv from the mesh uvs.
     void fill(const Ray& r, const kp_hit& h, Intersection& is) const {
         size_t lo = 0, hi = places_.size();
         while (hi - lo > 1) {
@@ -250,8 +266,40 @@ private:
         is.P = v[0] * b0 + v[1] * h.u + v[2] * h.v;
         is.N = normalize(cross(v[1] - v[0], v[2] - v[0]));
         is.shading_normal = is.N;
+        if (m.norms.size() == m.verts.size() && !m.norms.empty()) {
+            // normals transform with the inverse transpose of the placement
+            Vec3f n[3];
+            for (int k = 0; k < 3; ++k) n[k] = normalize(inverse_transpose(pl.xf, m.norms[m.indice[local][k]]));
+            is.shading_normal = normalize(n[0] * b0 + n[1] * h.u + n[2] * h.v);
+            Vec3f tu = is.P - v[0], tw = is.P - v[1], tx = is.P - v[2];
+            tu = tu - n[0] * std::fmin(0.0f, dot(tu, n[0]));
+            tw = tw - n[1] * std::fmin(0.0f, dot(tw, n[1]));
+            tx = tx - n[2] * std::fmin(0.0f, dot(tx, n[2]));
+            is.P = is.P + (tu * b0 + tw * h.u + tx * h.v);
+        }
+        is.tangent = normalize(v[1] - v[0]);
+        is.bitangent = cross(is.tangent, is.N);
         is.bary = Vec3f(b0, h.u, h.v);
         is.uv = Vec2f(h.u, h.v);
+        is.dpdu = is.tangent;
+        is.dpdv = is.bitangent;
+        if (m.uvs.size() == m.verts.size() && !m.uvs.empty()) {
+            const Vec2f& t0 = m.uvs[m.indice[local][0]];
+            const Vec2f& t1 = m.uvs[m.indice[local][1]];
+            const Vec2f& t2 = m.uvs[m.indice[local][2]];
+            is.uv = Vec2f(b0 * t0[0] + h.u * t1[0] + h.v * t2[0], b0 * t0[1] + h.u * t1[1] + h.v * t2[1]);
+            const float d13u = t0[0] - t2[0], d13v = t0[1] - t2[1], d23u = t1[0] - t2[0], d23v = t1[1] - t2[1];
+            const float det = d13u * d23v - d13v * d23u;
+            if (std::fabs(det) >= 1e-8f) {
+                const Vec3f dp13 = v[0] - v[2], dp23 = v[1] - v[2];
+                const float inv = 1.0f / det;
+                const Vec3f du = (dp13 * d23v - dp23 * d13v) * inv, dv = (dp23 * d13u - dp13 * d23u) * inv;
+                if (length_squared(cross(du, dv)) > 0.0f) {
+                    is.dpdu = normalize(du);
+                    is.dpdv = normalize(dv);
+                }
+            }
+        }
         is.ray_t = h.t;
         is.backface = dot(r.direction, is.N) > 0.0f;
         is.is_light = m.is_light;
@@ -259,6 +307,22 @@ private:
         is.geom_id = pl.mesh;
         is.prim_id = local;
         is.shader_name = m.shader_name;
+    }
+
+    // (M^-1)^T n for the upper 3x3 of a row-major object-to-world matrix
+    static Vec3f inverse_transpose(const Mat4f& x, const Vec3f& n) {
+        const float(*a)[4] = x.m;
+        const float c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1], c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2],
+                    c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+        const float c10 = a[0][2] * a[2][1] - a[0][1] * a[2][2], c11 = a[0][0] * a[2][2] - a[0][2] * a[2][0],
+                    c12 = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+        const float c20 = a[0][1] * a[1][2] - a[0][2] * a[1][1], c21 = a[0][2] * a[1][0] - a[0][0] * a[1][2],
+                    c22 = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+        // the cofactor matrix is det * (M^-1)^T: the scale drops out in
+        // normalize, the sign of det does not
+        const float sg = (a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02) < 0.0f ? -1.0f : 1.0f;
+        return Vec3f(c00 * n[0] + c01 * n[1] + c02 * n[2], c10 * n[0] + c11 * n[1] + c12 * n[2],
+                     c20 * n[0] + c21 * n[1] + c22 * n[2]) * sg;
     }
 
     struct Group {

@@ -5,9 +5,10 @@
 //   Ray                 src/core/ray.h:10-45 (direction normalised by the
 //                       constructor, tmin defaults to epsilon = 1e-4 from
 //                       src/base/basic_types.h:9, tmax to FLT_MAX)
-//   Intersection        src/core/intersection.h:24-70 (the fields the GPU hit
-//                       fills: P, N, shading_normal, uv, ray_t, backface,
-//                       is_light, light_id, geom_id, prim_id, shader_name)
+//   Intersection        src/core/intersection.h:24-70 (the fields a hit
+//                       fills: P, N, shading_normal, tangent, bitangent,
+//                       dpdu, dpdv, bary, uv, ray_t, backface, is_light,
+//                       light_id, geom_id, prim_id, shader_name)
 //   HitableInterface    src/core/hitable.h:9-37 (intersect x2, occluded x2)
 #pragma once
 #include <cfloat>
@@ -86,11 +87,13 @@ public:
 };
 
 struct Intersection {
-    Vec3f P;
+    Vec3f P;                 // hit point; with vertex normals, moved off the flat triangle
     Vec3f N;                 // geometric normal (unit, winding order)
-    Vec3f shading_normal;    // = N (meshes carry no vertex normals here)
+    Vec3f shading_normal;    // interpolated vertex normal, or N
+    Vec3f tangent, bitangent;   // world (v1 - v0) direction, cross(tangent, N)
+    Vec3f dpdu, dpdv;        // from the mesh uvs, else tangent / bitangent
     Vec3f bary;              // (1-u-v, u, v)
-    Vec2f uv;                // barycentric weights of vertices 1 and 2
+    Vec2f uv;                // interpolated mesh uv, else the barycentric (u, v)
     float ray_t = FLT_MAX;
     bool backface = false;
     bool is_light = false;

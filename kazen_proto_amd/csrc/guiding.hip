@@ -683,6 +683,85 @@ int kp_guiding_get_params(kp_ctx* c, float* params, uint32_t n) {
     return KP_OK;
 }
 
+// Checkpoint: a 64-byte header {magic, version, parameter count, Adam step,
+// training passes, beta1, beta2, eps, box origin[3], box inverse extent[3],
+// reserved}, then the FP32 masters, the first and the second Adam moments.
+namespace {
+constexpr uint32_t kCkMagic = 0x4347504Bu;   // "KPGC"
+constexpr uint32_t kCkVersion = 1u;
+struct CkHeader {
+    uint32_t magic, version, n_params;
+    int32_t step;
+    uint32_t train_passes;
+    float beta1, beta2, eps;
+    float bmin[3], binv[3];
+    uint32_t reserved[2];
+};
+static_assert(sizeof(CkHeader) == 64, "checkpoint header");
+}  // namespace
+
+int kp_guiding_save(kp_ctx* c, void* buf, uint64_t bytes, uint64_t* written) {
+    if (!c) return KP_ERR_INVALID;
+    if (written) *written = KP_GUIDE_CHECKPOINT_BYTES;
+    if (!buf) return KP_OK;   // size query
+    if (bytes < KP_GUIDE_CHECKPOINT_BYTES)
+        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_save: buffer smaller than KP_GUIDE_CHECKPOINT_BYTES");
+    int rc = kp::guide_ready(c);
+    if (rc) return rc;
+    KP_USE_DEVICE(c);
+    GuideState* g = c->guide;
+    CkHeader h{};
+    h.magic = kCkMagic;
+    h.version = kCkVersion;
+    h.n_params = KP_G_PARAMS;
+    h.step = g->step;
+    h.train_passes = g->train_passes;
+    h.beta1 = g->beta1;
+    h.beta2 = g->beta2;
+    h.eps = g->eps;
+    h.bmin[0] = g->bmin.x; h.bmin[1] = g->bmin.y; h.bmin[2] = g->bmin.z;
+    h.binv[0] = g->binv.x; h.binv[1] = g->binv.y; h.binv[2] = g->binv.z;
+    unsigned char* out = (unsigned char*)buf;
+    std::memcpy(out, &h, sizeof(h));
+    const size_t P = KP_G_PARAMS * sizeof(float);
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    KP_HIP_TRY(c, hipMemcpy(out + sizeof(h), g->params, P, hipMemcpyDeviceToHost));
+    KP_HIP_TRY(c, hipMemcpy(out + sizeof(h) + P, g->m, P, hipMemcpyDeviceToHost));
+    KP_HIP_TRY(c, hipMemcpy(out + sizeof(h) + 2 * P, g->v, P, hipMemcpyDeviceToHost));
+    return KP_OK;
+}
+
+int kp_guiding_load(kp_ctx* c, const void* buf, uint64_t bytes) {
+    if (!c || !buf) return KP_ERR_INVALID;
+    CkHeader h;
+    if (bytes < KP_GUIDE_CHECKPOINT_BYTES)
+        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_load: buffer smaller than KP_GUIDE_CHECKPOINT_BYTES");
+    std::memcpy(&h, buf, sizeof(h));
+    if (h.magic != kCkMagic || h.version != kCkVersion || h.n_params != KP_G_PARAMS)
+        return kp::fail(c, KP_ERR_INVALID, "kp_guiding_load: not a guiding checkpoint of this network");
+    if (h.step < 0) return kp::fail(c, KP_ERR_INVALID, "kp_guiding_load: negative Adam step");
+    KP_USE_DEVICE(c);
+    int rc = kp::guide_alloc(c);
+    if (rc) return rc;
+    GuideState* g = c->guide;
+    const unsigned char* in = (const unsigned char*)buf;
+    const size_t P = KP_G_PARAMS * sizeof(float);
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    KP_HIP_TRY(c, hipMemcpy(g->params, in + sizeof(h), P, hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMemcpy(g->m, in + sizeof(h) + P, P, hipMemcpyHostToDevice));
+    KP_HIP_TRY(c, hipMemcpy(g->v, in + sizeof(h) + 2 * P, P, hipMemcpyHostToDevice));
+    g->step = h.step;
+    g->train_passes = h.train_passes;
+    g->beta1 = h.beta1;
+    g->beta2 = h.beta2;
+    g->eps = h.eps;
+    g->bmin = make_float3(h.bmin[0], h.bmin[1], h.bmin[2]);
+    g->binv = make_float3(h.binv[0], h.binv[1], h.binv[2]);
+    if ((rc = kp::guide_pack(c))) return rc;
+    KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return KP_OK;
+}
+
 int kp_guiding_eval(kp_ctx* c, const float* in6, uint32_t n, float* out32) {
     if (!c || (n && (!in6 || !out32))) return KP_ERR_INVALID;
     return kp::guide_eval_common(c, in6, n, out32, nullptr, nullptr, nullptr, nullptr, 0);

@@ -182,6 +182,8 @@ SIGNATURES = {
     "kp_guiding_sample": (_i32, [_p, _p, _p, _u32, _p, _p]),
     "kp_guiding_train_step": (_i32, [_p, _p, _p, _p, _u32, ctypes.c_float, _i32, _p]),
     "kp_guiding_configure": (_i32, [_p, _p]),
+    "kp_guiding_save": (_i32, [_p, _p, _u64, ctypes.POINTER(_u64)]),
+    "kp_guiding_load": (_i32, [_p, _p, _u64]),
     "kp_batch_create": (_i32, [_p, _p, _u32, ctypes.POINTER(_p)]),
     "kp_batch_destroy": (_i32, [_p]),
     "kp_batch_set_tuning": (_i32, [_p, _p]),
@@ -603,6 +605,18 @@ class Context:
         p = np.zeros(self.GUIDE_PARAMS, np.float32)
         self._check(lib().kp_guiding_get_params(self._h, _np_ptr(p), p.size))
         return p
+
+    def guiding_save(self):
+        """The network's checkpoint (parameters, Adam moments and step) as bytes."""
+        n = _u64()
+        self._check(lib().kp_guiding_save(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(lib().kp_guiding_save(self._h, buf, n.value, ctypes.byref(n)))
+        return buf.raw
+
+    def guiding_load(self, data):
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        self._check(lib().kp_guiding_load(self._h, buf, len(data)))
 
     def guiding_eval(self, in6):
         in6 = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)

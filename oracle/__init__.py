@@ -91,6 +91,7 @@ def lib():
             "or_mf_lambda": (ctypes.c_double, [_i32, ctypes.c_float, ctypes.c_float, _p]),
             "or_mf_sample_vndf": (None, [_i32, ctypes.c_float, ctypes.c_float, _p, _p, _p]),
             "or_fresnel_dielectric": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+            "or_shading_point": (None, [_p, _p, ctypes.c_float, ctypes.c_float, _p, _p]),
             "or_erfinv": (ctypes.c_double, [ctypes.c_double]),
             "or_bsdf_eval_batch": (None, [_p, _p, _p, _i32, _p, _p]),
             "or_bsdf_sample_batch": (None, [_p, _p, _p, _i32, _p, _p, _p, _p]),
@@ -352,6 +353,16 @@ def mf_sample_vndf(dist, alpha, wo, u2, alpha_y=None):
 
 def fresnel_dielectric(cos_i, eta):
     return lib().or_fresnel_dielectric(cos_i, eta)
+
+
+def shading_point(tri9, normals9, u, v):
+    """(P, shading normal) of the integrator's post-hit step."""
+    t = np.ascontiguousarray(tri9, np.float32)
+    n = np.ascontiguousarray(normals9, np.float32)
+    P = np.zeros(3, np.float32)
+    ns = np.zeros(3, np.float32)
+    lib().or_shading_point(_ptr(t), _ptr(n), u, v, _ptr(P), _ptr(ns))
+    return P, ns
 
 
 def erfinv(x):

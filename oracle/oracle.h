@@ -140,6 +140,9 @@ double or_mf_lambda(int dist, float ax, float ay, const float* w);
 void or_mf_sample_vndf(int dist, float ax, float ay, const float* wo, const float* u2,
                        float* m);
 double or_fresnel_dielectric(double cos_i, double eta);
+/* Shading point and normal of a hit at barycentrics (u, v) of a triangle
+ * with vertex normals (the integrator's post-hit step). */
+void or_shading_point(const float* tri9, const float* normals9, float u, float v, float* P, float* ns);
 double or_erfinv(double x);
 /* Batched forms (n directions, 3 floats each) for quadrature tests. */
 void or_bsdf_eval_batch(const or_material* m, const float* wo, const float* wi, int n,

@@ -687,6 +687,24 @@ struct Guide {
 V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir,
               const Guide* g = nullptr);
 
+// TriangleMesh::post_hit (shape.cpp:615-654) with vertex normals nv (9
+// floats): the interpolated shading normal ns, and the shading point P moved
+// off the flat triangle by the method of Hanika, "Hacking the Shadow
+// Terminator" (2021): each vertex's offset to P, projected onto the tangent
+// plane of its normal where P lies below it, blended barycentrically.  World
+// space throughout (the reference dots object-space normals with object-space
+// offsets, identical for identity transforms: D21).
+void shading_point(V3 v0, V3 v1, V3 v2, const float* nv, float u, float v, V3& P, V3& ns) {
+    const float b0 = 1.0f - u - v;
+    V3 n0 = load3(nv), n1 = load3(nv + 3), n2 = load3(nv + 6);
+    ns = normalize(n0 * b0 + n1 * u + n2 * v);
+    V3 tu = P - v0, tw = P - v1, tx = P - v2;
+    tu = tu - n0 * std::min(0.0f, dot(tu, n0));
+    tw = tw - n1 * std::min(0.0f, dot(tw, n1));
+    tx = tx - n2 * std::min(0.0f, dot(tx, n2));
+    P = P + (tu * b0 + tw * u + tx * v);
+}
+
 V3 path_radiance(const or_scene& s, const or_render_desc& d, const Camera& cam, uint32_t px,
                  uint32_t py, uint32_t sample, const Guide* g = nullptr) {
     Pcg32 rng;
@@ -720,23 +738,8 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
         float b0 = 1.0f - h.u - h.v;
         V3 P = v0 * b0 + v1 * h.u + v2 * h.v;
         V3 ns = ng;
-        if (!s.has_norms.empty() && s.has_norms[h.prim]) {
-            // TriangleMesh::post_hit (shape.cpp:615-654): interpolated shading
-            // normal, and the shading point moved off the flat triangle by the
-            // method of Hanika, "Hacking the Shadow Terminator" (2021): each
-            // vertex's offset to P, projected onto the tangent plane of its
-            // normal where P lies below it, blended barycentrically.  World
-            // space throughout (the reference dots object-space normals with
-            // object-space offsets, identical for identity transforms: D21).
-            const float* nv = &s.norms[9 * (size_t)h.prim];
-            V3 n0 = load3(nv), n1 = load3(nv + 3), n2 = load3(nv + 6);
-            ns = normalize(n0 * b0 + n1 * h.u + n2 * h.v);
-            V3 tu = P - v0, tw = P - v1, tx = P - v2;
-            tu = tu - n0 * std::min(0.0f, dot(tu, n0));
-            tw = tw - n1 * std::min(0.0f, dot(tw, n1));
-            tx = tx - n2 * std::min(0.0f, dot(tx, n2));
-            P = P + (tu * b0 + tw * h.u + tx * h.v);
-        }
+        if (!s.has_norms.empty() && s.has_norms[h.prim])
+            shading_point(v0, v1, v2, &s.norms[9 * (size_t)h.prim], h.u, h.v, P, ns);
         const Mat& m = s.mats[s.tri_mat[h.prim]];
         float cos_hit = dot(dir, ng);
         if (!is_zero(m.Le) && cos_hit < 0.0f) {
@@ -871,6 +874,14 @@ void or_mf_sample_vndf(int dist, float ax, float ay, const float* wo, const floa
     m[0] = r.x; m[1] = r.y; m[2] = r.z;
 }
 double or_fresnel_dielectric(double cos_i, double eta) { return fresnel_dielectric(cos_i, eta); }
+
+void or_shading_point(const float* tri9, const float* normals9, float u, float v, float* P, float* ns) {
+    const V3 v0 = load3(tri9), v1 = load3(tri9 + 3), v2 = load3(tri9 + 6);
+    V3 p = v0 * (1.0f - u - v) + v1 * u + v2 * v, n;
+    shading_point(v0, v1, v2, normals9, u, v, p, n);
+    P[0] = p.x; P[1] = p.y; P[2] = p.z;
+    ns[0] = n.x; ns[1] = n.y; ns[2] = n.z;
+}
 double or_erfinv(double x) { return erfinv_d(x); }
 
 void or_bsdf_eval_batch(const or_material* m, const float* wo, const float* wi, int n, float* f,

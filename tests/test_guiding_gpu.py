@@ -121,3 +121,33 @@ def test_training_reduces_loss(gctx):
     after = np.mean(np.log(gctx.guiding_pdf(in6[:1024], d[:1024])))
     print("mean log pdf of the training directions: %.3f -> %.3f" % (before, after))
     assert after > before + 1.5
+
+
+def test_checkpoint_resumes_training_exactly(ctx):
+    # save after some steps; further steps from the saved state (in another
+    # context) equal the further steps of the original, bit for bit
+    from kazen_proto_amd import native
+    rng = np.random.default_rng(31)
+    n = 4096
+    in6 = np.concatenate([rng.random((n, 3)), rng.normal(size=(n, 3))], 1).astype(np.float32)
+    in6[:, 3:] /= np.linalg.norm(in6[:, 3:], axis=1, keepdims=True)
+    dirs = rng.normal(size=(n, 3)).astype(np.float32)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    w = rng.random(n).astype(np.float32)
+    ctx.guiding_init(7)
+    for _ in range(3):
+        ctx.guiding_train_step(in6, dirs, w, lr=1e-3)
+    ck = ctx.guiding_save()
+    assert len(ck) == 64 + 3 * 4 * ctx.GUIDE_PARAMS
+    for _ in range(4):
+        ctx.guiding_train_step(in6, dirs, w, lr=1e-3)
+    a = ctx.guiding_get_params()
+    other = native.Context(0)
+    other.guiding_load(ck)
+    for _ in range(4):
+        other.guiding_train_step(in6, dirs, w, lr=1e-3)
+    b = other.guiding_get_params()
+    with pytest.raises(native.KpError):
+        other.guiding_load(b"\0" * len(ck))
+    other.close()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -136,3 +136,13 @@ def test_concurrent_host_queries_are_identical():
     for k in range(16):
         assert mismatches(out[k], want) == 0
     ctx.close()
+
+
+def test_guiding_checkpoint_needs_a_network():
+    ctx = native.Context(native.DEVICE_HOST)
+    with pytest.raises(native.KpError) as e:
+        ctx.guiding_save()
+    assert e.value.code == 2
+    with pytest.raises(native.KpError):
+        ctx.guiding_load(b"KPGC" + b"\0" * 100)   # too short
+    ctx.close()

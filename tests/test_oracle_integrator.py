@@ -149,3 +149,29 @@ def test_geometric_shading_normals_change_nothing():
     s.mesh_normals = [np.repeat(n[:, None, :], 3, axis=1).reshape(-1, 9).astype(np.float32)]
     smooth = oracle_scene(s).render(desc(s))
     assert np.allclose(plain, smooth, rtol=1e-5, atol=1e-6)
+
+
+def test_shading_point_offset_closed_form():
+    # triangle (0,0,0), (1,0,0), (0,1,0) with every vertex normal
+    # n = (a, 0, c): at the centroid, (P - v_i).n = a/3, -2a/3, a/3, so only
+    # v1's offset is projected and P' = P + (1/3)(2a/3) n = P + (2a/9) n
+    tri = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32)
+    for a in (0.3, 0.6):
+        n = np.array([a, 0.0, np.sqrt(1 - a * a)], np.float32)
+        P, ns = oracle.shading_point(tri, np.tile(n, 3), 1 / 3, 1 / 3)
+        want = np.array([1 / 3, 1 / 3, 0.0]) + (2 * a / 9) * n
+        assert np.allclose(P, want, atol=1e-6), (P, want)
+        assert np.allclose(ns, n, atol=1e-6)
+    # one shared normal n: P' = P + sum_i b_i max(0, -(P - v_i).n) n, so with
+    # n = (-0.5, 0, sqrt(.75)) at (u, v) = (0.2, 0.3) vertices 0 and 2 lift P
+    # by 0.1 each, weighted 0.5 and 0.3
+    n = np.array([-0.5, 0.0, np.sqrt(0.75)], np.float32)
+    P, _ = oracle.shading_point(tri, np.tile(n, 3), 0.2, 0.3)
+    assert np.allclose(P, np.array([0.2, 0.3, 0.0]) + 0.08 * n, atol=1e-6), P
+    # a vertex under P lifts it by its own weight: at vertex 1 exactly, P' = v1
+    P, ns = oracle.shading_point(tri, np.array([0, 0, 1, 0.6, 0, 0.8, 0, 0, 1], np.float32), 1.0, 0.0)
+    assert np.allclose(P, [1, 0, 0], atol=1e-6)
+    # the interpolated normal at the edge midpoint of v0 and v1
+    _, ns = oracle.shading_point(tri, np.array([0, 0, 1, 0.6, 0, 0.8, 0, 0, 1], np.float32), 0.5, 0.0)
+    m = np.array([0.3, 0, 0.9])
+    assert np.allclose(ns, m / np.linalg.norm(m), atol=1e-6)
