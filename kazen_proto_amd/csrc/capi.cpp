@@ -73,7 +73,12 @@ int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, b
     a.rays = rays;
     a.hits = hits;
     a.n = n;
-    hipError_t e = launch_closest(a, c->grid_closest, count, c->stream);
+    // one-level: the 5-wave build (96 registers) is 1.8 % faster on
+    // incoherent rays than the 4-wave one; the counted and the two-level
+    // variants have kernels of their own
+    const bool five = !count && !c->two_level;
+    a.five_waves = five ? 1u : 0u;
+    hipError_t e = launch_closest(a, five ? c->grid_closest5 : c->grid_closest, count, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "closest-hit launch");
     return KP_OK;
 }

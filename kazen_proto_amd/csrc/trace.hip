@@ -138,7 +138,6 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, const float4 a, const 
     return true;
 }
 
-__device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
 // Per-ray node-test constants: the ray direction's octant (bit k set when
 // direction k is negative) and the 16-byte word of each near / far plane in a
@@ -168,18 +167,6 @@ __device__ __forceinline__ float qplane(const uint4& plane, int slot) {
 // Relative widening of a far distance, 2*gamma(3) (Ize 2013): tf + |tf| * eps.
 #define KP_FAR_EPS 3.5762793e-7f
 
-// Bit s set for every slot whose meta byte is nonzero (a used slot).
-__device__ __forceinline__ uint32_t used_slots(uint32_t m0, uint32_t m1) {
-    auto nz4 = [](uint32_t w) {
-        w |= w >> 4;
-        w |= w >> 2;
-        w |= w >> 1;
-        w &= 0x01010101u;
-        return (w | (w >> 7) | (w >> 14) | (w >> 21)) & 0xFu;
-    };
-    return nz4(m0) | (nz4(m1) << 4);
-}
-
 // Result of one node visit.
 struct NodeHits {
     uint4 w0, w1;        // header words (origin / exponents / inner mask, bases / meta)
@@ -190,9 +177,9 @@ struct NodeHits {
 };
 
 // Order key of a child entry distance: non-negative distances by their bits
-// (monotone), negative ones all first.
+// (monotone), negative ones (sign bit set: negative as integers) all first.
 __device__ __forceinline__ uint32_t tkey(float tn) {
-    return tn > 0.0f ? (__float_as_uint(tn) & 0xFFFFFFF8u) : 0u;
+    return (uint32_t)max((int32_t)__float_as_uint(tn), 0) & 0xFFFFFFF8u;
 }
 
 // Tests the 8 children of node `idx` against [tmin, tmax].
@@ -208,9 +195,10 @@ __device__ __forceinline__ void visit_node(const uint4* __restrict__ nodes, uint
     const float px = __uint_as_float(h.w0.x), py = __uint_as_float(h.w0.y), pz = __uint_as_float(h.w0.z);
     const uint32_t e = h.w0.w;
     const int ex = (int)(int8_t)(e & 0xFF), ey = (int)(int8_t)((e >> 8) & 0xFF), ez = (int)(int8_t)((e >> 16) & 0xFF);
-    const float ax = exp2i(ex) * r.ix, ay = exp2i(ey) * r.iy, az = exp2i(ez) * r.iz;
+    // ix 2^ex: a product with a power of two, exact as the multiplication
+    const float ax = ldexpf(r.ix, ex), ay = ldexpf(r.iy, ey), az = ldexpf(r.iz, ez);
     const float bx = (px - r.ox) * r.ix, by = (py - r.oy) * r.iy, bz = (pz - r.oz) * r.iz;
-    const uint32_t imask = e >> 24, nin = ~imask;
+    const uint32_t imask = e >> 24;
     uint32_t hit = 0u, k1 = 0xFFFFFFFFu, k2 = 0xFFFFFFFFu;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -224,13 +212,14 @@ __device__ __forceinline__ void visit_node(const uint4* __restrict__ nodes, uint
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         const bool hs = tn <= fmaf(fabsf(tf), KP_FAR_EPS, tf);
         hit |= hs ? (1u << s) : 0u;
-        // key: entry distance bits (non-positive -> 0), slot, bit 31 for leaves
-        const uint32_t kv = (tkey(tn) | (uint32_t)s) | (((nin >> s) & 1u) << 31);
-        const uint32_t key = hs ? kv : 0xFFFFFFFFu;
+        // key of a hit inner child: entry distance bits (non-positive -> 0), slot
+        const uint32_t key = (hs && ((imask >> s) & 1u)) ? (tkey(tn) | (uint32_t)s) : 0xFFFFFFFFu;
         k2 = min(k2, max(k1, key));
         k1 = min(k1, key);
     }
-    hit &= used_slots(h.w1.z, h.w1.w);   // empty slots never hit
+    // Empty slots hold inverted boxes (lower plane 2047, upper plane 0), which
+    // miss unless the far-distance widening exceeds the node's extent; such a
+    // slot then counts as a leaf child with no triangles and tests nothing.
     h.inner = hit & imask;
     h.leaves = hit & ~imask;
     h.near_key = k1;
@@ -269,7 +258,7 @@ __device__ __forceinline__ float slot_tnear(const uint4* __restrict__ nodes, uin
     const uint4 nx = p[w & 0xFu], ny = p[(w >> 8) & 0xFu], nz = p[(w >> 16) & 0xFu];
     const uint32_t e = w0.w;
     const int ex = (int)(int8_t)(e & 0xFF), ey = (int)(int8_t)((e >> 8) & 0xFF), ez = (int)(int8_t)((e >> 16) & 0xFF);
-    const float ax = exp2i(ex) * r.ix, ay = exp2i(ey) * r.iy, az = exp2i(ez) * r.iz;
+    const float ax = ldexpf(r.ix, ex), ay = ldexpf(r.iy, ey), az = ldexpf(r.iz, ez);
     const float bx = (__uint_as_float(w0.x) - r.ox) * r.ix, by = (__uint_as_float(w0.y) - r.oy) * r.iy,
                 bz = (__uint_as_float(w0.z) - r.oz) * r.iz;
     const int s = (int)slot;
