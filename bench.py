@@ -103,7 +103,8 @@ def guiding_equal_error(args):
             c.guiding_configure(bsdf_fraction=0.5, min_alpha=0.5, lr=args.guiding_lr,
                                 train_steps=args.guiding_steps, guide_after=args.guiding_after,
                                 max_records=1 << 18, pass_paths=args.guiding_pass_paths,
-                                train_passes=args.guiding_train_passes)
+                                train_passes=args.guiding_train_passes,
+                                max_guided_depth=args.guiding_depth)
         c.film_reset(res, res)
         spp = 0
         sec = 0.0
@@ -312,6 +313,8 @@ def main():
     ap.add_argument("--guiding-target", type=float, default=0.01)
     ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
     ap.add_argument("--guiding-lr", type=float, default=1e-3)
+    ap.add_argument("--guiding-depth", type=int, default=0,
+                    help="guide vertices up to this depth only (0: every vertex)")
     ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
     ap.add_argument("--guiding-pass-paths", type=int, default=1 << 24,
                     help="paths per guided pass (each pass trains once)")

@@ -417,6 +417,9 @@ typedef struct {
                                   relative to its pixel's current estimate R, divided by
                                   sqrt(R^2 + 0.01) (the per-pixel relative error the image
                                   is judged by); 0: by its incident radiance */
+    uint32_t max_guided_depth; /* only vertices at depth <= this (the camera's hit is
+                                  depth 1) are guided and recorded; deeper vertices use
+                                  the BSDF alone and skip the network query; 0 = all */
 } kp_guiding_desc;
 /* Needs kp_guiding_init first; a later kp_guiding_init keeps the settings. */
 int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);

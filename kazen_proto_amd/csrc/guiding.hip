@@ -187,11 +187,26 @@ __host__ __device__ constexpr int g_tile_base(int l) {
 #define KP_G_TILES 68
 #define KP_G_WTILES 17
 
+// Eight consecutive samples s0 .. s0 + 7 of column f of a sample-major bf16
+// block (row stride KP_G_ROW words), packed two per word in sample order: the
+// matrix-core operand with samples along k, gathered by 16-bit LDS reads.
+__device__ __forceinline__ u32x4 g_column8(const uint32_t* rows, int s0, int f) {
+    const uint16_t* p = reinterpret_cast<const uint16_t*>(rows + s0 * KP_G_ROW) + f;
+    constexpr int st = 2 * KP_G_ROW;   // row stride in halves
+    u32x4 v;
+    v[0] = (uint32_t)p[0] | ((uint32_t)p[st] << 16);
+    v[1] = (uint32_t)p[2 * st] | ((uint32_t)p[3 * st] << 16);
+    v[2] = (uint32_t)p[4 * st] | ((uint32_t)p[5 * st] << 16);
+    v[3] = (uint32_t)p[6 * st] | ((uint32_t)p[7 * st] << 16);
+    return v;
+}
+
 // dW[k][n] += sum over the group's 64 samples of X[s][k] dY[s][n]: A = X^T
-// (16 features x 32 samples), B = dY (32 samples x 16 outputs), both read as
-// one 16-byte word from the feature-major blocks.
+// (16 features x 32 samples), B = dY (32 samples x 16 outputs), both gathered
+// from the sample-major blocks (no transposed copies: they would double the
+// block's LDS and keep it at one block per CU).
 template <int L>
-__device__ __forceinline__ void g_dw_tiles(const uint32_t* XT, const uint32_t* dYT, int wave, int lane,
+__device__ __forceinline__ void g_dw_tiles(const uint32_t* X, const uint32_t* dY, int wave, int lane,
                                            f32x4* acc) {
     constexpr int ntn = g_dim_out(L) / 16;
     const int g = lane >> 4, r = lane & 15;
@@ -204,8 +219,8 @@ __device__ __forceinline__ void g_dw_tiles(const uint32_t* XT, const uint32_t* d
             f32x4 a = acc[id >> 2];
 #pragma unroll
             for (int sc = 0; sc < 2; ++sc) {
-                const u32x4 av = *(const u32x4*)(XT + (16 * kt + r) * KP_G_TROW + 16 * sc + 4 * g);
-                const u32x4 bv = *(const u32x4*)(dYT + (16 * nt + r) * KP_G_TROW + 16 * sc + 4 * g);
+                const u32x4 av = g_column8(X, 32 * sc + 8 * g, 16 * kt + r);
+                const u32x4 bv = g_column8(dY, 32 * sc + 8 * g, 16 * nt + r);
                 a = mfma16(av, bv, a);
             }
             acc[id >> 2] = a;
@@ -230,25 +245,19 @@ __device__ __forceinline__ void g_dw_store(float* part, int wave, int lane, cons
 
 // Back-propagation through layer L for one 64-sample group: accumulate its
 // weight and bias gradients, then (L > 0) write dX = (W dY) masked by the
-// ReLU of the layer input, row-major and feature-major, for this wave's 16
-// samples.
+// ReLU of the layer input, sample-major, for this wave's 16 samples.
 template <int L>
 __device__ __forceinline__ void g_backward_layer(const GuideNet& net, uint32_t (*act)[64 * KP_G_ROW],
-                                                 uint32_t (*actT)[64 * KP_G_TROW],
-                                                 const uint32_t* dY, const uint32_t* dYT, uint32_t* dX,
-                                                 uint32_t* dXT, int tid, int wave, int lane, f32x4* acc,
-                                                 float* bacc) {
+                                                 const uint32_t* dY, uint32_t* dX, int tid, int wave, int lane,
+                                                 f32x4* acc, float* bacc) {
     constexpr int dout = g_dim_out(L);
-    g_dw_tiles<L>(actT[L], dYT, wave, lane, acc);
+    g_dw_tiles<L>(act[L], dY, wave, lane, acc);
     if (tid < dout) {
+        // bias gradient: column tid of dY, summed in sample order
         float sb = bacc[L];
-        const uint32_t* row = dYT + tid * KP_G_TROW;
+        const uint16_t* col = reinterpret_cast<const uint16_t*>(dY) + tid;
 #pragma unroll 8
-        for (int w = 0; w < 32; ++w) {
-            const uint32_t x = row[w];
-            sb += bf_lo(x);
-            sb += bf_hi(x);
-        }
+        for (int s = 0; s < 64; ++s) sb += __uint_as_float((uint32_t)col[s * 2 * KP_G_ROW] << 16);
         bacc[L] = sb;
     }
     if constexpr (L > 0) {
@@ -275,8 +284,6 @@ __device__ __forceinline__ void g_backward_layer(const GuideNet& net, uint32_t (
             }
             dxrow[k0 / 2] = pack_bf2(v[0], v[1]);
             dxrow[k0 / 2 + 1] = pack_bf2(v[2], v[3]);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) g_store_t(dXT, k0 + j, 16 * wave + r, f2bf_bits(v[j]));
         }
     }
 }
@@ -285,17 +292,17 @@ __device__ __forceinline__ void g_backward_layer(const GuideNet& net, uint32_t (
 // ... (4 waves x 16 samples each), keeps its weight-gradient tiles in the
 // matrix-core accumulators and its bias gradients in registers across groups,
 // and writes one partial gradient (canonical layout) to partial[b].
-__global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const float* __restrict__ in6,
+__global__ __launch_bounds__(256, 2) void k_guide_train(GuideNet net, const float* __restrict__ in6,
                                                         const float* __restrict__ dirs,
                                                         const float* __restrict__ weights,
                                                         const uint32_t* n_dev, const float* scale_dev,
                                                         float* __restrict__ partial) {
-    // all read and written as 16-byte fragments
+    // 63 KB: two blocks per CU
     __shared__ __align__(16) uint32_t act[KP_G_LAYERS][64 * KP_G_ROW];    // layer inputs, sample-major (bf16)
-    __shared__ __align__(16) uint32_t actT[KP_G_LAYERS][64 * KP_G_TROW];  // the same, feature-major
     __shared__ __align__(16) uint32_t dyb[2][64 * KP_G_ROW];              // gradients, sample-major
-    __shared__ __align__(16) uint32_t dyT[2][64 * KP_G_TROW];             // gradients, feature-major
-    __shared__ __align__(16) float outs[64 * KP_G_ROW];                   // network outputs (f32)
+    // network outputs (f32): read by the output gradient, before the first
+    // backward layer writes dyb[1]
+    float* outs = reinterpret_cast<float*>(dyb[1]);
     const uint32_t n = *n_dev;
     const uint32_t groups = (n + 63) / 64;
     const int tid = threadIdx.x, wave = tid >> 6, lane = __lane_id();
@@ -335,12 +342,7 @@ __global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const floa
             }
             uint32_t* row = act[0] + smp * KP_G_ROW;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                const int w = 6 * part + j;
-                row[w] = wd[j];
-                g_store_t(actT[0], 2 * w, smp, wd[j] & 0xffffu);
-                g_store_t(actT[0], 2 * w + 1, smp, wd[j] >> 16);
-            }
+            for (int j = 0; j < 6; ++j) row[6 * part + j] = wd[j];
             row[24 + 2 * part] = 0u;
             row[25 + 2 * part] = 0u;
         }
@@ -348,9 +350,7 @@ __global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const floa
         {
             uint32_t* a[4] = {act[1] + 16 * wave * KP_G_ROW, act[2] + 16 * wave * KP_G_ROW,
                               act[3] + 16 * wave * KP_G_ROW, act[4] + 16 * wave * KP_G_ROW};
-            uint32_t* at[4] = {actT[1], actT[2], actT[3], actT[4]};
-            g_forward_chunk(net, act[0] + 16 * wave * KP_G_ROW, a, outs + 16 * wave * KP_G_ROW, KP_G_ROW,
-                            nullptr, at, 16 * wave);
+            g_forward_chunk(net, act[0] + 16 * wave * KP_G_ROW, a, outs + 16 * wave * KP_G_ROW, KP_G_ROW);
         }
         __syncthreads();
         {
@@ -368,22 +368,20 @@ __global__ __launch_bounds__(256, 1) void k_guide_train(GuideNet net, const floa
                 const float* g = g8 + 4 * h;
                 row[2 * lobe] = valid ? pack_bf2(g[0], g[1]) : 0u;
                 row[2 * lobe + 1] = valid ? pack_bf2(g[2], g[3]) : 0u;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) g_store_t(dyT[0], 4 * lobe + j, smp, valid ? f2bf_bits(g[j]) : 0u);
             }
 #pragma unroll
             for (int w = 16 + 4 * part; w < 20 + 4 * part; ++w) row[w] = 0u;
         }
         __syncthreads();
-        g_backward_layer<4>(net, act, actT, dyb[0], dyT[0], dyb[1], dyT[1], tid, wave, lane, acc, bacc);
+        g_backward_layer<4>(net, act, dyb[0], dyb[1], tid, wave, lane, acc, bacc);
         __syncthreads();
-        g_backward_layer<3>(net, act, actT, dyb[1], dyT[1], dyb[0], dyT[0], tid, wave, lane, acc, bacc);
+        g_backward_layer<3>(net, act, dyb[1], dyb[0], tid, wave, lane, acc, bacc);
         __syncthreads();
-        g_backward_layer<2>(net, act, actT, dyb[0], dyT[0], dyb[1], dyT[1], tid, wave, lane, acc, bacc);
+        g_backward_layer<2>(net, act, dyb[0], dyb[1], tid, wave, lane, acc, bacc);
         __syncthreads();
-        g_backward_layer<1>(net, act, actT, dyb[1], dyT[1], dyb[0], dyT[0], tid, wave, lane, acc, bacc);
+        g_backward_layer<1>(net, act, dyb[1], dyb[0], tid, wave, lane, acc, bacc);
         __syncthreads();
-        g_backward_layer<0>(net, act, actT, dyb[0], dyT[0], nullptr, nullptr, tid, wave, lane, acc, bacc);
+        g_backward_layer<0>(net, act, dyb[0], nullptr, tid, wave, lane, acc, bacc);
         __syncthreads();
     }
     float* part = partial + (size_t)blockIdx.x * KP_G_PARAMS;
@@ -523,7 +521,10 @@ static int guide_alloc(kp_ctx* c) {
     KP_HIP_TRY(c, hipMalloc((void**)&g->grad, P));
     KP_HIP_TRY(c, hipMalloc((void**)&g->wf, KP_G_WF_FRAGS * 64 * sizeof(u32x4)));
     KP_HIP_TRY(c, hipMalloc((void**)&g->wb, KP_G_WB_FRAGS * 64 * sizeof(u32x4)));
-    g->train_blocks = (uint32_t)(c->cus > 0 ? c->cus : 256);
+    // persistent training grid: every block resident (two per CU at 63 KB of LDS)
+    int per_cu = 0;
+    KP_HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_guide_train, 256, 0));
+    g->train_blocks = (uint32_t)((c->cus > 0 ? c->cus : 256) * (per_cu > 0 ? per_cu : 1));
     KP_HIP_TRY(c, hipMalloc((void**)&g->partial, (size_t)g->train_blocks * P));
     KP_HIP_TRY(c, hipMalloc((void**)&g->seg, (size_t)KP_G_SEGS * P));
     KP_HIP_TRY(c, hipMalloc((void**)&g->in6, (size_t)g->max_batch * 6 * sizeof(float)));
@@ -821,6 +822,7 @@ int kp_guiding_configure(kp_ctx* c, const kp_guiding_desc* d) {
     g->max_records = d->max_records;
     g->pass_paths = d->pass_paths;
     g->max_train_passes = d->train_passes;
+    g->max_guided_depth = d->max_guided_depth;
     g->auto_box = d->bmin[0] == d->bmax[0] && d->bmin[1] == d->bmax[1] && d->bmin[2] == d->bmax[2];
     for (int i = 0; i < 3; ++i) {
         g->box_min[i] = d->bmin[i];

@@ -98,24 +98,12 @@ __device__ __forceinline__ void g_encode(float px, float py, float pz, float nx,
 // the 16 rows a fragment reads over the banks (at most 2-way conflicts).
 #define KP_G_ROW 36
 
-// Feature-major (transposed) copy of a 64-sample activation block: row f
-// holds feature f of the 64 samples as bf16, 32 words + 4 padding words, so a
-// matrix-core operand with samples along k is one 16-byte read.
-#define KP_G_TROW 36
-
-__device__ __forceinline__ void g_store_t(uint32_t* rowsT, int feature, int sample, uint32_t bf_bits) {
-    reinterpret_cast<uint16_t*>(rowsT + feature * KP_G_TROW)[sample] = (uint16_t)bf_bits;
-}
-
 // One 16-sample chunk through the network.  `xin` holds 16 rows of 64 bf16
 // input features (row stride KP_G_ROW words); hidden layer l (0..3) writes its
-// bf16 activations to act[l] (same shape) and, if actT is given, also to the
-// feature-major block actT[l] at samples col0 .. col0 + 15; the 32 outputs of
-// sample r are written to out[r*out_stride + o] (f32).  Every lane of the wave
-// must call it.
+// bf16 activations to act[l] (same shape); the 32 outputs of sample r are
+// written to out[r*out_stride + o] (f32).  Every lane of the wave must call it.
 __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin, uint32_t* const* act,
-                                       float* out, int out_stride, float* last8 = nullptr,
-                                       uint32_t* const* actT = nullptr, int col0 = 0) {
+                                       float* out, int out_stride, float* last8 = nullptr) {
     const int lane = __lane_id();
     const int g = lane >> 4, r = lane & 15;
     const uint32_t* src = xin;
@@ -145,10 +133,6 @@ __device__ inline void g_forward_chunk(const GuideNet& net, const uint32_t* xin,
                 for (int j = 0; j < 4; ++j) v[j] = fmaxf(v[j], 0.0f);
                 dst[r * KP_G_ROW + n0 / 2] = pack_bf2(v[0], v[1]);
                 dst[r * KP_G_ROW + n0 / 2 + 1] = pack_bf2(v[2], v[3]);
-                if (actT) {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) g_store_t(actT[l], n0 + j, col0 + r, f2bf_bits(v[j]));
-                }
             } else if (out) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) out[r * out_stride + n0 + j] = v[j];

@@ -38,6 +38,7 @@ struct GuideState {
     bool relative_weights = false;   // weights relative to the pixel estimate
     uint32_t train_steps = 1, guide_after = 1, max_records = 1u << 18, pass_paths = 1u << 18;
     uint32_t max_train_passes = 0;   // 0 = every pass trains
+    uint32_t max_guided_depth = 0;   // 0 = every depth
     bool auto_box = true;
     float box_min[3] = {0, 0, 0}, box_max[3] = {0, 0, 0};
     uint32_t train_passes = 0;   // passes whose records trained the network

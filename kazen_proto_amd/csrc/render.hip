@@ -104,7 +104,9 @@ struct RenderArgs {
     GuideNet net;
     float bsdf_frac, min_alpha;
     uint32_t guide_on;                      // sample from the network
+    uint32_t guide_depth;                   // guided vertices: depth <= guide_depth
     uint32_t rec_cap, rec_stride, rec_phase;   // training records (rec_cap 0: none)
+    uint32_t rec_depth;                     // records per recorded path: vertices 1..rec_depth
     float* rec_in6;
     float* rec_dirs;
     float4* rec_aux;
@@ -493,10 +495,11 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
             if constexpr (kGuided) {
                 if (a.rec_cap && slot % a.rec_stride == a.rec_phase) {
                     recorded = true;
-                    rec = (slot / a.rec_stride) * a.max_depth;
+                    rec = (slot / a.rec_stride) * a.rec_depth;
                     // radiance gathered before the previous vertex's direction
                     // (its emission and next-event estimate included)
-                    if (depth >= 2) a.rec_aux[2 * (size_t)(rec + depth - 2)] = make_float4(L.x, L.y, L.z, 1.0f);
+                    if (depth >= 2 && depth - 1 <= a.rec_depth)
+                        a.rec_aux[2 * (size_t)(rec + depth - 2)] = make_float4(L.x, L.y, L.z, 1.0f);
                 }
             }
             if (prim != 0xFFFFFFFFu) {
@@ -546,7 +549,8 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, int cur, uint32_t d
                     wo = fr.to_local(wo_w);
                     bs = bsdf_at(m, a.lobes, beta);
                     if constexpr (kGuided) {
-                        const bool eligible = !(m.flags & KP_MATF_TRANSMIT) && m.guide_alpha >= a.min_alpha;
+                        const bool eligible = !(m.flags & KP_MATF_TRANSMIT) && m.guide_alpha >= a.min_alpha &&
+                                              depth <= a.guide_depth;
                         guided = eligible && a.guide_on;
                         recorded = recorded && eligible;
                     }
@@ -722,7 +726,7 @@ __global__ __launch_bounds__(256) void k_connect(RenderArgs a) {
 }
 
 // Turns a pass's records into training samples.  Record i belongs to slot
-// (i / max_depth) * rec_stride + rec_phase and vertex i % max_depth + 1; its
+// (i / rec_depth) * rec_stride + rec_phase and vertex i % rec_depth + 1; its
 // weight is the luminance of (radiance at the end of the path - radiance
 // before the vertex's sampled direction) / throughput before the vertex, an
 // estimate of f cos L_i / p for the sampled direction.  With film != nullptr
@@ -735,7 +739,7 @@ __global__ __launch_bounds__(256) void k_guide_records(RenderArgs a, uint32_t n_
                                                        const float4* film, float inv_count) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n_rec) return;
-    const uint32_t slot = (i / a.max_depth) * a.rec_stride + a.rec_phase;
+    const uint32_t slot = (i / a.rec_depth) * a.rec_stride + a.rec_phase;
     const float4 x0 = a.rec_aux[2 * (size_t)i], x1 = a.rec_aux[2 * (size_t)i + 1];
     float w = 0.0f;
     const bool made = x0.w == 1.0f && x1.w == 1.0f && slot < a.n_paths;
@@ -1226,9 +1230,11 @@ static int guide_args(kp_ctx* c, RenderArgs& a) {
     a.bsdf_frac = g->bsdf_fraction;
     a.min_alpha = g->min_alpha;
     a.guide_on = 1;
+    a.guide_depth = g->max_guided_depth ? g->max_guided_depth : 0xFFFFFFFFu;
     a.rec_cap = 0;
     a.rec_stride = 1;
     a.rec_phase = 0;
+    a.rec_depth = 1;
     a.rec_in6 = g->in6;
     a.rec_dirs = g->dirs;
     a.rec_aux = g->rec_aux;
@@ -1309,8 +1315,11 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     const bool guided = (d->flags & KP_RENDER_GUIDED) != 0;
     GuideState* g = c->guide;
     if (guided && !g) return fail(c, KP_ERR_STATE, "guided render before kp_guiding_init");
-    if (guided && g->train_steps && d->max_depth > g->max_records)
-        return fail(c, KP_ERR_INVALID, "guiding max_records must be >= max_depth");
+    // records per recorded path: the guided vertices 1 .. min(max_depth, max_guided_depth)
+    const uint32_t rec_depth = guided && g->max_guided_depth ? std::min(d->max_depth, g->max_guided_depth)
+                                                             : d->max_depth;
+    if (guided && g->train_steps && rec_depth > g->max_records)
+        return fail(c, KP_ERR_INVALID, "guiding max_records must be >= the recorded depth");
     int rc = prepare_scene(c);
     if (rc) return rc;
     const uint32_t npix = d->width * d->height;
@@ -1338,11 +1347,12 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     a.ps = r->ps;
     if (guided && (rc = guide_args(c, a))) return rc;
     // training records: every rec_stride-th path (phase rotating per pass),
-    // max_depth records each, at most max_records per pass
+    // rec_depth records each, at most max_records per pass
     uint32_t stride = 1, rec_slots_max = 0;
     const bool train_any = guided && g->train_steps > 0;
+    a.rec_depth = rec_depth;
     if (train_any) {
-        const uint64_t D = d->max_depth;
+        const uint64_t D = rec_depth;
         stride = (uint32_t)std::max<uint64_t>(1, ((uint64_t)n_paths * D + g->max_records - 1) / g->max_records);
         while (((uint64_t)n_paths + stride - 1) / stride * D > g->max_records) ++stride;
         rec_slots_max = (uint32_t)(((uint64_t)n_paths + stride - 1) / stride);
@@ -1365,7 +1375,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
                 a.rec_stride = stride;
                 a.rec_phase = g->train_passes % stride;
                 const uint32_t slots = (n_paths - a.rec_phase + stride - 1) / stride;
-                n_rec = std::min(slots, rec_slots_max) * d->max_depth;
+                n_rec = std::min(slots, rec_slots_max) * rec_depth;
                 a.rec_cap = n_rec;
                 hipLaunchKernelGGL(k_zero_f4, dim3((2 * n_rec + 255) / 256), dim3(256), 0, c->stream, a.rec_aux,
                                    2 * n_rec);

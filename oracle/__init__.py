@@ -43,10 +43,11 @@ class RenderDesc(ctypes.Structure):
 
 class GuideCfg(ctypes.Structure):
     _fields_ = [("params", ctypes.c_void_p), ("bmin", ctypes.c_float * 3), ("binv", ctypes.c_float * 3),
-                ("bsdf_fraction", ctypes.c_float), ("min_alpha", ctypes.c_float), ("sample", _i32)]
+                ("bsdf_fraction", ctypes.c_float), ("min_alpha", ctypes.c_float), ("sample", _i32),
+                ("max_depth", ctypes.c_uint32)]
 
 
-def make_guide(params, lo, hi, bsdf_fraction=0.5, min_alpha=0.1, sample=True):
+def make_guide(params, lo, hi, bsdf_fraction=0.5, min_alpha=0.1, sample=True, max_depth=0):
     """Guide settings for the guided oracle paths; the box inverse is formed
     in FP32 as the HIP library forms it (1 / (hi - lo), 1 where the extent is 0)."""
     params = np.ascontiguousarray(params, np.float32)
@@ -60,6 +61,7 @@ def make_guide(params, lo, hi, bsdf_fraction=0.5, min_alpha=0.1, sample=True):
     g.bsdf_fraction = bsdf_fraction
     g.min_alpha = min_alpha
     g.sample = 1 if sample else 0
+    g.max_depth = max_depth
     g._keep = params   # the parameters must outlive the calls
     return g
 

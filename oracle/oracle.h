@@ -205,14 +205,16 @@ int or_integrate_rays(const or_scene* s, const float* rays, uint32_t n,
 
 /* Guided variants (path-guiding network; no reference counterpart). Every
  * vertex draws 12 numbers; with sample != 0, guided vertices (Lambert, or
- * conductor with alpha >= min_alpha) sample the network with probability
- * 1 - bsdf_fraction and weight by the one-sample MIS density. The network
- * position input is (P - bmin) * binv in FP32. */
+ * conductor with alpha >= min_alpha, at depth <= max_depth unless max_depth
+ * is 0) sample the network with probability 1 - bsdf_fraction and weight by
+ * the one-sample MIS density. The network position input is (P - bmin) * binv
+ * in FP32. */
 typedef struct {
     const float* params;
     float bmin[3], binv[3];
     float bsdf_fraction, min_alpha;
     int32_t sample;
+    uint32_t max_depth;
 } or_guide_cfg;
 int or_render_guided(const or_scene* s, const or_render_desc* d, const or_guide_cfg* g,
                      double* out, float* sample0, int threads);

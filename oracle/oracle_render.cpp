@@ -682,6 +682,7 @@ struct Guide {
     V3 bmin, binv;
     float frac, min_alpha;
     bool sample;   // sample from the network (else BSDF only, 12 draws per vertex)
+    uint32_t max_depth;   // guided vertices: depth <= max_depth (0: any depth)
 };
 
 V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 dir,
@@ -765,7 +766,8 @@ V3 trace_path(const or_scene& s, const or_render_desc& d, Pcg32& rng, V3 o, V3 d
         Frame fr(n);
         V3 wo = fr.to_local(wo_w);
         const LobeProbs lp = lobe_probs(m, beta);
-        const bool guided = g && g->sample && !m.dielectric && m.guide_alpha >= g->min_alpha;
+        const bool guided = g && g->sample && !m.dielectric && m.guide_alpha >= g->min_alpha &&
+                            (g->max_depth == 0 || depth <= g->max_depth);
         double gout[32];
         if (guided) {
             const float in6[6] = {(P.x - g->bmin.x) * g->binv.x, (P.y - g->bmin.y) * g->binv.y,
@@ -991,6 +993,7 @@ static Guide make_guide(const or_guide_cfg* c) {
     g.frac = c->bsdf_fraction;
     g.min_alpha = c->min_alpha;
     g.sample = c->sample != 0;
+    g.max_depth = c->max_depth;
     return g;
 }
 

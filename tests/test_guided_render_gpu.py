@@ -51,20 +51,22 @@ def camera_rays(s):
     return r
 
 
-@pytest.mark.parametrize("scene", ["guiding", "materials"])
-def test_guided_paths_match_oracle(scene):
+@pytest.mark.parametrize("scene,gdepth", [("guiding", 0), ("materials", 0), ("guiding", 2)])
+def test_guided_paths_match_oracle(scene, gdepth):
+    # gdepth > 0: vertices deeper than gdepth are not guided (no network query)
     if scene == "guiding":
         s = scenes.guiding_scene(width=64, height=64, spp=1, max_depth=6, n_spheres=16)
     else:
         s = scenes.materials_scene(width=48, height=48, spp=1)
     lo, hi = scene_box(s)
     p = random_guide_params(5)
-    c = guided_ctx(s, params=p, bsdf_fraction=0.5, min_alpha=0.1, box=(lo, hi))
+    c = guided_ctx(s, params=p, bsdf_fraction=0.5, min_alpha=0.1, box=(lo, hi), max_guided_depth=gdepth)
     rays = camera_rays(s)
     g = c.integrate_rays(rays, desc(s, guided=True)).astype(np.float64)
     c.close()
     o = oracle_scene(s).integrate_rays(rays, odesc(s), threads=CPU_THREADS,
-                                       guide=oracle.make_guide(p, lo, hi, 0.5, 0.1)).astype(np.float64)
+                                       guide=oracle.make_guide(p, lo, hi, 0.5, 0.1, max_depth=gdepth)
+                                       ).astype(np.float64)
     close = np.all(np.abs(g - o) <= 2e-3 * np.abs(o) + 1e-5, axis=1)
     lit = np.any(o > 0, axis=1)
     print("%s: %d paths, %.4f agree, %d lit" % (scene, close.size, close.mean(), lit.sum()))
@@ -76,9 +78,12 @@ def test_guided_paths_match_oracle(scene):
     assert close.mean() >= 0.995
 
 
-def test_guided_training_render_keeps_furnace_mean():
+@pytest.mark.parametrize("gdepth", [0, 2])
+def test_guided_training_render_keeps_furnace_mean(gdepth):
+    # gdepth 2: two records per recorded path (the guided vertices only)
     s = scenes.furnace_box(albedo=0.6, Le=1.0, max_depth=4, rr_depth=2, spp=64, width=32, height=32)
-    c = guided_ctx(s, bsdf_fraction=0.5, lr=5e-3, train_steps=2, guide_after=2, pass_paths=1024)
+    c = guided_ctx(s, bsdf_fraction=0.5, lr=5e-3, train_steps=2, guide_after=2, pass_paths=1024,
+                   max_guided_depth=gdepth)
     img = c.render(desc(s, guided=True))
     st = c.render_stats()
     c.close()

@@ -79,8 +79,8 @@ def random_guide_params(seed, out_scale=3.0):
     return p
 
 
-@pytest.mark.parametrize("albedo,depth,rr", [(0.5, 4, 64), (0.7, 5, 2)])
-def test_guided_furnace_closed_form(albedo, depth, rr):
+@pytest.mark.parametrize("albedo,depth,rr,gdepth", [(0.5, 4, 64, 0), (0.7, 5, 2, 0), (0.7, 5, 64, 2)])
+def test_guided_furnace_closed_form(albedo, depth, rr, gdepth):
     # One-sample MIS between the BSDF and an arbitrary (here random, sharply
     # peaked) network density must stay unbiased, and next-event estimation
     # must weight against the same combined density: any slip in either
@@ -88,7 +88,9 @@ def test_guided_furnace_closed_form(albedo, depth, rr):
     # the mean away from the closed form.
     s = scenes.furnace_box(albedo=albedo, Le=1.0, max_depth=depth, rr_depth=rr, spp=128)
     lo, hi = s.soup().reshape(-1, 3).min(0), s.soup().reshape(-1, 3).max(0)
-    g = oracle.make_guide(random_guide_params(7), lo, hi, bsdf_fraction=0.5)
+    # gdepth > 0: only the first gdepth vertices are guided, deeper ones sample
+    # the BSDF alone; the mean must not move either
+    g = oracle.make_guide(random_guide_params(7), lo, hi, bsdf_fraction=0.5, max_depth=gdepth)
     img = oracle_scene(s).render(desc(s), guide=g)
     expect = scenes.furnace_expected(albedo, 1.0, depth)
     pix = img[..., 0].ravel()
