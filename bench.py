@@ -177,7 +177,70 @@ def _render_terrain(s, instances):
             "image_mean": round(float(img.mean()), 5), "finite": bool(np.isfinite(img).all())}
 
 
-def path_trace_leg(args):
+def cornell_cpu_gpu(threads):
+    """The Cornell box workload (256^2, 16 spp, depth 5) rendered by the GPU
+    tracer and by the CPU oracle integrator (FP64 film) on `threads` host
+    threads: samples/s of both, and the CPU's rays/s from the GPU's ray count
+    (both trace the same paths)."""
+    import oracle
+    from kazen_proto_amd import native, scenes
+    s = scenes.cornell_box()
+    c = native.Context(0)
+    c.load_scene(s)
+    c.render(native.make_render_desc(s, spp=1))     # warm-up
+    c.render(native.make_render_desc(s))
+    rs = c.render_stats()
+    c.close()
+    rays = rs["extension_rays"] + rs["shadow_rays"]
+    sc = oracle.scene_from(s)
+    d = oracle.RenderDesc()
+    d.width, d.height, d.spp = s.width, s.height, s.spp
+    d.max_depth, d.rr_depth, d.seed, d.spp_offset = s.max_depth, s.rr_depth, s.seed, 0
+    d.eye[:], d.lookat[:], d.up[:] = s.eye, s.lookat, s.up
+    d.tan_half_fov = s.tan_half_fov
+    sc.render(d, threads=threads)                    # warm-up
+    runs = []
+    for _ in range(5):
+        t0 = time.time()
+        sc.render(d, threads=threads)
+        runs.append(time.time() - t0)
+    sec = float(np.median(runs))
+    n = s.width * s.height * s.spp
+    return {"workload": "Cornell box %dx%d, %d spp, depth %d" % (s.width, s.height, s.spp, s.max_depth),
+            "cpu_samples_per_s": round(n / sec, 1), "cpu_mrays_per_s": round(rays / sec / 1e6, 3),
+            "gpu_samples_per_s": round(rs["paths"] / rs["seconds"], 1),
+            "gpu_over_cpu": round((rs["paths"] / rs["seconds"]) / (n / sec), 1),
+            "sample": "whole image, 1 warm-up then median of 5 (%.3fs each)" % sec}
+
+
+def terrain_cpu(s, gpu, threads):
+    """CPU oracle integrator on every 16th image row of the terrain workload
+    (all spp): samples/s, extrapolated whole-frame seconds."""
+    import oracle
+    t0 = time.time()
+    sc = oracle.scene_from(s)
+    build = time.time() - t0
+    rows = np.arange(0, s.height, 16)
+    px = (rows[:, None] * s.width + np.arange(s.width)[None, :]).ravel()
+    d = oracle.RenderDesc()
+    d.width, d.height, d.spp = s.width, s.height, s.spp
+    d.max_depth, d.rr_depth, d.seed, d.spp_offset = s.max_depth, s.rr_depth, s.seed, 0
+    d.eye[:], d.lookat[:], d.up[:] = s.eye, s.lookat, s.up
+    d.tan_half_fov = s.tan_half_fov
+    t0 = time.time()
+    sc.render_pixels(d, px, threads=threads)
+    sec = time.time() - t0
+    n = px.size * s.spp
+    frac = px.size / float(s.width * s.height)
+    return {"cpu_samples_per_s": round(n / sec, 1),
+            "gpu_over_cpu": round(gpu["samples_per_s"] / (n / sec), 1),
+            "sample": "every 16th row (%d pixels, %.4f of the frame), %d spp, one run of %.2fs"
+                      % (px.size, frac, s.spp, sec),
+            "cpu_frame_seconds_extrapolated": round(sec / frac, 2),
+            "cpu_bvh_build_seconds": round(build, 2)}
+
+
+def path_trace_leg(args, cpu_threads=None):
     """Plain wavefront path trace of the terrain workload (2048^2 fBm
     heightfield, 256 instanced spheres, three material families, 4 area
     lights) at args.pt_res^2 x args.pt_spp, depth 8: samples/s and rays/s
@@ -197,6 +260,8 @@ def path_trace_leg(args):
            "triangles": int(s.n_tris)}
     out.update(_render_terrain(s, native.INSTANCES_AUTO))
     out["two_level_traversal"] = _render_terrain(s, native.INSTANCES_TWO_LEVEL)
+    if cpu_threads:
+        out["cpu"] = dict(terrain_cpu(s, out, cpu_threads), workload=out["workload"])
     print("[bench] path trace: %s" % out, file=sys.stderr, flush=True)
     return out
 
@@ -427,19 +492,28 @@ def main():
         import oracle
         host = host_info()
         threads = args.cpu_threads or min(host["affinity_cpus"], host["cpu_share"] or host["affinity_cpus"])
+        t0 = time.time()
         bvh = oracle.Bvh(tris)
+        cpu_build = time.time() - t0
         bvh.closest(inc[:65536], threads=threads)           # warm-up
-        runs = []
-        hits_cpu = None
-        for _ in range(3):
-            hits_cpu, sec = bvh.closest_timed(inc, threads=threads)
-            runs.append(sec)
-        sec = float(np.median(runs))
+
+        def cpu_median(rays):
+            runs, hits = [], None
+            for _ in range(5):
+                hits, sec = bvh.closest_timed(rays, threads=threads)
+                runs.append(sec)
+            return hits, float(np.median(runs))
+
+        hits_cpu, sec = cpu_median(inc)
+        _, sec_p = cpu_median(prim)
         cpu = {"value": round(inc.shape[0] / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                "kind": "port",
                "sample": "all %d incoherent rays, oracle binary-SAH BVH + watertight test, "
-                         "median of 3 runs (%.2fs each)" % (inc.shape[0], sec),
-               "host": host, "gpu_over_cpu": round(mrays_inc / (inc.shape[0] / sec / 1e6), 1)}
+                         "1 warm-up then median of 5 runs (%.2fs each)" % (inc.shape[0], sec),
+               "host": host, "gpu_over_cpu": round(mrays_inc / (inc.shape[0] / sec / 1e6), 1),
+               "primary_mrays_per_s": round(prim.shape[0] / sec_p / 1e6, 3),
+               "bvh_build_seconds": round(cpu_build, 3)}
+        cpu["renders"] = {"cornell": cornell_cpu_gpu(threads)}
         got = h_inc.numpy()
         mm = int(np.count_nonzero(np.any(got.view(np.uint32) != hits_cpu.view(np.uint32), axis=1)))
         ties = bvh.count_near_ties(inc, hits_cpu, threads=threads)
@@ -449,7 +523,9 @@ def main():
 
     path_trace = None
     if not args.no_pathtrace:
-        path_trace = path_trace_leg(args)
+        path_trace = path_trace_leg(args, None if args.no_cpu or cpu is None else cpu["cores"])
+        if cpu is not None and path_trace.get("cpu"):
+            cpu["renders"]["terrain"] = path_trace.pop("cpu")
     batch = None
     if not args.no_batch:
         batch = batch_leg_child(args)

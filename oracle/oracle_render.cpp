@@ -32,7 +32,12 @@
 //   D9 emission MIS uses the pdf of the light actually hit (integrator.cpp:452-461);
 //   D10 all randomness from the per-path PCG32 stream (sampling.h:15-23);
 //   D11 occlusion = any hit in [tmin,tmax] (hitable.h:28);
-//   D16 no composite-closure lobe mixing: one closure per material (bsdf.cpp:17-55);
+//   D16 a composite closure samples lobe i with the probabilities its
+//       compute_pdfs forms (dot(w_i, beta) normalised, bsdf.h:157-176, with
+//       beta the path throughput as integrator.cpp:425 passes it); the
+//       reference draws the lobe uniformly but divides by those probabilities
+//       (bsdf.cpp:32,38).  f and pdf are the one-sample-MIS sums over all
+//       lobes, as bsdf.cpp:40-52 forms them;
 //   D17 explicit pinhole camera with vertical fov (camera.h:67-96);
 //   D18 ray offsetting after Waechter & Binder, Ray Tracing Gems ch. 6, the
 //       method intersection.h:78-91 cites (its implementation has an
@@ -45,7 +50,11 @@
 //       camera path tracer over-weights every refraction into the camera's
 //       medium by eta^2 (2.25 at eta 1.5).  Pinned by the smooth-limit albedo
 //       R + (1 - R)/eta^2 from outside, R + (1 - R) eta^2 from inside, and the
-//       rough albedo written over microfacet normals.
+//       rough albedo written over microfacet normals;
+//   D21 the shading-point offset of post_hit (shape.cpp:615-654, Hanika 2021)
+//       is formed with world-space vertex normals and offsets; the reference
+//       mixes object-space normals with object-space offsets, which is the
+//       same for identity transforms and undefined otherwise.
 //   Beckmann Lambda uses the exact erf form rather than the rational fit of
 //   microfacet.h:134-141, so the VNDF pdf equals the sampled density; the
 //   conductor does not clamp D<=4 or f<=1 (bsdfs.cpp:706,716).

@@ -100,30 +100,35 @@ def _slit_scene(res, spp):
     return scenes.guiding_scene(width=res, height=res, spp=spp, max_depth=10)
 
 
-def test_trained_guide_beats_bsdf_sampling_at_equal_spp():
-    # Ablation of the learned part: train the network online during a
-    # 64-spp guided render, freeze it, then render fresh samples with it and
-    # compare against plain BSDF sampling at the same sample count.
-    res, spp = 96, 256
-    s = _slit_scene(res, spp)
+@pytest.mark.parametrize("room,n_spheres,min_gain", [("glossy", 64, 1.2), ("diffuse", 0, 1.35)])
+def test_trained_guide_beats_bsdf_sampling_at_equal_spp(room, n_spheres, min_gain):
+    # Ablation of the learned part: train the network online (lr 1e-3, 16
+    # Adam steps on each of 64 passes of 4 spp) during a 256-spp guided
+    # render, freeze it, then render fresh samples with it and compare
+    # against plain BSDF sampling at the same sample count.  Measured gains
+    # (relMSE plain / guided) over training seeds: 1.30-1.47 with the 64
+    # glossy spheres (the C5 room), 1.46-1.67 in the same room without them.
+    res, spp = 128, 256
+    s = scenes.guiding_scene(width=res, height=res, spp=spp, max_depth=10, n_spheres=n_spheres)
     ref_ctx = native.Context(0)
     ref_ctx.load_scene(s)
     ref = ref_ctx.render(desc(s, spp=8192, seed=99)).astype(np.float64)
     plain = ref_ctx.render(desc(s, spp=spp, seed=5))
     ref_ctx.close()
-    cfg = dict(bsdf_fraction=0.5, min_alpha=0.5, lr=5e-3, train_steps=16, guide_after=4, pass_paths=res * res)
+    cfg = dict(bsdf_fraction=0.5, min_alpha=0.5, lr=1e-3, train_steps=16, guide_after=4,
+               pass_paths=4 * res * res)
     c = guided_ctx(s, **cfg)
-    c.render(desc(s, spp=64, guided=True, seed=3))
+    c.render(desc(s, spp=256, guided=True, seed=3))
     st = c.render_stats()
     c.guiding_configure(**dict(cfg, train_steps=0, guide_after=0))
     guided = c.render(desc(s, spp=spp, guided=True, seed=5))
     c.close()
     e_plain, e_guided = rel_mse(plain, ref), rel_mse(guided, ref)
-    print("slit room %dx%d @%d spp: relMSE plain %.4f guided (frozen after %d steps, %d records) %.4f"
-          % (res, res, spp, e_plain, st["train_steps"], st["train_records"], e_guided))
+    print("%s slit room %dx%d @%d spp: relMSE plain %.4f guided (frozen after %d steps) %.4f: gain %.3f"
+          % (room, res, res, spp, e_plain, st["train_steps"], e_guided, e_plain / e_guided))
     assert st["train_steps"] == 64 * 16 and st["train_records"] > 0
     assert np.isfinite(guided).all()
-    assert e_guided < 0.9 * e_plain
+    assert e_plain > min_gain * e_guided
 
 
 def test_slit_room_gpu_reference_matches_oracle():

@@ -105,6 +105,31 @@ def test_cornell_matches_fp64_oracle_image():
     assert abs(img.mean() - ref["ref"].mean()) < 0.01 * ref["ref"].mean()
 
 
+def test_cornell_pixels_within_confidence_intervals_of_fp64_oracle():
+    # Per-pixel check against the FP64 16,384-spp oracle image: 16 independent
+    # 256-spp GPU renders give each pixel's mean (4,096 spp) and standard
+    # error; the reference's own error is sigma / sqrt(16384).  With t = 15
+    # degrees of freedom, |z| > 4 has probability 0.1 % per pixel channel.
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "cornell64_ref.npz"))["ref"]
+    s = scenes.cornell_box(width=64, height=64)
+    c = gpu_ctx(s)
+    K, spp = 16, 256
+    imgs = np.stack([c.render(gdesc(s, spp=spp, seed=5000 + k)).astype(np.float64) for k in range(K)])
+    c.close()
+    mean = imgs.mean(0)
+    sd = imgs.std(0, ddof=1)   # of one 256-spp image
+    se = sd * np.sqrt(1.0 / K + spp / 16384.0)
+    flat = se == 0.0           # pixels with no variance (the light seen directly)
+    assert np.allclose(mean[flat], ref[flat], rtol=1e-4, atol=1e-6)
+    z = (mean[~flat] - ref[~flat]) / se[~flat]
+    inside = np.mean(np.abs(z) <= 4.0)
+    print("cornell 64x64: %d pixel channels, %.4f within 4 sigma, mean z^2 %.3f, mean z %.3f"
+          % (z.size, inside, np.mean(z * z), np.mean(z)))
+    assert inside >= 0.995
+    assert 0.7 < np.mean(z * z) < 1.5     # t(15) variance 1.15
+    assert abs(np.mean(z)) < 0.1          # no bias of the whole image
+
+
 def test_all_closures_match_oracle_statistics():
     s = scenes.materials_scene(width=32, height=32)
     osc = oracle_scene(s)
