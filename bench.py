@@ -37,6 +37,7 @@ def profiled_traffic():
     except OSError:
         pass
     return None
+GUIDING_REF = os.path.join(ROOT, "tests", "golden", "guiding512_ref.npz")
 HEADLINE_METRIC = "incoherent Mrays/s (1 GPU); equal-relMSE time, guided vs plain GPU tracer"
 
 
@@ -77,13 +78,22 @@ def guiding_equal_error(args):
     c = native.Context(0)
     c.load_scene(s)
     t0 = time.time()
+    fp64 = None
+    if res == 512 and os.path.exists(GUIDING_REF) and not args.guiding_gpu_ref:
+        z = np.load(GUIDING_REF)
+        fp64 = z["ref"].astype(np.float64)
+        ref_desc = "FP64 CPU oracle integrator, %d spp, seed %d (%s)" % (int(z["spp"]), int(z["seed"]),
+                                                                        os.path.relpath(GUIDING_REF, ROOT))
     c.film_reset(res, res)
     chunk = 256
     done = 0
     ref_sec = 0.0
     paths = rays = 0
-    while done < args.guiding_ref_spp:
-        n = min(chunk, args.guiding_ref_spp - done)
+    # the plain GPU tracer's reference: the error judge when no FP64 image is
+    # there, and the plain throughput figure either way (1,024 spp then)
+    ref_spp = args.guiding_ref_spp if fp64 is None else 1024
+    while done < ref_spp:
+        n = min(chunk, ref_spp - done)
         d = native.make_render_desc(s, spp=n, spp_offset=done)
         d.seed = s.seed + 7919
         c.render_accumulate(d)
@@ -92,9 +102,15 @@ def guiding_equal_error(args):
         paths += st["paths"]
         rays += st["extension_rays"] + st["shadow_rays"]
         done += n
-        print("[bench] guiding reference %d/%d spp (%.1fs)" % (done, args.guiding_ref_spp, ref_sec),
+        print("[bench] guiding reference %d/%d spp (%.1fs)" % (done, ref_spp, ref_sec),
               file=sys.stderr, flush=True)
     ref = c.film_read(res, res).astype(np.float64)
+    if fp64 is not None:
+        gpu_ref_err = rel_mse(ref, fp64)     # the 1,024-spp GPU image against the FP64 one
+        ref = fp64
+    else:
+        ref_desc = "plain GPU tracer, %d spp (FP32 film, Kahan-compensated)" % ref_spp
+        gpu_ref_err = None
     plain_rate = {"samples_per_s": round(paths / ref_sec, 1), "mrays_per_s": round(rays / ref_sec / 1e6, 2)}
 
     def converge(guided):
@@ -148,8 +164,8 @@ def guiding_equal_error(args):
     if plain["seconds_to_target"] and guided["seconds_to_target"]:
         ratio = round(guided["seconds_to_target"] / plain["seconds_to_target"], 4)
     return {"workload": "slit-lit room, 64 GGX spheres, %dx%d, depth 10" % (res, res),
-            "target_relmse": args.guiding_target, "reference_spp": args.guiding_ref_spp,
-            "reference": "plain GPU tracer (FP32 film, Kahan-compensated)",
+            "target_relmse": args.guiding_target, "reference": ref_desc,
+            "plain_1024spp_relmse_vs_fp64": None if gpu_ref_err is None else round(gpu_ref_err, 5),
             "plain": plain, "guided": guided, "time_ratio_guided_over_plain": ratio,
             "plain_throughput": plain_rate, "wall_seconds": round(time.time() - t0, 1)}
 
@@ -363,6 +379,7 @@ def main():
                     help="first run one device copy kernel of MIB MiB (reference kernel of "
                          "known traffic for HBM counter calibration)")
     ap.add_argument("--no-pathtrace", action="store_true", help="skip the terrain path-trace leg")
+    ap.add_argument("--no-width-tune", action="store_true", help="skip the node-width tuner on the microbench")
     ap.add_argument("--pt-grid", type=int, default=2048)
     ap.add_argument("--pt-res", type=int, default=1024)
     ap.add_argument("--pt-spp", type=int, default=64)
@@ -374,7 +391,10 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain equal-error leg")
     ap.add_argument("--guiding-res", type=int, default=512)
-    ap.add_argument("--guiding-ref-spp", type=int, default=16384)
+    ap.add_argument("--guiding-ref-spp", type=int, default=16384,
+                    help="spp of the plain GPU reference when the FP64 oracle image is not used")
+    ap.add_argument("--guiding-gpu-ref", action="store_true",
+                    help="judge the error against the plain GPU tracer instead of the FP64 oracle image")
     ap.add_argument("--guiding-target", type=float, default=0.01)
     ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
     ap.add_argument("--guiding-lr", type=float, default=1e-3)
@@ -521,6 +541,19 @@ def main():
                   "near_ties_note": "rays whose closest hit has a second primitive within 1 ulp of t "
                                     "(resolved by the lowest primitive id on both sides)"}
 
+    # build-time node-width tuner (KP_NODE_WIDTH_AUTO): widths 4 / 6 / 8 timed
+    # on its own probe rays; the benchmark rays then timed at the width kept
+    width_tuning = None
+    if not args.no_width_tune:
+        ctx.bvh_build(leaf_size=args.leaf, node_width=native.NODE_WIDTH_AUTO)
+        wst = ctx.stats()
+        ms_w = timed(d_inc, h_inc, max(1, args.steps // 2))
+        width_tuning = {"probe_ms": {str(4 + 2 * i): round(m, 4) for i, m in enumerate(wst["width_ms"])},
+                        "chosen": int(wst["node_width"]),
+                        "incoherent_mrays_per_s_at_chosen": round(n / (ms_w * 1e-3) / 1e6, 2),
+                        "build_seconds_all_widths": round(wst["build_seconds"], 3)}
+        print("[bench] node-width tuner: %s" % width_tuning, file=sys.stderr, flush=True)
+
     path_trace = None
     if not args.no_pathtrace:
         path_trace = path_trace_leg(args, None if args.no_cpu or cpu is None else cpu["cores"])
@@ -554,6 +587,7 @@ def main():
         "primary_mrays_per_s": mrays_prim,
         "parity": parity,
         "bvh_build_seconds": round(st["build_seconds"], 3),
+        "node_width_tuning": width_tuning,
         "path_trace": path_trace,
         "batch": batch,
         "guiding": guiding,

@@ -69,11 +69,19 @@ typedef struct {
 #define KP_INSTANCES_BAKE 2u
 #define KP_BAKE_BUDGET (1u << 24)
 
+/* BVH node width: children per node, 2..8 in the same 128-byte node record
+ * (0 means 8). KP_NODE_WIDTH_AUTO builds widths 4, 6 and 8, times each on
+ * 2^20 probe rays (origins uniform in the scene's bounds, directions uniform
+ * on the sphere) and keeps the fastest; one-level BVHs on a device only
+ * (otherwise 8). Hits do not depend on the width. */
+#define KP_NODE_WIDTH_AUTO 1u
+
 typedef struct {
     uint32_t leaf_size;   /* max triangles per leaf, 1..4 (default 4) */
     uint32_t sah_bins;    /* SAH bins, 4..64 (default 32) */
     uint32_t instances;   /* KP_INSTANCES_* (default AUTO) */
-    uint32_t reserved[5];
+    uint32_t node_width;  /* 2..8 children per node, 0 = 8, or KP_NODE_WIDTH_AUTO */
+    uint32_t reserved[4];
 } kp_build_opts;
 
 typedef struct {
@@ -88,7 +96,9 @@ typedef struct {
     double build_seconds;      /* host BVH build wall time */
     uint32_t device_cus;
     uint32_t two_level;        /* 1: instances traversed two-level; 0: one-level (baked) */
-    uint32_t reserved[6];
+    uint32_t node_width;       /* children per node of the built BVH */
+    float width_ms[3];         /* KP_NODE_WIDTH_AUTO: probe time (ms) of widths 4, 6, 8; else 0 */
+    uint32_t reserved[2];
 } kp_stats;
 
 /* Context ---------------------------------------------------------------- */

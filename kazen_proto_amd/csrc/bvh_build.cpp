@@ -177,6 +177,8 @@ bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh
     if (n_tris == 0) { err = "empty scene: no triangles"; return false; }
     if (n_tris >= (1u << 28)) { err = "too many triangles (limit 2^28)"; return false; }
     const uint32_t leaf_max = std::max(1u, std::min(opt.leaf_size ? opt.leaf_size : 4u, 4u));
+    if (opt.max_children < 2 || opt.max_children > 8) { err = "node width must be 2..8"; return false; }
+    const size_t max_children = opt.max_children;
     Builder b;
     b.tris = tris;
     b.n = n_tris;
@@ -230,7 +232,7 @@ bool build_bvh8(const float* tris, uint32_t n_tris, const BuildOptions& opt, Bvh
         } else {
             kids.push_back(root.a);
             kids.push_back(root.b);
-            while (kids.size() < 8) {
+            while (kids.size() < max_children) {
                 int pick = -1;
                 double best = -1.0;
                 for (size_t i = 0; i < kids.size(); ++i) {

@@ -642,15 +642,142 @@ int kp_instance_get(kp_ctx* c, uint32_t instance_id, uint32_t* blas_id, float* o
     return KP_OK;
 }
 
+// Makes h the context's one-level BVH (host copy for the per-ray queries,
+// device copy and stack spill area on a device context).
+static int install_one_level(kp_ctx* c, kp::Bvh8Host& h, uint32_t n_all, uint32_t n_world, uint32_t width) {
+    if (c->device >= 0) KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    kp::free_bvh(c);
+    // Worst-case stack: 7 pushes per level below the root, plus slack; the
+    // top KP_TRACE_STACK entries live in LDS.
+    c->spill_cap = 7u * h.max_depth + 8u;
+    if (c->device >= 0) {
+        const size_t nb = h.nodes.size() * sizeof(KpNode8), tb = h.tris.size() * sizeof(float);
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tb));
+        KP_HIP_TRY(c, hipMemcpy(c->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice));
+        KP_HIP_TRY(c, hipMemcpy(c->d_tris, h.tris.data(), tb, hipMemcpyHostToDevice));
+        c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
+        KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
+    }
+    c->n_tris_built = n_all;
+    c->n_tris_stored = n_all;
+    c->n_world_prims = n_world;
+    c->n_nodes = (uint32_t)h.nodes.size();
+    c->n_bvh2 = h.n_bvh2_nodes;
+    c->max_depth = h.max_depth;
+    c->node_width = width;
+    // A traversal visits every node and leaf at most once.
+    c->max_steps = 2u * (uint32_t)h.nodes.size() + n_all + 64u;
+    // the host copy answers the thread-safe per-ray queries
+    c->host.nodes = std::move(h.nodes);
+    c->host.tris = std::move(h.tris);
+    c->built = true;
+    return KP_OK;
+}
+
+// Build-time node-width tuner: builds widths 4, 6 and 8, times closest hit
+// of 2^20 probe rays (origins uniform in the scene's bounds, directions
+// uniform on the sphere; a fixed generator) on each, and leaves the fastest
+// installed.  ms[i] receives the mean launch time of width 4 + 2i.
+static int tune_node_width(kp_ctx* c, const float* soup, uint32_t n_all, uint32_t n_world, kp::BuildOptions bo,
+                           float* ms, double* build_secs) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < (size_t)n_all * 3; ++i) {
+        const int a = (int)(i % 3);
+        lo[a] = std::min(lo[a], soup[3 * (i / 3) + a]);
+        hi[a] = std::max(hi[a], soup[3 * (i / 3) + a]);
+    }
+    const uint32_t n_probe = 1u << 20;
+    std::vector<float> rays((size_t)n_probe * 8);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto next = [&]() {   // splitmix64 -> [0, 1)
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (float)((z ^ (z >> 31)) >> 40) * (1.0f / 16777216.0f);
+    };
+    for (uint32_t i = 0; i < n_probe; ++i) {
+        float* r = &rays[8 * (size_t)i];
+        for (int a = 0; a < 3; ++a) r[a] = lo[a] + (hi[a] - lo[a]) * next();
+        const float z = 1.0f - 2.0f * next(), phi = 6.2831853f * next();
+        const float rr = std::sqrt(std::max(0.0f, 1.0f - z * z));
+        r[3] = 0.0f;
+        r[4] = rr * std::cos(phi);
+        r[5] = rr * std::sin(phi);
+        r[6] = z;
+        r[7] = INFINITY;
+    }
+    float4 *d_rays = nullptr, *d_hits = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = KP_OK;
+    float best = INFINITY;
+    uint32_t best_w = 8;
+    kp::Bvh8Host keep;
+    double secs = 0.0;
+    auto fail_hip = [&](hipError_t e, const char* what) { rc = kp::hip_fail(c, e, what); };
+    hipError_t e;
+    if ((e = hipMalloc((void**)&d_rays, (size_t)n_probe * 32)) != hipSuccess) { fail_hip(e, "width tuner"); goto done; }
+    if ((e = hipMalloc((void**)&d_hits, (size_t)n_probe * 16)) != hipSuccess) { fail_hip(e, "width tuner"); goto done; }
+    if ((e = hipMemcpy(d_rays, rays.data(), (size_t)n_probe * 32, hipMemcpyHostToDevice)) != hipSuccess) {
+        fail_hip(e, "width tuner");
+        goto done;
+    }
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+        fail_hip(e, "width tuner");
+        goto done;
+    }
+    for (int i = 0; i < 3; ++i) {
+        const uint32_t w = 4u + 2u * (uint32_t)i;
+        bo.max_children = w;
+        kp::Bvh8Host h;
+        std::string err;
+        auto t0 = std::chrono::steady_clock::now();
+        if (!kp::build_bvh8(soup, n_all, bo, h, err)) { rc = kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: " + err); goto done; }
+        secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        kp::Bvh8Host copy = h;   // the installed BVH moves its host arrays
+        if ((rc = install_one_level(c, h, n_all, n_world, w))) goto done;
+        if ((rc = kp::trace_closest_dev(c, d_rays, n_probe, d_hits, false))) goto done;   // warm-up
+        if ((e = hipEventRecord(e0, c->stream)) != hipSuccess) { fail_hip(e, "width tuner"); goto done; }
+        for (int k = 0; k < 3; ++k)
+            if ((rc = kp::trace_closest_dev(c, d_rays, n_probe, d_hits, false))) goto done;
+        if ((e = hipEventRecord(e1, c->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess) {
+            fail_hip(e, "width tuner");
+            goto done;
+        }
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        ms[i] = t / 3.0f;
+        if (ms[i] < best) {
+            best = ms[i];
+            best_w = w;
+            keep = std::move(copy);
+        }
+    }
+    if (best_w != 8u && (rc = install_one_level(c, keep, n_all, n_world, best_w))) goto done;
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d_rays) (void)hipFree(d_rays);
+    if (d_hits) (void)hipFree(d_hits);
+    *build_secs = secs;
+    return rc;
+}
+
 int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
     if (!c) return KP_ERR_INVALID;
     if (c->device >= 0) KP_USE_DEVICE(c);
     const uint32_t n = (uint32_t)(c->soup.size() / 9);
     kp::BuildOptions bo;
+    bool auto_width = false;
     if (o) {
         if (o->leaf_size) bo.leaf_size = o->leaf_size;
         if (o->sah_bins) bo.sah_bins = o->sah_bins;
+        if (o->node_width == KP_NODE_WIDTH_AUTO) auto_width = true;
+        else if (o->node_width > 8 || (o->node_width != 0 && o->node_width < 2))
+            return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: node_width must be 2..8, 0 or KP_NODE_WIDTH_AUTO");
+        else if (o->node_width) bo.max_children = o->node_width;
     }
+    for (float& m : c->width_ms) m = 0.0f;
     // first primitive id of every instance (the same in both instance modes)
     c->inst_first.resize(c->inst_blas.size());
     {
@@ -672,7 +799,10 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
         if (two) {
             double secs = 0.0;
             const int rc = kp::build_two_level(c, bo, &secs);
-            if (rc == KP_OK) c->build_seconds = secs;
+            if (rc == KP_OK) {
+                c->build_seconds = secs;
+                c->node_width = bo.max_children;
+            }
             return rc;
         }
         if ((uint64_t)n + placed >= (1ull << 28))
@@ -681,38 +811,22 @@ int kp_bvh_build(kp_ctx* c, const kp_build_opts* o) {
         soup = baked.data();
         n_all = (uint32_t)(baked.size() / 9);
     }
+    if (auto_width && c->device >= 0 && n_all > 0) {
+        double secs = 0.0;
+        const int rc = tune_node_width(c, soup, n_all, n, bo, c->width_ms, &secs);
+        if (rc) return rc;
+        c->build_seconds = secs;
+        kp::render_scene_dirty(c);
+        return KP_OK;
+    }
     kp::Bvh8Host h;
     std::string err;
     auto t0 = std::chrono::steady_clock::now();
     if (!kp::build_bvh8(soup, n_all, bo, h, err)) return kp::fail(c, KP_ERR_INVALID, "kp_bvh_build: " + err);
     auto t1 = std::chrono::steady_clock::now();
-    if (c->device >= 0) KP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-    kp::free_bvh(c);
-    // Worst-case stack: 7 pushes per level below the root, plus slack; the
-    // top KP_TRACE_STACK entries live in LDS.
-    c->spill_cap = 7u * h.max_depth + 8u;
-    if (c->device >= 0) {
-        const size_t nb = h.nodes.size() * sizeof(KpNode8), tb = h.tris.size() * sizeof(float);
-        KP_HIP_TRY(c, hipMalloc((void**)&c->d_nodes, nb));
-        KP_HIP_TRY(c, hipMalloc((void**)&c->d_tris, tb));
-        KP_HIP_TRY(c, hipMemcpy(c->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice));
-        KP_HIP_TRY(c, hipMemcpy(c->d_tris, h.tris.data(), tb, hipMemcpyHostToDevice));
-        c->spill_lanes = (size_t)kp::max_trace_grid(c) * KP_TRACE_BLOCK;
-        KP_HIP_TRY(c, hipMalloc((void**)&c->d_spill, c->spill_lanes * c->spill_cap * sizeof(uint2)));
-    }
-    c->n_tris_built = n_all;
-    c->n_tris_stored = n_all;
-    c->n_world_prims = n;
-    c->n_nodes = (uint32_t)h.nodes.size();
-    c->n_bvh2 = h.n_bvh2_nodes;
-    c->max_depth = h.max_depth;
-    // A traversal visits every node and leaf at most once.
-    c->max_steps = 2u * (uint32_t)h.nodes.size() + n_all + 64u;
+    const int rc = install_one_level(c, h, n_all, n, bo.max_children);
+    if (rc) return rc;
     c->build_seconds = std::chrono::duration<double>(t1 - t0).count();
-    // the host copy answers the thread-safe per-ray queries
-    c->host.nodes = std::move(h.nodes);
-    c->host.tris = std::move(h.tris);
-    c->built = true;
     kp::render_scene_dirty(c);
     return KP_OK;
 }
@@ -732,6 +846,8 @@ int kp_get_stats(kp_ctx* c, kp_stats* s) {
     s->build_seconds = c->build_seconds;
     s->device_cus = (uint32_t)c->cus;
     s->two_level = c->two_level ? 1u : 0u;
+    s->node_width = c->node_width;
+    for (int i = 0; i < 3; ++i) s->width_ms[i] = c->width_ms[i];
     return KP_OK;
 }
 

@@ -35,6 +35,11 @@ int kp_bvh8_build_host(const float* tris9, uint32_t n, const kp_build_opts* o, k
     if (o) {
         if (o->leaf_size) bo.leaf_size = o->leaf_size;
         if (o->sah_bins) bo.sah_bins = o->sah_bins;
+        if (o->node_width > 8 || o->node_width == 1) {   // no device to tune on: AUTO is not accepted here
+            kp::set_thread_error("kp_bvh8_build_host: node_width must be 2..8 or 0");
+            return KP_ERR_INVALID;
+        }
+        if (o->node_width) bo.max_children = o->node_width;
     }
     kp::Bvh8Host h;
     std::string err;

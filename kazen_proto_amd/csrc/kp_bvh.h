@@ -12,6 +12,7 @@ namespace kp {
 struct BuildOptions {
     uint32_t leaf_size = 4;   // max triangles per leaf (1..4)
     uint32_t sah_bins = 32;   // SAH bins per axis (4..64)
+    uint32_t max_children = 8;   // node width (2..8)
 };
 
 struct Bvh8Host {

@@ -48,6 +48,8 @@ struct kp_ctx {
     uint32_t n_tris_built = 0;       // primitive ids (instanced: every placement counted)
     uint32_t n_tris_stored = 0;      // triangles stored on the device
     uint32_t n_nodes = 0, n_bvh2 = 0, max_depth = 0;
+    uint32_t node_width = 8;   // children per node of the built BVH
+    float width_ms[3] = {0.0f, 0.0f, 0.0f};   // node-width tuner: probe ms of widths 4, 6, 8
     double build_seconds = 0.0;
     kp::HostBvh host;                // host copy of the device BVH (per-ray queries on host cores)
     uint4* d_nodes = nullptr;

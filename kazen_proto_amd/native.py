@@ -35,18 +35,21 @@ class KpError(RuntimeError):
 
 
 INSTANCES_AUTO, INSTANCES_TWO_LEVEL, INSTANCES_BAKE = 0, 1, 2
+NODE_WIDTH_AUTO = 1    # kp_build_opts.node_width: tune 4 / 6 / 8 children per node on the device
 BAKE_BUDGET = 1 << 24
 
 
 class BuildOpts(ctypes.Structure):
-    _fields_ = [("leaf_size", _u32), ("sah_bins", _u32), ("instances", _u32), ("reserved", _u32 * 5)]
+    _fields_ = [("leaf_size", _u32), ("sah_bins", _u32), ("instances", _u32), ("node_width", _u32),
+                ("reserved", _u32 * 4)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("n_tris", _u64), ("n_nodes", _u64), ("n_bvh2_nodes", _u64),
                 ("bvh_bytes", _u64), ("max_depth", _u32), ("spill_entries", _u32),
                 ("grid_closest", _u32), ("grid_occluded", _u32), ("build_seconds", ctypes.c_double),
-                ("device_cus", _u32), ("two_level", _u32), ("reserved", _u32 * 6)]
+                ("device_cus", _u32), ("two_level", _u32), ("node_width", _u32),
+                ("width_ms", ctypes.c_float * 3), ("reserved", _u32 * 2)]
 
 
 class Material(ctypes.Structure):
@@ -266,12 +269,14 @@ class DeviceBuffer:
             pass
 
 
-def bvh8_build_host(tris9, leaf_size=4, sah_bins=32):
-    """Host-only BVH8 build -> (nodes uint8[n*128], tris float32[n*12], max_depth)."""
+def bvh8_build_host(tris9, leaf_size=4, sah_bins=32, node_width=0):
+    """Host-only BVH8 build -> (nodes uint8[n*128], tris float32[n*12], max_depth);
+    node_width 2..8 children per node (0: 8)."""
     tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
     o = BuildOpts()
     o.leaf_size = leaf_size
     o.sah_bins = sah_bins
+    o.node_width = node_width
     blob = _p()
     rc = lib().kp_bvh8_build_host(_np_ptr(tris9), tris9.shape[0], ctypes.byref(o), ctypes.byref(blob))
     if rc != KP_OK:
@@ -294,11 +299,12 @@ class HostBvh8:
     """Host-only BVH8 (kp_bvh8_build_host) with host closest-hit / occlusion
     queries (no device, thread-safe)."""
 
-    def __init__(self, tris9, leaf_size=4, sah_bins=32):
+    def __init__(self, tris9, leaf_size=4, sah_bins=32, node_width=0):
         tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
         o = BuildOpts()
         o.leaf_size = leaf_size
         o.sah_bins = sah_bins
+        o.node_width = node_width
         self._h = _p()
         rc = lib().kp_bvh8_build_host(_np_ptr(tris9), tris9.shape[0], ctypes.byref(o), ctypes.byref(self._h))
         if rc != KP_OK:
@@ -449,17 +455,21 @@ class Context:
         """Drops every mesh, prototype, instance and the BVH (materials stay)."""
         self._check(lib().kp_geometry_clear(self._h))
 
-    def bvh_build(self, leaf_size=4, sah_bins=32, instances=INSTANCES_AUTO):
+    def bvh_build(self, leaf_size=4, sah_bins=32, instances=INSTANCES_AUTO, node_width=0):
+        """node_width: children per node 2..8 (0: 8) or NODE_WIDTH_AUTO."""
         o = BuildOpts()
         o.leaf_size = leaf_size
         o.sah_bins = sah_bins
         o.instances = instances
+        o.node_width = node_width
         self._check(lib().kp_bvh_build(self._h, ctypes.byref(o)))
 
     def stats(self):
         s = Stats()
         self._check(lib().kp_get_stats(self._h, ctypes.byref(s)))
-        return {k: getattr(s, k) for k, _ in Stats._fields_ if k != "reserved"}
+        out = {k: getattr(s, k) for k, _ in Stats._fields_ if k != "reserved"}
+        out["width_ms"] = list(s.width_ms)
+        return out
 
     def set_trace_tuning(self, refill_below_lanes):
         self._check(lib().kp_set_trace_tuning(self._h, refill_below_lanes))

@@ -41,6 +41,30 @@ def test_blob_closest_equals_bruteforce(leaf):
     assert np.array_equal(occ.astype(bool), ref[:, 1].view(np.uint32) != MISS)
 
 
+@pytest.mark.parametrize("width", [2, 3, 4, 6, 7])
+def test_narrow_nodes_give_the_same_hits(width):
+    # node width (children per node) changes the tree, never the hits; the
+    # oracle's traversal of the same records agrees too
+    tris, rays = soup_and_rays(40 + width)
+    b = native.HostBvh8(tris, node_width=width)
+    ref = oracle.closest_bruteforce(tris, rays)
+    assert mismatches(b.closest(rays), ref) == 0
+    nodes, t12, depth = native.bvh8_build_host(tris, node_width=width)
+    wide, _, wide_depth = native.bvh8_build_host(tris)
+    assert nodes.size > wide.size and depth >= wide_depth
+    meta = nodes.reshape(-1, 128)[:, 24:32]
+    assert np.max(np.count_nonzero(meta, axis=1)) <= width
+    h, _, _ = oracle.bvh8_closest(nodes, t12, rays)
+    assert mismatches(h, ref) == 0
+
+
+def test_node_width_out_of_range_is_refused():
+    tris, _ = soup_and_rays(3, n_tris=100, n_rays=10)
+    for w in (1, 9):   # 1 (the device tuner's value) needs a device
+        with pytest.raises(native.KpError):
+            native.HostBvh8(tris, node_width=w)
+
+
 def test_closed_mesh_edges_and_vertices_do_not_leak():
     # rays through every vertex and edge midpoint of a closed sphere from its
     # centre: each must hit (watertight), and the hit equals brute force

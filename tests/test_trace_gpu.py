@@ -65,6 +65,27 @@ def test_refill_threshold_does_not_change_results(refill):
     c.close()
 
 
+@pytest.mark.parametrize("width", [2, 4, 6, native.NODE_WIDTH_AUTO])
+def test_node_width_does_not_change_results(width):
+    # narrower nodes (and the build-time width tuner) change the tree only
+    g = golden()
+    c = native.Context(0)
+    c.mesh_add_soup(g["tris"])
+    c.bvh_build(node_width=width)
+    st = c.stats()
+    assert mismatches(c.intersect_closest(g["rays"]), g["hits"]) == 0
+    occ_rays = g["rays"].copy()
+    occ_rays[:, 7] = 2.0
+    assert np.array_equal(c.occluded(occ_rays), oracle.Bvh(g["tris"]).occluded(occ_rays).astype(np.uint32))
+    c.close()
+    if width == native.NODE_WIDTH_AUTO:
+        print("width tuner: %s ms for widths 4/6/8, kept %d" % (st["width_ms"], st["node_width"]))
+        assert st["node_width"] in (4, 6, 8) and all(m > 0 for m in st["width_ms"])
+        assert st["width_ms"][(st["node_width"] - 4) // 2] == min(st["width_ms"])
+    else:
+        assert st["node_width"] == width and st["width_ms"] == [0.0, 0.0, 0.0]
+
+
 def test_occlusion_matches_oracle():
     g = golden()
     rays = g["rays"].copy()
