@@ -323,22 +323,18 @@ __global__ __launch_bounds__(256, 2) void k_guide_train(GuideNet net, const floa
             const float* src = in6 + 6 * (size_t)(valid ? base + smp : 0);
             uint32_t wd[6];
             if (part < 3) {
-                const float x = src[part];
+                float sn[6], cs[6];
+                g_octaves<6>(src[part], sn, cs);
 #pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    float sn, cs;
-                    sincospif(x * (float)(1 << k), &sn, &cs);
-                    wd[k] = valid ? pack_bf2(sn, cs) : 0u;
-                }
+                for (int k = 0; k < 6; ++k) wd[k] = valid ? pack_bf2(sn[k], cs[k]) : 0u;
             } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
+                for (int c = 0; c < 3; ++c) {
+                    float sn[2], cs[2];
+                    g_octaves<2>(src[3 + c], sn, cs);
 #pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        float sn, cs;
-                        sincospif(src[3 + c] * (float)(1 << k), &sn, &cs);
-                        wd[2 * c + k] = valid ? pack_bf2(sn, cs) : 0u;
-                    }
+                    for (int k = 0; k < 2; ++k) wd[2 * c + k] = valid ? pack_bf2(sn[k], cs[k]) : 0u;
+                }
             }
             uint32_t* row = act[0] + smp * KP_G_ROW;
 #pragma unroll

@@ -67,6 +67,21 @@ __device__ __forceinline__ f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c) {
                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
+// sin / cos of pi x 2^k for k = 0 .. N-1: k = 0 from sincospif, each higher
+// octave by the double-angle formulas sin 2a = 2 sin a cos a and
+// cos 2a = (cos a - sin a)(cos a + sin a).  The absolute error about doubles
+// per octave (about 2e-6 at k = 5), far below the bf16 spacing (2^-8 near 1)
+// the features are rounded to; 6 sincospif per sample instead of 24.
+template <int N>
+__device__ __forceinline__ void g_octaves(float x, float* s, float* c) {
+    sincospif(x, &s[0], &c[0]);
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+        s[k] = 2.0f * s[k - 1] * c[k - 1];
+        c[k] = (c[k - 1] - s[k - 1]) * (c[k - 1] + s[k - 1]);
+    }
+}
+
 // Frequency encoding of one sample into 48 features (see the file comment).
 __device__ __forceinline__ void g_encode(float px, float py, float pz, float nx, float ny, float nz,
                                          float* f) {
@@ -74,23 +89,25 @@ __device__ __forceinline__ void g_encode(float px, float py, float pz, float nx,
     const float nc[3] = {nx, ny, nz};
     int o = 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
+        float s[6], co[6];
+        g_octaves<6>(pc[c], s, co);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            float s, co;
-            sincospif(pc[c] * (float)(1 << k), &s, &co);
-            f[o++] = s;
-            f[o++] = co;
+            f[o++] = s[k];
+            f[o++] = co[k];
         }
+    }
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
+        float s[2], co[2];
+        g_octaves<2>(nc[c], s, co);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            float s, co;
-            sincospif(nc[c] * (float)(1 << k), &s, &co);
-            f[o++] = s;
-            f[o++] = co;
+            f[o++] = s[k];
+            f[o++] = co[k];
         }
+    }
 }
 
 // LDS row stride (in 32-bit words) of a 64-wide bf16 activation row: 32 words
@@ -230,21 +247,19 @@ __device__ __forceinline__ void g_encode_row(uint32_t* row, float px, float py, 
     uint32_t wd[32];
     int w = 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
+        float s[6], co[6];
+        g_octaves<6>(pc[c], s, co);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            float s, co;
-            sincospif(pc[c] * (float)(1 << k), &s, &co);
-            wd[w++] = pack_bf2(s, co);
-        }
+        for (int k = 0; k < 6; ++k) wd[w++] = pack_bf2(s[k], co[k]);
+    }
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
+        float s[2], co[2];
+        g_octaves<2>(nc[c], s, co);
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            float s, co;
-            sincospif(nc[c] * (float)(1 << k), &s, &co);
-            wd[w++] = pack_bf2(s, co);
-        }
+        for (int k = 0; k < 2; ++k) wd[w++] = pack_bf2(s[k], co[k]);
+    }
 #pragma unroll
     for (; w < 32; ++w) wd[w] = 0u;
     // 16-byte stores: a lane's row is 8 stores, not 32
