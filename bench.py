@@ -120,6 +120,7 @@ def guiding_equal_error(args):
                                 train_steps=args.guiding_steps, guide_after=args.guiding_after,
                                 max_records=1 << 18, pass_paths=args.guiding_pass_paths,
                                 train_passes=args.guiding_train_passes,
+                                late_train_steps=args.guiding_late_steps,
                                 max_guided_depth=args.guiding_depth)
         c.film_reset(res, res)
         spp = 0
@@ -403,8 +404,10 @@ def main():
     ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
     ap.add_argument("--guiding-pass-paths", type=int, default=1 << 24,
                     help="paths per guided pass (each pass trains once)")
-    ap.add_argument("--guiding-train-passes", type=int, default=0,
-                    help="passes that train the network before it is frozen (0 = all: online)")
+    ap.add_argument("--guiding-train-passes", type=int, default=32,
+                    help="passes that take --guiding-steps Adam steps (0 = all)")
+    ap.add_argument("--guiding-late-steps", type=int, default=1,
+                    help="Adam steps per pass after them (online training continues; 0 = frozen)")
     ap.add_argument("--guiding-after", type=int, default=2, help="training passes before guided sampling")
     ap.add_argument("--seed-guide", type=int, default=17)
     args = ap.parse_args()

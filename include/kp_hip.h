@@ -407,7 +407,8 @@ int kp_guiding_train_step(kp_ctx* ctx, const float* in6, const float* dirs, cons
  * divided by the path throughput before it) and takes train_steps Adam steps
  * of the loss of kp_guiding_train_step on them, weights normalised to sum 1.
  * Sampling from the network starts once guide_after passes have trained it;
- * after train_passes training passes (if nonzero) the network stays fixed.
+ * after train_passes training passes (if nonzero) each pass takes
+ * late_train_steps steps instead (0: the network stays fixed).
  * Guided renders run at most pass_paths paths per pass. The position is
  * normalised with the box [bmin, bmax] (bmin == bmax: the scene's bounds).
  * kp_integrate_rays with the flag samples from the network without training. */
@@ -430,6 +431,7 @@ typedef struct {
     uint32_t max_guided_depth; /* only vertices at depth <= this (the camera's hit is
                                   depth 1) are guided and recorded; deeper vertices use
                                   the BSDF alone and skip the network query; 0 = all */
+    uint32_t late_train_steps; /* Adam steps per pass after train_passes passes; 0 = frozen */
 } kp_guiding_desc;
 /* Needs kp_guiding_init first; a later kp_guiding_init keeps the settings. */
 int kp_guiding_configure(kp_ctx* ctx, const kp_guiding_desc* desc);

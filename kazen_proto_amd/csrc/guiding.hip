@@ -818,6 +818,7 @@ int kp_guiding_configure(kp_ctx* c, const kp_guiding_desc* d) {
     g->max_records = d->max_records;
     g->pass_paths = d->pass_paths;
     g->max_train_passes = d->train_passes;
+    g->late_train_steps = d->late_train_steps;
     g->max_guided_depth = d->max_guided_depth;
     g->auto_box = d->bmin[0] == d->bmax[0] && d->bmin[1] == d->bmax[1] && d->bmin[2] == d->bmax[2];
     for (int i = 0; i < 3; ++i) {

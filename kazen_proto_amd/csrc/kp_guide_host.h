@@ -37,7 +37,8 @@ struct GuideState {
     float weight_clamp = 0.0f;   // cap on training weights, multiple of the batch mean (0: none)
     bool relative_weights = false;   // weights relative to the pixel estimate
     uint32_t train_steps = 1, guide_after = 1, max_records = 1u << 18, pass_paths = 1u << 18;
-    uint32_t max_train_passes = 0;   // 0 = every pass trains
+    uint32_t max_train_passes = 0;   // passes that train train_steps steps (0 = every pass)
+    uint32_t late_train_steps = 0;   // steps per pass after them (0 = frozen)
     uint32_t max_guided_depth = 0;   // 0 = every depth
     bool auto_box = true;
     float box_min[3] = {0, 0, 0}, box_max[3] = {0, 0, 0};

@@ -1318,7 +1318,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     // records per recorded path: the guided vertices 1 .. min(max_depth, max_guided_depth)
     const uint32_t rec_depth = guided && g->max_guided_depth ? std::min(d->max_depth, g->max_guided_depth)
                                                              : d->max_depth;
-    if (guided && g->train_steps && rec_depth > g->max_records)
+    if (guided && (g->train_steps || g->late_train_steps) && rec_depth > g->max_records)
         return fail(c, KP_ERR_INVALID, "guiding max_records must be >= the recorded depth");
     int rc = prepare_scene(c);
     if (rc) return rc;
@@ -1349,7 +1349,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     // training records: every rec_stride-th path (phase rotating per pass),
     // rec_depth records each, at most max_records per pass
     uint32_t stride = 1, rec_slots_max = 0;
-    const bool train_any = guided && g->train_steps > 0;
+    const bool train_any = guided && (g->train_steps > 0 || g->late_train_steps > 0);
     a.rec_depth = rec_depth;
     if (train_any) {
         const uint64_t D = rec_depth;
@@ -1360,6 +1360,7 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
     const uint32_t gen_blocks = (n_paths + 255) / 256;
     uint32_t steps = 0, guided_passes = 0;
     bool train = false;
+    uint32_t pass_steps = 0;
     auto t0 = std::chrono::steady_clock::now();
     unsigned long long zero2[2] = {0, 0};
     KP_HIP_TRY(c, hipMemcpyAsync(r->ps.totals, zero2, 16, hipMemcpyHostToDevice, c->stream));
@@ -1370,7 +1371,11 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
             a.guide_on = g->train_passes >= g->guide_after ? 1u : 0u;
             guided_passes += a.guide_on;
             a.rec_cap = 0;
-            train = train_any && (g->max_train_passes == 0 || g->train_passes < g->max_train_passes);
+            // train_steps per pass for the first train_passes passes (all
+            // passes if 0), late_train_steps per pass after them
+            const bool early = g->max_train_passes == 0 || g->train_passes < g->max_train_passes;
+            pass_steps = early ? g->train_steps : g->late_train_steps;
+            train = train_any && pass_steps > 0;
             if (train) {
                 a.rec_stride = stride;
                 a.rec_phase = g->train_passes % stride;
@@ -1393,9 +1398,9 @@ static int render_accumulate(kp_ctx* c, const kp_render_desc* d) {
                                g->weights, g->relative_weights ? (const float4*)r->film_sum : nullptr, inv_count);
             KP_HIP_TRY(c, hipGetLastError());
             if ((rc = guide_normalise_records(c, n_rec))) return rc;
-            for (uint32_t sidx = 0; sidx < g->train_steps; ++sidx)
+            for (uint32_t sidx = 0; sidx < pass_steps; ++sidx)
                 if ((rc = guide_train_device(c, g->lr, true))) return rc;
-            steps += g->train_steps;
+            steps += pass_steps;
             ++g->train_passes;
         }
     }

@@ -97,7 +97,7 @@ class GuidingDesc(ctypes.Structure):
                 ("lr", ctypes.c_float), ("train_steps", _u32), ("guide_after", _u32),
                 ("max_records", _u32), ("pass_paths", _u32), ("train_passes", _u32),
                 ("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3), ("weight_clamp", ctypes.c_float),
-                ("relative_weights", _u32), ("max_guided_depth", _u32)]
+                ("relative_weights", _u32), ("max_guided_depth", _u32), ("late_train_steps", _u32)]
 
 
 def make_material(d):
@@ -661,7 +661,7 @@ class Context:
 
     def guiding_configure(self, bsdf_fraction=0.5, min_alpha=0.5, lr=1e-3, train_steps=1, guide_after=1,
                           max_records=1 << 18, pass_paths=1 << 18, box=None, train_passes=0, weight_clamp=0.0,
-                          relative_weights=False, max_guided_depth=0):
+                          relative_weights=False, max_guided_depth=0, late_train_steps=0):
         """Guided-render settings (kp_guiding_configure); box = (lo, hi) or None for the scene's bounds;
         max_guided_depth 0 guides every vertex."""
         d = GuidingDesc()
@@ -676,6 +676,7 @@ class Context:
         d.weight_clamp = weight_clamp
         d.relative_weights = 1 if relative_weights else 0
         d.max_guided_depth = max_guided_depth
+        d.late_train_steps = late_train_steps
         if box is not None:
             d.bmin[:] = [float(x) for x in box[0]]
             d.bmax[:] = [float(x) for x in box[1]]

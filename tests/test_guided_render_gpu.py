@@ -78,16 +78,18 @@ def test_guided_paths_match_oracle(scene, gdepth):
     assert close.mean() >= 0.995
 
 
-@pytest.mark.parametrize("gdepth", [0, 2])
-def test_guided_training_render_keeps_furnace_mean(gdepth):
-    # gdepth 2: two records per recorded path (the guided vertices only)
+@pytest.mark.parametrize("gdepth,late", [(0, 0), (2, 0), (0, 1)])
+def test_guided_training_render_keeps_furnace_mean(gdepth, late):
+    # gdepth 2: two records per recorded path (the guided vertices only);
+    # late 1: two steps on each of the first 16 passes, then one per pass
     s = scenes.furnace_box(albedo=0.6, Le=1.0, max_depth=4, rr_depth=2, spp=64, width=32, height=32)
     c = guided_ctx(s, bsdf_fraction=0.5, lr=5e-3, train_steps=2, guide_after=2, pass_paths=1024,
-                   max_guided_depth=gdepth)
+                   max_guided_depth=gdepth, train_passes=16 if late else 0, late_train_steps=late)
     img = c.render(desc(s, guided=True))
     st = c.render_stats()
     c.close()
-    assert st["passes"] == 64 and st["train_steps"] == 128 and st["guided_passes"] == 62
+    assert st["passes"] == 64 and st["guided_passes"] == 62
+    assert st["train_steps"] == (16 * 2 + 48 * 1 if late else 128)
     assert st["train_records"] > 0
     pix = img[..., 0].astype(np.float64).ravel()
     err = pix.std() / np.sqrt(pix.size)
