@@ -381,6 +381,8 @@ def main():
                          "known traffic for HBM counter calibration)")
     ap.add_argument("--no-pathtrace", action="store_true", help="skip the terrain path-trace leg")
     ap.add_argument("--no-width-tune", action="store_true", help="skip the node-width tuner on the microbench")
+    ap.add_argument("--launch-scaling", action="store_true",
+                    help="also time incoherent launches of 2^16..2^25 rays on the same BVH")
     ap.add_argument("--pt-grid", type=int, default=2048)
     ap.add_argument("--pt-res", type=int, default=1024)
     ap.add_argument("--pt-spp", type=int, default=64)
@@ -509,6 +511,35 @@ def main():
                 "tris_per_ray": round(o_tris, 3), "counts_source": "oracle BVH8 traversal of %d rays" % o_sample,
                 "kernel_nodes_per_ray": round(k_nodes, 3), "kernel_tris_per_ray": round(k_tris, 3)}
 
+    # per-launch time against the launch size, same BVH: the incoherent set
+    # plus 7 more of the same distribution, traced as prefixes of 2^16..2^25
+    launch_scaling = None
+    if args.launch_scaling:
+        rng = np.random.default_rng(77)
+        big = np.concatenate([inc] + [scenes.incoherent_rays(inc.shape[0], rng) for _ in range(7)])
+        d_big = native.DeviceBuffer.from_array(ctx, big)
+        h_big = native.DeviceBuffer(ctx, (big.shape[0], 4), np.float32)
+        del big
+        launch_scaling = []
+        for lg in range(16, 26):
+            k = 1 << lg
+            for _ in range(2):
+                ctx.intersect_closest(d_big, out=h_big, n=k)
+            best = float("inf")
+            for _ in range(3):
+                ctx.synchronize()
+                ctx.timer_begin()
+                for _ in range(5):
+                    ctx.intersect_closest(d_big, out=h_big, n=k)
+                best = min(best, ctx.timer_end() / 5)
+            ctx.synchronize()
+            row = {"rays": k, "ms": round(best, 4), "mrays_per_s": round(k / best / 1e3, 1),
+                   "frac_at_oracle_bytes": round(b_ray * k / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            launch_scaling.append(row)
+            print("[bench] launch scaling %s" % row, file=sys.stderr, flush=True)
+        d_big.free()
+        h_big.free()
+
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -591,6 +622,7 @@ def main():
         "parity": parity,
         "bvh_build_seconds": round(st["build_seconds"], 3),
         "node_width_tuning": width_tuning,
+        "launch_scaling": launch_scaling,
         "path_trace": path_trace,
         "batch": batch,
         "guiding": guiding,

@@ -140,20 +140,36 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, const float4 a, const 
 
 
 // Per-ray node-test constants: the ray direction's octant (bit k set when
-// direction k is negative) and the 16-byte word of each near / far plane in a
-// node (planes lo.x lo.y lo.z hi.x hi.y hi.z are words 2..7), so each lane
-// loads its own near and far planes instead of selecting them per child.
+// direction k is negative) and the byte offset in a node of each near / far
+// plane word (planes lo.x lo.y lo.z hi.x hi.y hi.z are the 16-byte words
+// 2..7), one byte each, so each lane loads its own near and far planes
+// instead of selecting them per child.
 struct NodeTest {
     uint32_t oct;
-    uint32_t words;   // 4 bits each: near x, far x, near y, far y, near z, far z
+    uint32_t offa;   // bytes: near x, far x, near y, far y
+    uint32_t offb;   // bytes: near z, far z
 };
 
 __device__ __forceinline__ NodeTest node_test_setup(const RayPre& r) {
     NodeTest t;
     const bool nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
     t.oct = (nx ? 1u : 0u) | (ny ? 2u : 0u) | (nz ? 4u : 0u);
-    t.words = (nx ? 0x25u : 0x52u) | ((ny ? 0x36u : 0x63u) << 8) | ((nz ? 0x47u : 0x74u) << 16);
+    t.offa = (nx ? 0x2050u : 0x5020u) | ((ny ? 0x3060u : 0x6030u) << 16);
+    t.offb = nz ? 0x4070u : 0x7040u;
     return t;
+}
+
+// The node array as a buffer resource: a node visit addresses its words by
+// 32-bit byte offsets from one scalar base (no 64-bit address arithmetic per
+// load); the range is left open, as every node index is valid.
+typedef __amdgpu_buffer_rsrc_t NodeBuf;
+
+__device__ __forceinline__ NodeBuf node_buffer(const uint4* nodes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)nodes, (short)0, (int)0xFFFFFFFF, 0x00020000);
+}
+
+__device__ __forceinline__ uint4 node_word(NodeBuf nb, uint32_t byte_off) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(nb, byte_off, 0, 0));
 }
 
 // Plane value of a slot: a binary16 integer read straight from its half of
@@ -183,15 +199,14 @@ __device__ __forceinline__ uint32_t tkey(float tn) {
 }
 
 // Tests the 8 children of node `idx` against [tmin, tmax].
-__device__ __forceinline__ void visit_node(const uint4* __restrict__ nodes, uint32_t idx, const RayPre& r,
-                                           const NodeTest& nt, float tmax, NodeHits& h) {
-    const uint4* p = nodes + (size_t)idx * 8;
-    h.w0 = p[0];
-    h.w1 = p[1];
-    const uint32_t w = nt.words;
-    const uint4 nx = p[w & 0xFu], fx = p[(w >> 4) & 0xFu];
-    const uint4 ny = p[(w >> 8) & 0xFu], fy = p[(w >> 12) & 0xFu];
-    const uint4 nz = p[(w >> 16) & 0xFu], fz = p[(w >> 20) & 0xFu];
+__device__ __forceinline__ void visit_node(NodeBuf nb, uint32_t idx, const RayPre& r, const NodeTest& nt,
+                                           float tmax, NodeHits& h) {
+    const uint32_t p = idx << 7;   // 128-byte nodes
+    h.w0 = node_word(nb, p);
+    h.w1 = node_word(nb, p + 16u);
+    const uint4 nx = node_word(nb, p + (nt.offa & 0xFFu)), fx = node_word(nb, p + ((nt.offa >> 8) & 0xFFu));
+    const uint4 ny = node_word(nb, p + ((nt.offa >> 16) & 0xFFu)), fy = node_word(nb, p + (nt.offa >> 24));
+    const uint4 nz = node_word(nb, p + (nt.offb & 0xFFu)), fz = node_word(nb, p + (nt.offb >> 8));
     const float px = __uint_as_float(h.w0.x), py = __uint_as_float(h.w0.y), pz = __uint_as_float(h.w0.z);
     const uint32_t e = h.w0.w;
     const int ex = (int)(int8_t)(e & 0xFF), ey = (int)(int8_t)((e >> 8) & 0xFF), ez = (int)(int8_t)((e >> 16) & 0xFF);
@@ -254,8 +269,7 @@ __device__ __forceinline__ float slot_tnear(const uint4* __restrict__ nodes, uin
                                             const RayPre& r, const NodeTest& nt) {
     const uint4* p = nodes + (size_t)idx * 8;
     const uint4 w0 = p[0];
-    const uint32_t w = nt.words;
-    const uint4 nx = p[w & 0xFu], ny = p[(w >> 8) & 0xFu], nz = p[(w >> 16) & 0xFu];
+    const uint4 nx = p[(nt.offa & 0xFFu) >> 4], ny = p[((nt.offa >> 16) & 0xFFu) >> 4], nz = p[(nt.offb & 0xFFu) >> 4];
     const uint32_t e = w0.w;
     const int ex = (int)(int8_t)(e & 0xFF), ey = (int)(int8_t)((e >> 8) & 0xFF), ez = (int)(int8_t)((e >> 16) & 0xFF);
     const float ax = ldexpf(r.ix, ex), ay = ldexpf(r.iy, ey), az = ldexpf(r.iz, ez);
@@ -475,6 +489,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
     st.sp = 0;
     st.tid = tid;
     const uint4* __restrict__ nodes = args.nodes;
+    const NodeBuf nbuf = node_buffer(nodes);
     const uint32_t n_rays = args.n_dev ? *args.n_dev : args.n;   // device-side count for queues
     const float4* __restrict__ tris = args.tris;
 
@@ -521,7 +536,7 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                 bool fault = false;
                 if (cur != KP_INVALID) {
                     NodeHits h;
-                    visit_node(nodes, cur, r, nt, best_t, h);
+                    visit_node(nbuf, cur, r, nt, best_t, h);
                     if (kCount) cnt_nodes++;
                     const uint32_t vis = cur;
                     cur = KP_INVALID;

@@ -482,12 +482,16 @@ class Context:
         """Frees cached path state and staging buffers (reallocated on demand)."""
         self._check(lib().kp_release_buffers(self._h))
 
-    def intersect_closest(self, rays, out=None, counted=False):
+    def intersect_closest(self, rays, out=None, counted=False, n=None):
         """rays: [n,8] float32 numpy array or DeviceBuffer.  Returns [n,4]
         float32 hits {t, prim bits, u, v} of the same kind (plus the node and
-        triangle counts when counted=True)."""
+        triangle counts when counted=True).  For DeviceBuffers, n traces only
+        the first n rays (all by default)."""
         if isinstance(rays, DeviceBuffer):
-            n = rays.nbytes // 32
+            cap = rays.nbytes // 32
+            n = cap if n is None else int(n)
+            if not 0 <= n <= cap or (out is not None and n * 16 > out.nbytes):
+                raise ValueError("ray count %d exceeds the buffers" % n)
             if out is None:
                 out = DeviceBuffer(self, (n, 4), np.float32)
             rp, hp, kind = rays.ptr, out.ptr, KP_PTR_DEVICE
