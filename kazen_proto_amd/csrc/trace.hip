@@ -342,11 +342,13 @@ constexpr int kStack = KP_TRACE_STACK;
 // Work distribution of the persistent kernels.  A wave takes chunks of
 // consecutive rays: a first chunk by its index (about half the rays split
 // evenly over the waves), then chunks of kChunk from the global counter (one
-// atomic per chunk), and hands them to its lanes from a wave-uniform cursor;
-// every lane keeps its next ray prefetched in registers, so replacing a
-// finished ray costs no memory round trip.  All members except the prefetch
-// are wave-uniform.
+// atomic per chunk; kTailChunk once fewer than one kChunk per wave remained
+// after the wave's last chunk, so the waves run dry at about the same time),
+// and hands them to its lanes from a wave-uniform cursor; every lane keeps
+// its next ray prefetched in registers, so replacing a finished ray costs no
+// memory round trip.  All members except the prefetch are wave-uniform.
 constexpr uint32_t kChunk = 256;
+constexpr uint32_t kTailChunk = 64;
 
 struct RayFeeder {
     uint32_t next, end, n;
@@ -355,8 +357,7 @@ struct RayFeeder {
     uint32_t* counter;
     const float4* rays;
     const uint32_t* roots;
-    bool pending;       // this lane holds a prefetched ray
-    uint32_t pid, proot;
+    uint32_t pid, proot;   // prefetched ray of this lane (pid KP_INVALID: none) and its root
     float4 p0, p1;
 
     __device__ __forceinline__ void init(uint32_t* ctr, const float4* rs, const uint32_t* rts, uint32_t count) {
@@ -365,7 +366,6 @@ struct RayFeeder {
         roots = rts;
         proot = 0u;
         n = count;
-        pending = false;
         pid = KP_INVALID;
         // Wave w's first chunk is rays [w F, (w+1) F) without an atomic
         // (every wave of the grid would otherwise hit the one counter address
@@ -390,32 +390,35 @@ struct RayFeeder {
     }
     // Gives a prefetched ray to every lane without one, as far as rays last.
     __device__ __forceinline__ void assign() {
-        const uint64_t m = __ballot(!pending);
+        const uint64_t m = __ballot(pid == KP_INVALID);
         if (m == 0 || drained) return;
         if (next >= end) {
             if (dyn_base >= n) {   // the first chunks covered every ray
                 drained = true;
                 return;
             }
+            // the last chunk taken ended at `end`: small chunks once fewer
+            // than kChunk rays per wave were left after it
+            const uint64_t late = (uint64_t)(gridDim.x * (blockDim.x >> 6)) * kChunk;
+            const uint32_t size = (uint64_t)end + late >= n ? kTailChunk : kChunk;
             uint32_t base = 0;
-            if (__lane_id() == 0) base = atomicAdd(counter, kChunk);
+            if (__lane_id() == 0) base = atomicAdd(counter, size);
             base = dyn_base + __builtin_amdgcn_readfirstlane(base);
             if (base >= n) {
                 drained = true;
                 return;
             }
             next = base;
-            end = base + kChunk < n ? base + kChunk : n;
+            end = base + size < n ? base + size : n;
         }
         const uint32_t avail = end - next;
         const uint32_t want = (uint32_t)__popcll(m);
         const uint32_t prefix = (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
-        if (!pending && prefix < avail) {
+        if (pid == KP_INVALID && prefix < avail) {
             pid = next + prefix;
             p0 = rays[2 * (size_t)pid];
             p1 = rays[2 * (size_t)pid + 1];
             if (roots) proot = roots[pid];
-            pending = true;
         }
         next += want < avail ? want : avail;
     }
@@ -424,35 +427,162 @@ struct RayFeeder {
 // Lanes holding postponed leaf children before the wave tests them together.
 constexpr int kLeafBatch = 16;
 
-// Tests the leaf children `lm` of a node (prim_base, meta words m0 / m1):
-// closest hit updates best_*, any-hit sets occl and stops.
+// LDS of the wave-wide leaf test (per block): the item map of each wave and,
+// per lane, the merged result of the triangles tested for its ray.
+struct LeafLds {
+    uint32_t map[KP_TRACE_BLOCK];             // wave's 64 entries: owner lane << 8 | item of the owner
+    unsigned long long key[KP_TRACE_BLOCK];   // closest hit: minimum hit key; any hit: nonzero once hit
+    float4 hit[KP_TRACE_BLOCK];               // closest hit: {t, u, v} of the minimum key
+};
+
+__device__ __forceinline__ unsigned long long hit_key(float t, uint32_t prim);
+
+// Tests the postponed leaf sets of all active lanes of the wave together.
+// Each lane's set (leaf slots `pend` of a node with primitive base `pbase`
+// and meta words m0 / m1: per slot first offset and count) is a run of
+// triangle tests; the runs are laid out as one list (exclusive prefix sums
+// over the lanes) and dealt out to the active lanes, as many as there are
+// per round, so the triangle tests run with nearly every lane busy instead of
+// one loop per lane at the pace of its longest set.  A tester reads the
+// owner's ray and closest distance from the owner's registers (lane
+// permutes) and merges a hit into the owner's LDS slot: the 64-bit minimum of
+// the (distance, primitive) key, then the t, u, v of the lane whose key is
+// the minimum.  Every triangle is tested against the owner's closest distance
+// at the start of the batch, so the owner's new closest hit is the minimum
+// key over its hits, the same as testing them one after another.
 template <bool kAny, bool kCount>
-__device__ __forceinline__ void test_leaves(const float4* __restrict__ tris, const RayPre& r, uint32_t lm,
-                                            uint32_t prim_base, uint32_t m0, uint32_t m1, float& best_t,
-                                            uint32_t& best_p, float& bu, float& bv, bool& occl,
-                                            uint64_t& cnt_tris, uint32_t prim_off = 0u) {
-    while (lm) {
-        const uint32_t s = __ffs(lm) - 1u;
-        lm &= lm - 1u;
-        const uint32_t mw = s < 4u ? m0 : m1;
-        const uint32_t meta = (mw >> ((s & 3u) * 8u)) & 0xFFu;
-        const uint32_t first = prim_base + (meta & 31u), count = meta >> 5;
-        if (kCount) cnt_tris += count;
-        for (uint32_t i = 0; i < count; ++i) {
-            const float4* tp = tris + 3 * (size_t)(first + i);
+__device__ __forceinline__ void test_leaves_wave(const float4* __restrict__ tris, const RayPre& r, uint32_t pend,
+                                                 uint32_t pbase, uint32_t m0, uint32_t m1, float& best_t,
+                                                 uint32_t& best_p, float& bu, float& bv, bool& occl,
+                                                 uint64_t& cnt_tris, uint32_t prim_off, LeafLds& L, int tid) {
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long active = __ballot(true);
+    const uint32_t nact = (uint32_t)__popcll(active);
+    const uint32_t rank = (uint32_t)__popcll(active & below);
+    uint32_t T = 0u;   // triangles of this lane's set
+    if (!(kAny && occl)) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t c = ((s < 4 ? m0 : m1) >> ((s & 3) * 8 + 5)) & 7u;
+            T += ((pend >> s) & 1u) ? c : 0u;
+        }
+    }
+    uint32_t E = 0u, P = 0u;   // exclusive prefix and total over the wave (T < 64)
+#pragma unroll
+    for (int bit = 0; bit < 6; ++bit) {
+        const unsigned long long m = __ballot((T >> bit) & 1u);
+        E += (uint32_t)__popcll(m & below) << bit;
+        P += (uint32_t)__popcll(m) << bit;
+    }
+    if (P == 0u) return;
+    const int wbase = tid & ~63;
+    uint32_t* map = &L.map[wbase];
+    if (T != 0u) L.key[tid] = kAny ? 0ull : ~0ull;
+    for (uint32_t base = 0u; base < P; base += nact) {
+        if (T != 0u) {
+            const uint32_t lo = max(E, base), hi = min(E + T, base + nact);
+            for (uint32_t i = lo; i < hi; ++i) map[i - base] = (lane << 8) | (i - E);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const bool has = base + rank < P;
+        const uint32_t me = has ? map[rank] : (lane << 8);
+        const int ow = (int)(me >> 8);
+        const uint32_t o_pend = __shfl(pend, ow), o_base = __shfl(pbase, ow);
+        const uint32_t o_m0 = __shfl(m0, ow), o_m1 = __shfl(m1, ow);
+        RayPre o;
+        o.ox = __shfl(r.ox, ow);
+        o.oy = __shfl(r.oy, ow);
+        o.oz = __shfl(r.oz, ow);
+        o.tmin = __shfl(r.tmin, ow);
+        o.Sx = __shfl(r.Sx, ow);
+        o.Sy = __shfl(r.Sy, ow);
+        o.Sz = __shfl(r.Sz, ow);
+        o.kx = __shfl(r.kx, ow);
+        o.ky = __shfl(r.ky, ow);
+        o.kz = __shfl(r.kz, ow);
+        const float o_best = __shfl(best_t, ow);
+        const uint32_t o_off = __shfl(prim_off, ow);
+        if (has) {
+            // the item's triangle: slots in increasing order, triangles in order
+            uint32_t lm = o_pend, q = me & 0xFFu, tri = 0u;
+            while (lm) {
+                const uint32_t s = __ffs(lm) - 1u;
+                lm &= lm - 1u;
+                const uint32_t meta = ((s < 4u ? o_m0 : o_m1) >> ((s & 3u) * 8u)) & 0xFFu;
+                const uint32_t c = meta >> 5;
+                if (q < c) {
+                    tri = o_base + (meta & 31u) + q;
+                    break;
+                }
+                q -= c;
+            }
+            if (kCount) cnt_tris += 1u;
+            const float4* tp = tris + 3 * (size_t)tri;
             const float4 ta = tp[0], tb = tp[1], tc = tp[2];
             float t, u, v;
-            if (tri_test(r, ta, tb, tc, best_t, t, u, v)) {
+            if (tri_test(o, ta, tb, tc, o_best, t, u, v)) {
+                const int ot = wbase + ow;
                 if (kAny) {
-                    occl = true;
-                    return;
+                    L.key[ot] = 1ull;
+                } else {
+                    const unsigned long long hk = hit_key(t, __float_as_uint(ta.w) + o_off);
+                    __hip_atomic_fetch_min(&L.key[ot], hk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    __builtin_amdgcn_wave_barrier();
+                    if (__hip_atomic_load(&L.key[ot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) == hk)
+                        L.hit[ot] = make_float4(t, u, v, 0.0f);
                 }
-                const uint32_t pr = __float_as_uint(ta.w) + prim_off;
-                if (t < best_t || pr < best_p) { best_t = t; best_p = pr; bu = u; bv = v; }
             }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (T != 0u) {
+        const unsigned long long k = __hip_atomic_load(&L.key[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        if (kAny) {
+            if (k != 0ull) occl = true;
+        } else if (k != ~0ull && (best_p == KP_INVALID || k < hit_key(best_t, best_p))) {
+            const float4 h = L.hit[tid];
+            best_t = h.x;
+            best_p = (uint32_t)k;
+            bu = h.y;
+            bv = h.z;
         }
     }
 }
+
+// Order key of a closest-hit candidate: distance first (float bits mapped to
+// unsigned order, -0 taken as +0 as the comparisons do), then primitive, so
+// the minimum key is the closest hit with the lowest primitive among equal
+// distances (the tie rule).
+__device__ __forceinline__ unsigned long long hit_key(float t, uint32_t prim) {
+    uint32_t b = __float_as_uint(t);
+    if (b == 0x80000000u) b = 0u;
+    b ^= (b & 0x80000000u) ? 0xFFFFFFFFu : 0x80000000u;
+    return ((unsigned long long)b << 32) | prim;
+}
+
+// Position of the set bit of rank r (0-based) in m; r < popcount(m).
+__device__ __forceinline__ uint32_t nth_set_bit(unsigned long long m, uint32_t r) {
+    uint32_t pos = 0u;
+    uint32_t c = __popc((uint32_t)m);
+    if (r >= c) { r -= c; m >>= 32; pos += 32u; }
+    c = __popc((uint32_t)m & 0xFFFFu);
+    if (r >= c) { r -= c; m >>= 16; pos += 16u; }
+    c = __popc((uint32_t)m & 0xFFu);
+    if (r >= c) { r -= c; m >>= 8; pos += 8u; }
+    c = __popc((uint32_t)m & 0xFu);
+    if (r >= c) { r -= c; m >>= 4; pos += 4u; }
+    c = __popc((uint32_t)m & 0x3u);
+    if (r >= c) { r -= c; m >>= 2; pos += 2u; }
+    return pos + (r >= ((uint32_t)m & 1u) ? 1u : 0u);
+}
+
+// Lane roles of a split closest-hit ray (low 8 bits: block thread of the
+// lane that owns the ray).
+constexpr uint32_t kHelper = 1u << 8;    // traverses a group of another lane's ray
+constexpr uint32_t kDonated = 1u << 9;   // owner whose ray has had groups handed out
+constexpr uint32_t kWaiting = 1u << 10;  // owner done with its part, helpers still at work
+constexpr uint32_t kStepsWaiting = 0xFFFFFFFFu;   // step count of a waiting owner (skips the traversal)
 
 // Persistent traversal shared by closest hit (kAny = false) and occlusion
 // (kAny = true).  A node visit tests the 8 children; the nearest hit inner
@@ -476,10 +606,28 @@ __device__ __forceinline__ void test_leaves(const float4* __restrict__ tris, con
 // only switches the primitive base.  Postponed leaf children are tested
 // before any of these switches, so a set is always tested in the space and
 // with the primitive base it was found in.
+//
+// Split rays (closest hit, one level): once the rays are handed out, lanes
+// whose rays are done would idle while the wave finishes its longest rays.
+// Such a lane takes the top group entry of a lane still at work and
+// traverses that group for the same ray (reloaded from the ray buffer) with
+// the donor's closest distance so far; the groups of a ray are disjoint
+// subtrees, so every triangle is still tested by exactly one lane.  A helper
+// merges its closest hit into its owner's slot in LDS (64-bit minimum of the
+// (distance, primitive) key, then the t, u, v of the winning key); the owner
+// writes the ray's hit once no helper of its ray is left.  The result is the
+// minimum key over all triangles the ray hits in [tmin, tmax], the same as
+// without splitting.
 template <bool kAny, bool kCount, bool kInst>
 __device__ __forceinline__ void traverse(const TraceArgs& args) {
     __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
+    __shared__ LeafLds leafl;
     __shared__ float4 wray[kInst ? 2 : 1][kInst ? KP_TRACE_BLOCK : 1];   // world ray per lane (two-level)
+    constexpr bool kSplit = !kAny && !kInst;
+    __shared__ unsigned long long skey[kSplit ? KP_TRACE_BLOCK : 1];      // owner: minimum helper hit key
+    __shared__ float4 shit[kSplit ? KP_TRACE_BLOCK : 1];                  // owner: {t, u, v} of that key
+    __shared__ uint32_t srole[kSplit ? KP_TRACE_BLOCK : 1];               // owner thread | kHelper | kDonated | kWaiting
+    __shared__ unsigned long long sbusy[kSplit ? KP_TRACE_BLOCK / 64 : 1];  // wave: owners with helpers at work
     const int tid = threadIdx.x;
     const uint32_t gl = blockIdx.x * KP_TRACE_BLOCK + tid;
     Stack<kStack> st;
@@ -501,12 +649,13 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
     uint32_t pend = 0u, pbase = 0u, pm0 = 0u, pm1 = 0u;   // postponed leaf children
     uint32_t inst_pbase = 0u;                              // primitive base of the instance entered
     bool occl = false;
+    bool split_on = false;   // rays are being split (wave-uniform; roles live in srole)
     uint64_t cnt_nodes = 0, cnt_tris = 0;
     RayFeeder feed;
     feed.init(args.counter, args.rays, args.roots, n_rays);
 
     for (;;) {
-        if (ray_id == KP_INVALID && feed.pending) {
+        if (ray_id == KP_INVALID && feed.pid != KP_INVALID) {
             ray_id = feed.pid;
             setup_ray(feed.p0, feed.p1, r);
             nt = node_test_setup(r);
@@ -519,7 +668,11 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
             cur = ray_valid(feed.p0, feed.p1) ? feed.proot : KP_INVALID;   // root
             st.sp = 0;
             steps = 0;
-            feed.pending = false;
+            if (kSplit && split_on) {
+                srole[tid] = (uint32_t)tid;
+                skey[tid] = ~0ull;
+            }
+            feed.pid = KP_INVALID;
             if (kInst) {
                 wray[0][tid] = feed.p0;
                 wray[kInst ? 1 : 0][tid] = feed.p1;
@@ -530,10 +683,86 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
             }
         }
         feed.assign();
-        if (__ballot(ray_id != KP_INVALID || feed.pending) == 0) break;
-        if (ray_id != KP_INVALID) {
+        if (kSplit && feed.drained) {
+            const uint32_t lane = __lane_id();
+            if (!split_on) {   // every ray in flight is its own owner
+                split_on = true;
+                srole[tid] = (uint32_t)tid;
+                skey[tid] = ~0ull;
+            }
+            uint32_t role = srole[tid];
+            unsigned long long* busy_w = &sbusy[tid >> 6];
+            if (__ballot(ray_id != KP_INVALID && (role & kWaiting)) != 0ull) {
+                // owners whose helpers are all done write their ray's hit (the
+                // lanes of a wave reach LDS in instruction order)
+                if (lane == 0u) __hip_atomic_store(busy_w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                __builtin_amdgcn_wave_barrier();
+                if (ray_id != KP_INVALID && (role & kHelper))
+                    __hip_atomic_fetch_or(busy_w, 1ull << (role & 63u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                __builtin_amdgcn_wave_barrier();
+                const unsigned long long busy = __hip_atomic_load(busy_w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                if (ray_id != KP_INVALID && (role & kWaiting) && !((busy >> lane) & 1ull)) {
+                    const unsigned long long k = skey[tid];
+                    if (k != ~0ull && (best_p == KP_INVALID || k < hit_key(best_t, best_p))) {
+                        const float4 h = shit[tid];
+                        best_t = h.x;
+                        best_p = (uint32_t)k;
+                        bu = h.y;
+                        bv = h.z;
+                    }
+                    float4 hh;
+                    if (best_p == KP_INVALID) hh = make_float4(__int_as_float(0x7f800000), __uint_as_float(KP_INVALID), 0.0f, 0.0f);
+                    else hh = make_float4(best_t, __uint_as_float(best_p), bu, bv);
+                    args.hits[ray_id] = hh;
+                    ray_id = KP_INVALID;
+                }
+            }
+            // idle lanes take the top group entry of lanes still at work
+            const unsigned long long idle = __ballot(ray_id == KP_INVALID && feed.pid == KP_INVALID);
+            if (idle != 0ull) {
+                const bool can = ray_id != KP_INVALID && !(role & kWaiting) && st.sp > 0;
+                uint2 e = make_uint2(0u, 0u);
+                if (can) e = st.top();
+                const bool donor = can && !(__uint_as_float(e.y & 0xFFFFFF00u) > best_t);
+                const unsigned long long donors = __ballot(donor);
+                if (donors != 0ull) {
+                    const uint32_t pairs = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
+                    const unsigned long long below = (1ull << lane) - 1ull;
+                    const bool give = donor && (uint32_t)__popcll(donors & below) < pairs;
+                    const uint32_t rank = (uint32_t)__popcll(idle & below);
+                    const bool take = ((idle >> lane) & 1ull) && rank < pairs;
+                    const int src = take ? (int)nth_set_bit(donors, rank) : (int)lane;
+                    const uint32_t gx = __shfl(e.x, src), gy = __shfl(e.y, src);
+                    const uint32_t rid = __shfl(ray_id, src);
+                    const float bt = __shfl(best_t, src);
+                    const uint32_t own = __shfl(role & 0xFFu, src);
+                    if (give) {
+                        st.drop();
+                        srole[tid] = role | kDonated;
+                    }
+                    if (take) {
+                        ray_id = rid;
+                        srole[tid] = own | kHelper;
+                        setup_ray(args.rays[2 * (size_t)rid], args.rays[2 * (size_t)rid + 1], r);
+                        nt = node_test_setup(r);
+                        best_t = bt;
+                        best_p = KP_INVALID;
+                        bu = bv = 0.0f;
+                        pend = 0u;
+                        cur = KP_INVALID;
+                        steps = 0;
+                        st.sp = 0;
+                        st.push(gx, gy);   // the stack is empty: cannot overflow
+                    }
+                }
+            }
+        }
+        if (__ballot(ray_id != KP_INVALID || feed.pid != KP_INVALID) == 0) break;
+        // an owner waiting for its helpers has steps == kStepsWaiting
+        if (ray_id != KP_INVALID && steps != kStepsWaiting) {
             for (;;) {
                 bool fault = false;
+                uint32_t pend2 = 0u, pbase2 = 0u, pm02 = 0u, pm12 = 0u;   // a second leaf set found this iteration
                 if (cur != KP_INVALID) {
                     NodeHits h;
                     visit_node(nbuf, cur, r, nt, best_t, h);
@@ -564,9 +793,11 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                             pbase = h.w1.y;
                             pm0 = h.w1.z;
                             pm1 = h.w1.w;
-                        } else {   // this lane already holds a set: test these now
-                            test_leaves<kAny, kCount>(tris, r, h.leaves, h.w1.y, h.w1.z, h.w1.w, best_t, best_p, bu,
-                                                      bv, occl, cnt_tris, inst_pbase);
+                        } else {   // this lane already holds a set: the wave tests it below
+                            pend2 = h.leaves;
+                            pbase2 = h.w1.y;
+                            pm02 = h.w1.z;
+                            pm12 = h.w1.w;
                         }
                     }
                     if (h.inner && !fault && !(kAny && occl)) {
@@ -648,14 +879,16 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                     }
                 }
                 // postponed leaves: tested together once enough lanes hold some,
-                // or when a lane has no node left to visit
+                // when a lane has no node left to visit, or when a lane found a
+                // second set (held for this iteration only)
                 const bool stuck = pend != 0u && (cur == KP_INVALID || fault);
-                if (__ballot(stuck) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch) {
-                    if (pend != 0u && !(kAny && occl)) {
-                        test_leaves<kAny, kCount>(tris, r, pend, pbase, pm0, pm1, best_t, best_p, bu, bv, occl,
-                                                  cnt_tris, inst_pbase);
-                    }
-                    pend = 0u;
+                if (__ballot(stuck || pend2 != 0u) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch) {
+                    test_leaves_wave<kAny, kCount>(tris, r, pend, pbase, pm0, pm1, best_t, best_p, bu, bv, occl,
+                                                   cnt_tris, inst_pbase, leafl, tid);
+                    pend = pend2;
+                    pbase = pbase2;
+                    pm0 = pm02;
+                    pm1 = pm12;
                 }
                 bool done = fault || (kAny && occl) || (cur == KP_INVALID && pend == 0u && st.sp == 0);
                 if (!done && ++steps > args.max_steps) {
@@ -663,8 +896,25 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                     done = true;
                 }
                 if (done) {
+                    const uint32_t role = kSplit && split_on ? srole[tid] : 0u;
                     if (kAny) {
                         args.occl[ray_id] = occl ? 1u : 0u;
+                    } else if (kSplit && (role & kHelper)) {
+                        // merge into the owner's slot: minimum key, then the
+                        // t, u, v of the lane whose key is the minimum
+                        const uint32_t o = role & 0xFFu;
+                        if (best_p != KP_INVALID) {
+                            const unsigned long long k = hit_key(best_t, best_p);
+                            __hip_atomic_fetch_min(&skey[o], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            __builtin_amdgcn_wave_barrier();
+                            if (__hip_atomic_load(&skey[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) == k)
+                                shit[o] = make_float4(best_t, bu, bv, 0.0f);
+                        }
+                    } else if (kSplit && (role & kDonated)) {
+                        srole[tid] = role | kWaiting;   // the hit is written once the helpers are done
+                        steps = kStepsWaiting;
+                        pend = 0u;
+                        break;
                     } else {
                         float4 hh;
                         if (best_p == KP_INVALID) hh = make_float4(__int_as_float(0x7f800000), __uint_as_float(KP_INVALID), 0.0f, 0.0f);
