@@ -460,19 +460,19 @@ __device__ __forceinline__ void test_leaves_wave(const float4* __restrict__ tris
     const unsigned long long active = __ballot(true);
     const uint32_t nact = (uint32_t)__popcll(active);
     const uint32_t rank = (uint32_t)__popcll(active & below);
-    uint32_t T = 0u;   // triangles of this lane's set
+    uint32_t T = 0u;   // triangles of this lane's set: the counts (bits 5..7 of each meta byte) of its slots
     if (!(kAny && occl)) {
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const uint32_t c = ((s < 4 ? m0 : m1) >> ((s & 3) * 8 + 5)) & 7u;
-            T += ((pend >> s) & 1u) ? c : 0u;
-        }
+        // slot bits of pend spread to byte masks (bit k -> byte k), then a byte sum
+        const uint32_t b0 = (((pend & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
+        const uint32_t b1 = ((((pend >> 4) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
+        T = __builtin_amdgcn_sad_u8((m0 >> 5) & 0x07070707u & b0, 0u,
+                                    __builtin_amdgcn_sad_u8((m1 >> 5) & 0x07070707u & b1, 0u, 0u));
     }
     uint32_t E = 0u, P = 0u;   // exclusive prefix and total over the wave (T < 64)
 #pragma unroll
     for (int bit = 0; bit < 6; ++bit) {
         const unsigned long long m = __ballot((T >> bit) & 1u);
-        E += (uint32_t)__popcll(m & below) << bit;
+        E += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
         P += (uint32_t)__popcll(m) << bit;
     }
     if (P == 0u) return;
