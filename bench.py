@@ -116,7 +116,8 @@ def guiding_equal_error(args):
     def converge(guided):
         if guided:
             c.guiding_init(args.seed_guide)
-            c.guiding_configure(bsdf_fraction=0.5, min_alpha=0.5, lr=args.guiding_lr,
+            c.guiding_configure(bsdf_fraction=args.guiding_bsdf_fraction, min_alpha=args.guiding_min_alpha,
+                                lr=args.guiding_lr,
                                 train_steps=args.guiding_steps, guide_after=args.guiding_after,
                                 max_records=1 << 18, pass_paths=args.guiding_pass_paths,
                                 train_passes=args.guiding_train_passes,
@@ -401,6 +402,10 @@ def main():
     ap.add_argument("--guiding-target", type=float, default=0.01)
     ap.add_argument("--guiding-max-seconds", type=float, default=30.0)
     ap.add_argument("--guiding-lr", type=float, default=1e-3)
+    ap.add_argument("--guiding-bsdf-fraction", type=float, default=0.5,
+                    help="one-sample MIS probability of sampling the BSDF at a guided vertex")
+    ap.add_argument("--guiding-min-alpha", type=float, default=0.5,
+                    help="conductors guided from this roughness up (Lambert always)")
     ap.add_argument("--guiding-depth", type=int, default=0,
                     help="guide vertices up to this depth only (0: every vertex)")
     ap.add_argument("--guiding-steps", type=int, default=8, help="Adam steps per pass")
