@@ -373,10 +373,18 @@ struct RayFeeder {
         // and a wave whose first chunk lies beyond the rays has none.
         const uint32_t waves = gridDim.x * (blockDim.x >> 6);
         const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        // first chunks hold about half the rays (256 .. 4096 per wave); the
-        // counter's chunks of kChunk balance the rest
-        uint32_t first = (n / (2u * waves)) & ~63u;
-        first = first < kChunk ? kChunk : (first > 4096u ? 4096u : first);
+        // up to kChunk rays per wave: every ray in a first chunk (multiples of
+        // 64, as many waves as that takes); more: the first chunks hold
+        // about half the rays (256 .. 4096 per wave) and the counter's chunks
+        // balance the rest
+        uint32_t first;
+        if (n <= waves * kChunk) {
+            first = ((n + waves - 1u) / waves + 63u) & ~63u;
+            if (first == 0u) first = 64u;
+        } else {
+            first = (n / (2u * waves)) & ~63u;
+            first = first < kChunk ? kChunk : (first > 4096u ? 4096u : first);
+        }
         dyn_base = waves * first;
         next = wave * first;
         if (next < n) {
