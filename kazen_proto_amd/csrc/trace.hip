@@ -625,7 +625,9 @@ constexpr uint32_t kStepsWaiting = 0xFFFFFFFFu;   // step count of a waiting own
 // (distance, primitive) key, then the t, u, v of the winning key); the owner
 // writes the ray's hit once no helper of its ray is left.  The result is the
 // minimum key over all triangles the ray hits in [tmin, tmax], the same as
-// without splitting.
+// without splitting.  A helper also culls with its owner's closest distance
+// once that is nearer (read at every pass of the drain phase).  Occlusion
+// does not split: split occlusion rays made the integrator 2 % slower.
 template <bool kAny, bool kCount, bool kInst>
 __device__ __forceinline__ void traverse(const TraceArgs& args) {
     __shared__ uint2 lds[kStack][KP_TRACE_BLOCK];
@@ -723,6 +725,15 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                     else hh = make_float4(best_t, __uint_as_float(best_p), bu, bv);
                     args.hits[ray_id] = hh;
                     ray_id = KP_INVALID;
+                }
+            }
+            {
+                // a helper culls with its owner's closest distance once that
+                // is nearer; its own hit, farther than the owner's, cannot win
+                const float ob = __shfl(best_t, (int)(role & 63u));
+                if (ray_id != KP_INVALID && (role & kHelper) && ob < best_t) {
+                    best_t = ob;
+                    best_p = KP_INVALID;
                 }
             }
             // idle lanes take the top group entry of lanes still at work
