@@ -101,6 +101,54 @@ def test_occlusion_matches_oracle():
     c.close()
 
 
+def _tie_scene(rng, n_rays):
+    """A 16x16 grid of triangle pairs in the plane z = 0 (shared edges), exact
+    copies of some triangles, copies with reversed winding and copies raised
+    by 1e-7; rays through edges and vertices, a third of them starting on the
+    plane itself (hits at t = +0 and -0 there)."""
+    g = np.linspace(-1, 1, 17)
+    tris = []
+    for i in range(16):
+        for j in range(16):
+            a, b, c, d = (g[i], g[j], 0), (g[i + 1], g[j], 0), (g[i + 1], g[j + 1], 0), (g[i], g[j + 1], 0)
+            tris += [a + b + c, a + c + d]
+    tris = np.asarray(tris, np.float32)
+    rev = tris[200:300][:, [0, 1, 2, 6, 7, 8, 3, 4, 5]]
+    tris = np.concatenate([tris, tris[40:120], rev, tris[300:340] + np.float32([0, 0, 1e-7] * 3)])
+    o = np.zeros((n_rays, 3), np.float32)
+    o[:, :2] = rng.uniform(-1, 1, (n_rays, 2))
+    o[:n_rays // 2, :2] = np.round(o[:n_rays // 2, :2] * 8) / 8
+    idx = np.arange(n_rays)
+    o[:, 2] = np.where(idx % 3 == 0, 0.0, 2.0)
+    d = np.zeros((n_rays, 3), np.float32)
+    d[:, 2] = np.where(idx % 6 == 0, 1.0, -1.0)
+    d[:, :2] = rng.normal(size=(n_rays, 2)) * 0.05
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n_rays, 8), np.float32)
+    rays[:, :3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    return tris, rays
+
+
+def test_exact_ties_and_signed_zero_distances():
+    """Ties (same t, several primitives) resolve to the lowest primitive and
+    t = -0 ties with +0, bit for bit as the oracle's one-by-one rule, through
+    the wave-wide leaf tests (hits merged by key) and split rays (a small
+    launch drains from its start; the repeated set runs in steady state)."""
+    tris, rays = _tie_scene(np.random.default_rng(31), 20000)
+    exp = oracle.closest_bruteforce(tris, rays, threads=CPU_THREADS)
+    tb = exp[:, 0].view(np.uint32)
+    assert np.count_nonzero(tb == 0x80000000) > 1000 and np.count_nonzero(tb == 0) > 1000
+    assert oracle.count_near_ties(tris, rays, threads=CPU_THREADS) > 5000
+    c = make_ctx(tris)
+    assert mismatches(c.intersect_closest(rays), exp) == 0
+    big = np.tile(rays, (16, 1))
+    assert mismatches(c.intersect_closest(big), np.tile(exp, (16, 1))) == 0
+    assert np.array_equal(c.occluded(rays).astype(bool), exp[:, 1].view(np.uint32) != 0xFFFFFFFF)
+    c.close()
+
+
 def test_degenerate_rays_and_empty_batch():
     g = golden()
     c = make_ctx(g["tris"])
