@@ -434,6 +434,10 @@ struct RayFeeder {
 
 // Lanes holding postponed leaf children before the wave tests them together.
 constexpr int kLeafBatch = 16;
+// Lanes with nothing but postponed leaves left (they wait) before the wave
+// tests them; a wave test has a fixed cost of about 150 instructions and an
+// idle lane costs a 64th of an iteration, so waiting for a fuller batch pays.
+constexpr uint32_t kStuckBatch = 8u;
 
 // LDS of the wave-wide leaf test (per block): the item map of each wave and,
 // per lane, the merged result of the triangles tested for its ray.
@@ -446,11 +450,12 @@ struct LeafLds {
 __device__ __forceinline__ unsigned long long hit_key(float t, uint32_t prim);
 
 // Tests the postponed leaf sets of all active lanes of the wave together.
-// Each lane's set (leaf slots `pend` of a node with primitive base `pbase`
-// and meta words m0 / m1: per slot first offset and count) is a run of
-// triangle tests; the runs are laid out as one list (exclusive prefix sums
-// over the lanes) and dealt out to the active lanes, as many as there are
-// per round, so the triangle tests run with nearly every lane busy instead of
+// Each lane holds up to two sets (leaf slots `pend` of a node with primitive
+// base `pbase` and meta words m0 / m1: per slot first offset and count; the
+// second set is the one found in the current iteration); a lane's sets are
+// one run of triangle tests (at most 2 x 8 slots x 4 = 64); the runs are laid
+// out as one list (exclusive prefix sums over the lanes) and dealt out to the
+// active lanes, as many as there are per round, so the triangle tests run with nearly every lane busy instead of
 // one loop per lane at the pace of its longest set.  A tester reads the
 // owner's ray and closest distance from the owner's registers (lane
 // permutes) and merges a hit into the owner's LDS slot: the 64-bit minimum of
@@ -458,9 +463,18 @@ __device__ __forceinline__ unsigned long long hit_key(float t, uint32_t prim);
 // the minimum.  Every triangle is tested against the owner's closest distance
 // at the start of the batch, so the owner's new closest hit is the minimum
 // key over its hits, the same as testing them one after another.
+__device__ __forceinline__ uint32_t leaf_set_tris(uint32_t pend, uint32_t m0, uint32_t m1) {
+    // slot bits of pend spread to byte masks (bit k -> byte k), then a byte sum
+    const uint32_t b0 = (((pend & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
+    const uint32_t b1 = ((((pend >> 4) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
+    return __builtin_amdgcn_sad_u8((m0 >> 5) & 0x07070707u & b0, 0u,
+                                   __builtin_amdgcn_sad_u8((m1 >> 5) & 0x07070707u & b1, 0u, 0u));
+}
+
 template <bool kAny, bool kCount>
 __device__ __forceinline__ void test_leaves_wave(const float4* __restrict__ tris, const RayPre& r, uint32_t pend,
-                                                 uint32_t pbase, uint32_t m0, uint32_t m1, float& best_t,
+                                                 uint32_t pbase, uint32_t m0, uint32_t m1, uint32_t pend2,
+                                                 uint32_t pbase2, uint32_t m02, uint32_t m12, float& best_t,
                                                  uint32_t& best_p, float& bu, float& bv, bool& occl,
                                                  uint64_t& cnt_tris, uint32_t prim_off, LeafLds& L, int tid) {
     const uint32_t lane = __lane_id();
@@ -468,17 +482,15 @@ __device__ __forceinline__ void test_leaves_wave(const float4* __restrict__ tris
     const unsigned long long active = __ballot(true);
     const uint32_t nact = (uint32_t)__popcll(active);
     const uint32_t rank = (uint32_t)__popcll(active & below);
-    uint32_t T = 0u;   // triangles of this lane's set: the counts (bits 5..7 of each meta byte) of its slots
+    // triangles of this lane's sets: the counts (bits 5..7 of each meta byte) of their slots
+    uint32_t T1 = 0u, T = 0u;
     if (!(kAny && occl)) {
-        // slot bits of pend spread to byte masks (bit k -> byte k), then a byte sum
-        const uint32_t b0 = (((pend & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
-        const uint32_t b1 = ((((pend >> 4) & 0xFu) * 0x00204081u) & 0x01010101u) * 0xFFu;
-        T = __builtin_amdgcn_sad_u8((m0 >> 5) & 0x07070707u & b0, 0u,
-                                    __builtin_amdgcn_sad_u8((m1 >> 5) & 0x07070707u & b1, 0u, 0u));
+        T1 = leaf_set_tris(pend, m0, m1);
+        T = T1 + leaf_set_tris(pend2, m02, m12);
     }
-    uint32_t E = 0u, P = 0u;   // exclusive prefix and total over the wave (T < 64)
+    uint32_t E = 0u, P = 0u;   // exclusive prefix and total over the wave (T <= 64)
 #pragma unroll
-    for (int bit = 0; bit < 6; ++bit) {
+    for (int bit = 0; bit < 7; ++bit) {
         const unsigned long long m = __ballot((T >> bit) & 1u);
         E += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
         P += (uint32_t)__popcll(m) << bit;
@@ -496,8 +508,11 @@ __device__ __forceinline__ void test_leaves_wave(const float4* __restrict__ tris
         const bool has = base + rank < P;
         const uint32_t me = has ? map[rank] : (lane << 8);
         const int ow = (int)(me >> 8);
+        const uint32_t o_t1 = __shfl(T1, ow);
         const uint32_t o_pend = __shfl(pend, ow), o_base = __shfl(pbase, ow);
         const uint32_t o_m0 = __shfl(m0, ow), o_m1 = __shfl(m1, ow);
+        const uint32_t o_pend2 = __shfl(pend2, ow), o_base2 = __shfl(pbase2, ow);
+        const uint32_t o_m02 = __shfl(m02, ow), o_m12 = __shfl(m12, ow);
         RayPre o;
         o.ox = __shfl(r.ox, ow);
         o.oy = __shfl(r.oy, ow);
@@ -513,14 +528,18 @@ __device__ __forceinline__ void test_leaves_wave(const float4* __restrict__ tris
         const uint32_t o_off = __shfl(prim_off, ow);
         if (has) {
             // the item's triangle: slots in increasing order, triangles in order
-            uint32_t lm = o_pend, q = me & 0xFFu, tri = 0u;
+            uint32_t q = me & 0xFFu, tri = 0u;
+            const bool second = q >= o_t1;
+            uint32_t lm = second ? o_pend2 : o_pend;
+            const uint32_t sb = second ? o_base2 : o_base, sm0 = second ? o_m02 : o_m0, sm1 = second ? o_m12 : o_m1;
+            if (second) q -= o_t1;
             while (lm) {
                 const uint32_t s = __ffs(lm) - 1u;
                 lm &= lm - 1u;
-                const uint32_t meta = ((s < 4u ? o_m0 : o_m1) >> ((s & 3u) * 8u)) & 0xFFu;
+                const uint32_t meta = ((s < 4u ? sm0 : sm1) >> ((s & 3u) * 8u)) & 0xFFu;
                 const uint32_t c = meta >> 5;
                 if (q < c) {
-                    tri = o_base + (meta & 31u) + q;
+                    tri = sb + (meta & 31u) + q;
                     break;
                 }
                 q -= c;
@@ -898,16 +917,17 @@ __device__ __forceinline__ void traverse(const TraceArgs& args) {
                     }
                 }
                 // postponed leaves: tested together once enough lanes hold some,
-                // when a lane has no node left to visit, or when a lane found a
-                // second set (held for this iteration only)
+                // once kStuckBatch lanes (or every lane here) have nothing but
+                // leaves left, or when a lane found a second set (held for
+                // this iteration only; the wave tests both sets)
                 const bool stuck = pend != 0u && (cur == KP_INVALID || fault);
-                if (__ballot(stuck || pend2 != 0u) != 0ull || __popcll(__ballot(pend != 0u)) >= kLeafBatch) {
-                    test_leaves_wave<kAny, kCount>(tris, r, pend, pbase, pm0, pm1, best_t, best_p, bu, bv, occl,
-                                                   cnt_tris, inst_pbase, leafl, tid);
-                    pend = pend2;
-                    pbase = pbase2;
-                    pm0 = pm02;
-                    pm1 = pm12;
+                const uint32_t n_stuck = (uint32_t)__popcll(__ballot(stuck));
+                const uint32_t n_here = (uint32_t)__popcll(__ballot(true));
+                if (n_stuck >= min(kStuckBatch, n_here) || __ballot(pend2 != 0u) != 0ull ||
+                    __popcll(__ballot(pend != 0u)) >= kLeafBatch) {
+                    test_leaves_wave<kAny, kCount>(tris, r, pend, pbase, pm0, pm1, pend2, pbase2, pm02, pm12, best_t,
+                                                   best_p, bu, bv, occl, cnt_tris, inst_pbase, leafl, tid);
+                    pend = 0u;
                 }
                 bool done = fault || (kAny && occl) || (cur == KP_INVALID && pend == 0u && st.sp == 0);
                 if (!done && ++steps > args.max_steps) {
