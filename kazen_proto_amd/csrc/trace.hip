@@ -617,9 +617,10 @@ constexpr uint32_t kStepsWaiting = 0xFFFFFFFFu;   // step count of a waiting own
 // group entry, taken later in octant order (the builder puts children in
 // octant slots) and skipped as a whole when their nearest entry distance lies
 // beyond the closest hit so far.  Hit leaf children are postponed (one set
-// per lane) and tested when at least kLeafBatch lanes of the wave hold some,
-// or a lane has nothing else to do, so triangle tests run with many lanes
-// active instead of one or two per iteration.
+// per lane) and tested wave-wide (test_leaves_wave) when at least kLeafBatch
+// lanes of the wave hold some, when kStuckBatch lanes (or every lane in the
+// loop) have nothing else left, or when a lane finds a second set, so the
+// triangle tests run with many lanes active instead of one or two.
 //
 // Two-level variant (kInst): a ray first traverses the BVH of the meshes
 // (world_root, one-level, in world space) with a top-level marker below it on
