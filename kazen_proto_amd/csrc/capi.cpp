@@ -73,10 +73,9 @@ int trace_closest_dev(kp_ctx* c, const float4* rays, uint32_t n, float4* hits, b
     a.rays = rays;
     a.hits = hits;
     a.n = n;
-    // one-level: the 5-wave build (96 registers) is 1.8 % faster on
-    // incoherent rays than the 4-wave one; the counted and the two-level
-    // variants have kernels of their own
-    const bool five = !count && !c->two_level;
+    // the 5-wave builds (96 registers, one-level or two-level); the counted
+    // variant has a 4-wave kernel of its own
+    const bool five = !count;
     a.five_waves = five ? 1u : 0u;
     hipError_t e = launch_closest(a, five ? c->grid_closest5 : c->grid_closest, count, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "closest-hit launch");
@@ -89,7 +88,8 @@ int trace_occluded_dev(kp_ctx* c, const float4* rays, uint32_t n, uint32_t* out)
     a.rays = rays;
     a.occl = out;
     a.n = n;
-    hipError_t e = launch_occluded(a, c->grid_occluded, c->stream);
+    a.five_waves = 1u;   // the 5-wave builds, as the integrator's shadow rays
+    hipError_t e = launch_occluded(a, c->grid_occluded5, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "occlusion launch");
     return KP_OK;
 }

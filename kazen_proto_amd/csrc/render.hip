@@ -1154,13 +1154,10 @@ static TraceCfg ctx_trace_cfg(kp_ctx* c) {
     TraceCfg t;
     std::memset(&t, 0, sizeof(t));
     t.base = trace_base(c);
-    t.grid_closest = c->grid_closest;
-    t.grid_occluded = c->grid_occluded;
-    if (!c->two_level) {
-        t.base.five_waves = 1;
-        t.grid_closest = c->grid_closest5;
-        t.grid_occluded = c->grid_occluded5;
-    }
+    // the 5-wave traversal builds, one-level or two-level
+    t.base.five_waves = 1;
+    t.grid_closest = c->grid_closest5;
+    t.grid_occluded = c->grid_occluded5;
     // as many blocks as stay resident: every wave strides over the queue in
     // one round, no second partial round of blocks
     t.shade_blocks = c->shade_blocks;
