@@ -376,7 +376,7 @@ def main():
     ap.add_argument("--skip-primary", action="store_true",
                     help="launch only the incoherent set (profiling runs: one kernel population)")
     ap.add_argument("--leaf", type=int, default=4)
-    ap.add_argument("--refill", type=int, default=48)
+    ap.add_argument("--refill", type=int, default=40)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy kernel of MIB MiB (reference kernel of "
                          "known traffic for HBM counter calibration)")

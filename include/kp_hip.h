@@ -244,7 +244,7 @@ int kp_bvh8_blob_closest(const kp_bvh8_blob* b, const kp_ray* rays, uint32_t n, 
 int kp_bvh8_blob_occluded(const kp_bvh8_blob* b, const kp_ray* rays, uint32_t n, uint32_t* out);
 
 /* Tuning: refill a wave once fewer than `lanes` (1..64) of its lanes are
- * busy (default 48). */
+ * busy (default 40). */
 int kp_set_trace_tuning(kp_ctx* ctx, uint32_t refill_below_lanes);
 /* Tuning: at most `max_paths` paths (2^16 .. 2^28) in flight per pass of
  * kp_render / kp_render_accumulate and per chunk of kp_integrate_rays

@@ -60,7 +60,7 @@ struct kp_ctx {
     uint32_t grid_closest = 0, grid_occluded = 0;
     uint32_t shade_blocks = 0, shade_blocks_guided = 0;   // resident blocks of k_shade, whole device
     uint32_t grid_closest5 = 0, grid_occluded5 = 0;       // the integrator's 5-wave traversal kernels
-    uint32_t refill_below = 48;   // measured best for incoherent rays (leaf 4)
+    uint32_t refill_below = 40;   // measured best for incoherent rays (leaf 4; 36..60 swept)
     uint32_t pass_paths = 1u << 26;   // paths per pass of the integrator (measured: larger is faster)
     uint32_t max_steps = 64;
 

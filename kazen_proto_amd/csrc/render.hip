@@ -1856,7 +1856,7 @@ static int batch_autotune(BatchState* b, const kp_render_desc* descs, uint32_t p
     if ((rc = try_values(&kp_batch_tuning::pass_paths, {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25}))) return rc;
     if ((rc = try_values(&kp_batch_tuning::trace_blocks_per_cu, {tb_max, std::max(1u, tb_max / 2), std::max(1u, tb_max - 1)})))
         return rc;
-    if ((rc = try_values(&kp_batch_tuning::refill_below, {16u, 32u, 48u, 56u}))) return rc;
+    if ((rc = try_values(&kp_batch_tuning::refill_below, {16u, 32u, 40u, 48u, 56u}))) return rc;
     if ((rc = try_values(&kp_batch_tuning::shade_blocks_per_cu, {4u, 8u, 16u}))) return rc;
     b->tune = best;
     if (out) *out = best;
